@@ -1,0 +1,569 @@
+// C ABI of the gfx950 K-SVD library (declarations and reference citations in
+// include/ksvd_hip.h).  Host orchestration only: buffer sizing, state checks,
+// launches on the context's stream, HIP-event timing of every phase.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ksvd_internal.hpp"
+
+namespace {
+
+enum SpanKind { SPAN_GATHER, SPAN_INIT, SPAN_OMP, SPAN_OMP_KERNEL, SPAN_UPDATE, SPAN_REBUILD, SPAN_KINDS };
+
+struct Span {
+    hipEvent_t a, b;
+    int kind;
+};
+
+// Per-context event bookkeeping, kept beside the context so that the public
+// struct stays plain.
+struct Timeline {
+    std::vector<hipEvent_t> pool;
+    std::vector<Span> pending;
+};
+
+Timeline &timeline(ksvd_ctx *ctx)
+{
+    if (!ctx->timeline) ctx->timeline = new Timeline();
+    return *(Timeline *)ctx->timeline;
+}
+
+void drop_timeline(ksvd_ctx *ctx)
+{
+    if (!ctx->timeline) return;
+    Timeline *tl = (Timeline *)ctx->timeline;
+    for (auto &s : tl->pending) {
+        hipEventDestroy(s.a);
+        hipEventDestroy(s.b);
+    }
+    for (auto e : tl->pool) hipEventDestroy(e);
+    delete tl;
+    ctx->timeline = nullptr;
+}
+
+int get_event(ksvd_ctx *ctx, hipEvent_t *e)
+{
+    Timeline &tl = timeline(ctx);
+    if (!tl.pool.empty()) {
+        *e = tl.pool.back();
+        tl.pool.pop_back();
+        return KSVD_OK;
+    }
+    KSVD_HIP_CHECK(ctx, hipEventCreate(e));
+    return KSVD_OK;
+}
+
+int span_begin(ksvd_ctx *ctx, Span *s, int kind)
+{
+    s->kind = kind;
+    int rc = get_event(ctx, &s->a);
+    if (rc) return rc;
+    rc = get_event(ctx, &s->b);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(ctx, hipEventRecord(s->a, ctx->stream));
+    return KSVD_OK;
+}
+
+int span_end(ksvd_ctx *ctx, Span *s)
+{
+    KSVD_HIP_CHECK(ctx, hipEventRecord(s->b, ctx->stream));
+    timeline(ctx).pending.push_back(*s);
+    return KSVD_OK;
+}
+
+// Wait for the stream and fold every finished span into ctx->tm.
+int resolve(ksvd_ctx *ctx)
+{
+    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    Timeline &tl = timeline(ctx);
+    for (auto &s : tl.pending) {
+        float ms = 0.f;
+        KSVD_HIP_CHECK(ctx, hipEventElapsedTime(&ms, s.a, s.b));
+        switch (s.kind) {
+        case SPAN_GATHER: ctx->tm.gather_ms += ms; break;
+        case SPAN_INIT: ctx->tm.init_ms += ms; break;
+        case SPAN_OMP: ctx->tm.omp_ms += ms; ctx->tm.omp_calls++; break;
+        case SPAN_OMP_KERNEL: ctx->tm.omp_kernel_ms += ms; break;
+        case SPAN_UPDATE: ctx->tm.update_ms += ms; ctx->tm.update_calls++; break;
+        case SPAN_REBUILD: ctx->tm.rebuild_ms += ms; break;
+        default: break;
+        }
+        tl.pool.push_back(s.a);
+        tl.pool.push_back(s.b);
+    }
+    tl.pending.clear();
+    return KSVD_OK;
+}
+
+int set_device(ksvd_ctx *ctx)
+{
+    KSVD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return KSVD_OK;
+}
+
+int check_params(ksvd_ctx *ctx, const ksvd_params *p)
+{
+    if (p->patch_w <= 0 || p->patch_h <= 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "patch dimensions must be positive");
+    if (p->slide_w <= 0 || p->slide_h <= 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "sliding steps must be positive");
+    if (p->atoms <= 0 || p->atoms > 1024) return ksvd_fail(ctx, KSVD_ERR_ARG, "atoms must be in [1, 1024]");
+    if (p->omp_iter <= 0 || p->omp_iter > KSVD_MAX_SUPPORT)
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "ompIter must be in [1, 16]");
+    if (p->ksvd_iter < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "ksvdIter must be >= 0");
+    if (p->user_cap < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "user_cap must be >= 0");
+    const int dim = p->patch_w * p->patch_h;
+    if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
+    // the rank-1 solve keeps the dim x dim Gram of E in LDS
+    if ((size_t)dim * dim * 4 + (size_t)(dim + 1024) * 4 > 160 * 1024)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 190 not supported by the rank-1 update");
+    if (p->precision != KSVD_PREC_FP32 && p->precision != KSVD_PREC_BF16)
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown precision");
+    if (p->precision == KSVD_PREC_BF16) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "BF16 form not built");
+    if (p->geometry != KSVD_GEOM_TRUE && p->geometry != KSVD_GEOM_COMPAT)
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown geometry");
+    return KSVD_OK;
+}
+
+int sync_copy_d2h(ksvd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    int rc = resolve(ctx);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return KSVD_OK;
+}
+
+}  // namespace
+
+int ksvd_fail(ksvd_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return KSVD_OK;
+    if (b.ptr) {
+        KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return ksvd_fail(ctx, KSVD_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                                  " bytes failed: " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return KSVD_OK;
+}
+
+extern "C" {
+
+void ksvd_default_params(ksvd_params *p)
+{
+    if (!p) return;
+    p->patch_w = 8;
+    p->patch_h = 8;
+    p->slide_w = 2;
+    p->slide_h = 2;
+    p->atoms = 256;
+    p->omp_iter = 5;
+    p->ksvd_iter = 10;
+    p->min_omp_iter_batch = 0;
+    p->user_cap = 1024;
+    p->precision = KSVD_PREC_FP32;
+    p->geometry = KSVD_GEOM_TRUE;
+    p->device = 0;
+}
+
+const char *ksvd_last_error(const ksvd_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
+{
+    if (!p || !out) return KSVD_ERR_ARG;
+    *out = nullptr;
+    ksvd_ctx *ctx = new ksvd_ctx();
+    int rc = check_params(ctx, p);
+    if (rc) {
+        // keep the context so the caller can read the message, then destroy it
+        *out = ctx;
+        return rc;
+    }
+    ctx->prm = *p;
+    ctx->device = p->device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        *out = ctx;
+        return ksvd_fail(ctx, KSVD_ERR_HIP, "no HIP device available");
+    }
+    if (p->device < 0 || p->device >= ndev) {
+        *out = ctx;
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "device ordinal out of range");
+    }
+    *out = ctx;
+    rc = set_device(ctx);
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    KSVD_HIP_CHECK(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    KSVD_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->dim = p->patch_w * p->patch_h;
+    ctx->K = p->atoms;
+    ctx->kmax = p->omp_iter;
+    rc = ksvd_reserve(ctx, ctx->counters, sizeof(unsigned long long) * CNT_SLOTS);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(ctx, hipMemsetAsync(ctx->counters.ptr, 0, ctx->counters.bytes, ctx->stream));
+    rc = ksvd_reserve(ctx, ctx->D, sizeof(float) * (size_t)ctx->dim * ctx->K);
+    if (rc) return rc;
+    rc = ksvd_reserve(ctx, ctx->Dt, sizeof(float) * (size_t)ctx->dim * ctx->K);
+    if (rc) return rc;
+    rc = ksvd_reserve(ctx, ctx->G, sizeof(float) * (size_t)ctx->K * ctx->K);
+    if (rc) return rc;
+    rc = ksvd_reserve(ctx, ctx->wbuf, sizeof(float) * (size_t)ctx->dim * ctx->dim);
+    if (rc) return rc;
+    rc = ksvd_reserve(ctx, ctx->rlam, 16);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return KSVD_OK;
+}
+
+int ksvd_destroy(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_OK;
+    if (ctx->stream) {
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+    }
+    drop_timeline(ctx);
+    DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
+                      &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
+                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf,    &ctx->rsum,    &ctx->rweight,
+                      &ctx->rout, &ctx->rpart,  &ctx->rlam};
+    for (DevBuf *b : bufs)
+        if (b->ptr) hipFree(b->ptr);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KSVD_OK;
+}
+
+static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMemcpyKind kind)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
+    if (!buf || m <= 0 || n <= 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "image buffer and sizes must be valid");
+    const ksvd_params &p = ctx->prm;
+    if (p.patch_w > m || p.patch_h > n)
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "patch larger than the image");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    rc = resolve(ctx);
+    if (rc) return rc;
+    ctx->m = m;
+    ctx->n = n;
+    ctx->Pr = (n - p.patch_h) / p.slide_h + 1;
+    ctx->Pc = (m - p.patch_w) / p.slide_w + 1;
+    ctx->P = (int64_t)ctx->Pr * ctx->Pc;
+    const int64_t P = ctx->P, N = (int64_t)m * n;
+    const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
+    const int64_t nb = (P + 255) / 256;
+    const int64_t ecols = p.user_cap > 0 ? p.user_cap : P;
+    struct {
+        DevBuf *b;
+        size_t bytes;
+    } plan[] = {
+        {&ctx->img, sizeof(float) * (size_t)N},
+        {&ctx->Y, sizeof(float) * (size_t)dim * P},
+        {&ctx->idx, sizeof(int) * (size_t)P * kmax},
+        {&ctx->coef, sizeof(float) * (size_t)P * kmax},
+        {&ctx->cnt, sizeof(int) * (size_t)P},
+        {&ctx->blockcnt, sizeof(int) * (size_t)nb * K},
+        {&ctx->ucount, sizeof(int) * (size_t)K},
+        {&ctx->uoff, sizeof(int) * (size_t)(K + 1)},
+        {&ctx->users, sizeof(int) * (size_t)P * kmax},
+        {&ctx->uslot, sizeof(int) * (size_t)P * kmax},
+        {&ctx->E, sizeof(float) * (size_t)dim * ecols},
+        {&ctx->rsum, sizeof(float) * (size_t)N},
+        {&ctx->rweight, sizeof(float) * (size_t)N},
+        {&ctx->rout, sizeof(float) * (size_t)N},
+        {&ctx->rpart, sizeof(double) * (size_t)((N + 255) / 256)},
+    };
+    for (auto &e : plan) {
+        rc = ksvd_reserve(ctx, *e.b, e.bytes);
+        if (rc) return rc;
+    }
+    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
+    ctx->has_image = true;
+    ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_set_image(ksvd_ctx *ctx, const float *buf, int m, int n)
+{
+    return set_image_common(ctx, buf, m, n, hipMemcpyHostToDevice);
+}
+
+int ksvd_set_image_device(ksvd_ctx *ctx, const float *dbuf, int m, int n)
+{
+    return set_image_common(ctx, dbuf, m, n, hipMemcpyDeviceToDevice);
+}
+
+int ksvd_extract_patches(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_image) return ksvd_fail(ctx, KSVD_ERR_STATE, "no image set");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_GATHER))) return rc;
+    if ((rc = launch_gather(ctx))) return rc;
+    if ((rc = span_end(ctx, &s))) return rc;
+    ctx->has_patches = true;
+    ctx->has_codes = ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_init_dictionary(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_patches) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches not extracted");
+    if (ctx->P / 2 + ctx->K > ctx->P)
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "fewer patches than P/2 + atoms: the dictionary cannot be initialised");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_INIT))) return rc;
+    if ((rc = launch_init_dict(ctx))) return rc;
+    if ((rc = span_end(ctx, &s))) return rc;
+    ctx->has_dict = true;
+    ctx->gram_valid = false;
+    return KSVD_OK;
+}
+
+int ksvd_omp(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_patches || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches and dictionary required");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Span outer, inner;
+    if ((rc = span_begin(ctx, &outer, SPAN_OMP))) return rc;
+    if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
+    if ((rc = span_begin(ctx, &inner, SPAN_OMP_KERNEL))) return rc;
+    if ((rc = launch_omp(ctx))) return rc;
+    if ((rc = span_end(ctx, &inner))) return rc;
+    if ((rc = span_end(ctx, &outer))) return rc;
+    ctx->has_codes = true;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_update_dictionary(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_codes || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "codes and dictionary required");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_UPDATE))) return rc;
+    if ((rc = launch_update(ctx))) return rc;
+    if ((rc = span_end(ctx, &s))) return rc;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_iterate(ksvd_ctx *ctx, int iters)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (iters < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "iters must be >= 0");
+    for (int i = 0; i < iters; i++) {
+        int rc = ksvd_omp(ctx);
+        if (rc) return rc;
+        rc = ksvd_update_dictionary(ctx);
+        if (rc) return rc;
+    }
+    return KSVD_OK;
+}
+
+int ksvd_run(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    int rc = ksvd_iterate(ctx, ctx->prm.ksvd_iter);
+    if (rc) return rc;
+    return ksvd_omp(ctx);
+}
+
+int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_codes || !ctx->has_dict || !ctx->has_image)
+        return ksvd_fail(ctx, KSVD_ERR_STATE, "image, dictionary and codes required");
+    if (!(sigma2 > 0.f) || !std::isfinite(sigma2))
+        return ksvd_fail(ctx, KSVD_ERR_ARG, "sigma2 must be a positive finite noise variance");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_REBUILD))) return rc;
+    if ((rc = launch_rebuild(ctx, sigma2))) return rc;
+    if ((rc = span_end(ctx, &s))) return rc;
+    ctx->has_rebuild = true;
+    if (out) return sync_copy_d2h(ctx, out, ctx->rout.ptr, sizeof(float) * (size_t)ctx->m * ctx->n);
+    return KSVD_OK;
+}
+
+int ksvd_get_rebuild_parts(ksvd_ctx *ctx, float *sum, float *weight, float *lambda, double *dsum)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_rebuild) return ksvd_fail(ctx, KSVD_ERR_STATE, "no rebuild yet");
+    const size_t N = (size_t)ctx->m * ctx->n;
+    int rc;
+    if (sum && (rc = sync_copy_d2h(ctx, sum, ctx->rsum.ptr, sizeof(float) * N))) return rc;
+    if (weight && (rc = sync_copy_d2h(ctx, weight, ctx->rweight.ptr, sizeof(float) * N))) return rc;
+    float lam[4];
+    if ((rc = sync_copy_d2h(ctx, lam, ctx->rlam.ptr, sizeof(lam)))) return rc;
+    if (lambda) *lambda = lam[0];
+    if (dsum) memcpy(dsum, &lam[2], sizeof(double));
+    return KSVD_OK;
+}
+
+int ksvd_rebuild_device_ptr(ksvd_ctx *ctx, const float **dptr)
+{
+    if (!ctx || !dptr) return KSVD_ERR_ARG;
+    if (!ctx->has_rebuild) return ksvd_fail(ctx, KSVD_ERR_STATE, "no rebuild yet");
+    *dptr = (const float *)ctx->rout.ptr;
+    return KSVD_OK;
+}
+
+int ksvd_get_dims(ksvd_ctx *ctx, int *dim, int64_t *patches, int *atoms, int *kmax)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (dim) *dim = ctx->dim;
+    if (patches) *patches = ctx->P;
+    if (atoms) *atoms = ctx->K;
+    if (kmax) *kmax = ctx->kmax;
+    return KSVD_OK;
+}
+
+int ksvd_get_patches(ksvd_ctx *ctx, float *Y)
+{
+    if (!ctx || !Y) return KSVD_ERR_ARG;
+    if (!ctx->has_patches) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches not extracted");
+    return sync_copy_d2h(ctx, Y, ctx->Y.ptr, sizeof(float) * (size_t)ctx->dim * ctx->P);
+}
+
+int ksvd_get_dictionary(ksvd_ctx *ctx, float *D)
+{
+    if (!ctx || !D) return KSVD_ERR_ARG;
+    if (!ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "no dictionary");
+    return sync_copy_d2h(ctx, D, ctx->D.ptr, sizeof(float) * (size_t)ctx->dim * ctx->K);
+}
+
+int ksvd_set_dictionary(ksvd_ctx *ctx, const float *D)
+{
+    if (!ctx || !D) return KSVD_ERR_ARG;
+    if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = resolve(ctx))) return rc;
+    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->D.ptr, D, sizeof(float) * (size_t)ctx->dim * ctx->K, hipMemcpyHostToDevice));
+    ctx->has_dict = true;
+    ctx->gram_valid = false;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_get_codes(ksvd_ctx *ctx, int32_t *idx, float *coef, int32_t *cnt)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "no codes");
+    const size_t pk = (size_t)ctx->P * ctx->kmax;
+    int rc;
+    if (idx && (rc = sync_copy_d2h(ctx, idx, ctx->idx.ptr, sizeof(int) * pk))) return rc;
+    if (coef && (rc = sync_copy_d2h(ctx, coef, ctx->coef.ptr, sizeof(float) * pk))) return rc;
+    if (cnt && (rc = sync_copy_d2h(ctx, cnt, ctx->cnt.ptr, sizeof(int) * (size_t)ctx->P))) return rc;
+    return KSVD_OK;
+}
+
+int ksvd_set_codes(ksvd_ctx *ctx, const int32_t *idx, const float *coef, const int32_t *cnt)
+{
+    if (!ctx || !idx || !coef || !cnt) return KSVD_ERR_ARG;
+    if (!ctx->has_image) return ksvd_fail(ctx, KSVD_ERR_STATE, "no image set (patch count unknown)");
+    const int64_t P = ctx->P;
+    const int kmax = ctx->kmax, K = ctx->K;
+    for (int64_t p = 0; p < P; p++) {
+        if (cnt[p] < 0 || cnt[p] > kmax) return ksvd_fail(ctx, KSVD_ERR_ARG, "code count out of range");
+        for (int i = 0; i < cnt[p]; i++) {
+            const int a = idx[p * kmax + i];
+            if (a < 0 || a >= K) return ksvd_fail(ctx, KSVD_ERR_ARG, "atom index out of range");
+            for (int q = 0; q < i; q++)
+                if (idx[p * kmax + q] == a) return ksvd_fail(ctx, KSVD_ERR_ARG, "repeated atom in a support");
+        }
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = resolve(ctx))) return rc;
+    const size_t pk = (size_t)P * kmax;
+    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->idx.ptr, idx, sizeof(int) * pk, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->coef.ptr, coef, sizeof(float) * pk, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->cnt.ptr, cnt, sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
+    ctx->has_codes = true;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_get_codes_dense(ksvd_ctx *ctx, float *X)
+{
+    if (!ctx || !X) return KSVD_ERR_ARG;
+    if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "no codes");
+    const int64_t P = ctx->P;
+    const int kmax = ctx->kmax, K = ctx->K;
+    std::vector<int> idx((size_t)P * kmax), cnt((size_t)P);
+    std::vector<float> coef((size_t)P * kmax);
+    int rc = ksvd_get_codes(ctx, idx.data(), coef.data(), cnt.data());
+    if (rc) return rc;
+    memset(X, 0, sizeof(float) * (size_t)K * P);
+    for (int64_t p = 0; p < P; p++)
+        for (int i = 0; i < cnt[p]; i++) X[(size_t)p * K + idx[(size_t)p * kmax + i]] = coef[(size_t)p * kmax + i];
+    return KSVD_OK;
+}
+
+int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
+{
+    if (!ctx || !t) return KSVD_ERR_ARG;
+    if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
+    int rc = resolve(ctx);
+    if (rc) return rc;
+    unsigned long long c[CNT_SLOTS];
+    KSVD_HIP_CHECK(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    ctx->tm.omp_dup_stops = (int64_t)c[CNT_DUP];
+    ctx->tm.omp_degenerate = (int64_t)c[CNT_DEGEN];
+    ctx->tm.eig_iters = (int64_t)c[CNT_EIG_IT];
+    *t = ctx->tm;
+    return KSVD_OK;
+}
+
+int ksvd_reset_timings(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    int rc = resolve(ctx);
+    if (rc) return rc;
+    ctx->tm = ksvd_timings{};
+    KSVD_HIP_CHECK(ctx, hipMemset(ctx->counters.ptr, 0, ctx->counters.bytes));
+    return KSVD_OK;
+}
+
+int ksvd_synchronize(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
+    return resolve(ctx);
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Patch gather, dictionary init and dictionary Gram kernels (gfx950).
+//
+// Gather: one workgroup per (patch row, run of TP patches along that row).
+// The ph image rows the run touches are staged in LDS with coalesced loads,
+// then the run's TP*dim outputs -- one contiguous range of Y -- are written
+// with coalesced stores.  Pure copies: bit-exact against the oracle.
+#include "ksvd_internal.hpp"
+
+namespace {
+
+__global__ __launch_bounds__(256) void gather_patches_kernel(const float *__restrict__ img, int m,
+                                                             int pw, int ph, int sw, int sh, int Pc,
+                                                             int TP, int TW, float *__restrict__ Y)
+{
+    extern __shared__ float tile[];  // ph x TW
+    const int pr = blockIdx.y;
+    const int pc0 = blockIdx.x * TP;
+    const int np = min(TP, Pc - pc0);
+    const int a0 = pr * sh, b0 = pc0 * sw;
+    const int tw = (np - 1) * sw + pw;  // columns this run touches
+    for (int q = threadIdx.x; q < ph * tw; q += blockDim.x) {
+        const int r = q / tw, c = q - r * tw;
+        tile[r * TW + c] = img[(size_t)(a0 + r) * m + b0 + c];
+    }
+    __syncthreads();
+    const int dim = pw * ph;
+    float *out = Y + ((size_t)pr * Pc + pc0) * dim;
+    for (int o = threadIdx.x; o < np * dim; o += blockDim.x) {
+        const int qp = o / dim, e = o - qp * dim;
+        const int r = e / pw, c = e - r * pw;
+        out[o] = tile[r * TW + qp * sw + c];
+    }
+}
+
+// One thread per atom.  The squared norm is accumulated the way the
+// reference's in-kernel thrust::transform_reduce(square, 0, plus<float>)
+// does it (cuda_svd_denoiser.cu:59): the int init value makes the running
+// sum an int, so each float partial sum is truncated toward zero.
+__global__ void init_dict_kernel(const float *__restrict__ Y, int dim, int64_t off, int K,
+                                 float *__restrict__ D)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const float *y = Y + (off + k) * (int64_t)dim;
+    int acc = 0;
+    for (int e = 0; e < dim; e++) {
+        const float sq = y[e] * y[e];
+        acc = (int)((float)acc + sq);
+    }
+    const float norm = sqrtf((float)acc);
+    for (int e = 0; e < dim; e++) D[(int64_t)k * dim + e] = y[e] / norm;
+}
+
+// G = D^T D (K x K) and Dt = D^T laid out dim x K (row e holds every atom's
+// element e).  One thread per Gram entry; element order, fused multiply-adds.
+__global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ G)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)K * K) return;
+    const int a = (int)(q / K), b = (int)(q - (int64_t)a * K);
+    const float *da = D + (int64_t)a * dim, *db = D + (int64_t)b * dim;
+    float s = 0.f;
+    for (int e = 0; e < dim; e++) s = fmaf(da[e], db[e], s);
+    G[q] = s;
+}
+
+__global__ void transpose_dict_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ Dt)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)K * dim) return;
+    const int e = (int)(q / K), a = (int)(q - (int64_t)e * K);
+    Dt[q] = D[(int64_t)a * dim + e];
+}
+
+}  // namespace
+
+int launch_gather(ksvd_ctx *ctx)
+{
+    const int pw = ctx->prm.patch_w, ph = ctx->prm.patch_h;
+    const int sw = ctx->prm.slide_w, sh = ctx->prm.slide_h;
+    // runs of TP patches, the staged tile kept under 32 KB
+    int TP = ((8192 / ph) - pw) / sw + 1;
+    TP = TP < 1 ? 1 : (TP > 64 ? 64 : TP);
+    const int TW = (TP - 1) * sw + pw;
+    const size_t lds = sizeof(float) * (size_t)ph * TW;
+    if (lds > 64 * 1024) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch too large for the gather tile");
+    dim3 grid((ctx->Pc + TP - 1) / TP, ctx->Pr);
+    hipLaunchKernelGGL(gather_patches_kernel, grid, dim3(256), lds, ctx->stream, (const float *)ctx->img.ptr,
+                       ctx->m, pw, ph, sw, sh, ctx->Pc, TP, TW, (float *)ctx->Y.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+int launch_init_dict(ksvd_ctx *ctx)
+{
+    hipLaunchKernelGGL(init_dict_kernel, dim3((ctx->K + 255) / 256), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->Y.ptr, ctx->dim, ctx->P / 2, ctx->K, (float *)ctx->D.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+int launch_dict_gram(ksvd_ctx *ctx)
+{
+    const int64_t kk = (int64_t)ctx->K * ctx->K;
+    hipLaunchKernelGGL(dict_gram_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->D.ptr, ctx->dim, ctx->K, (float *)ctx->G.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    const int64_t kd = (int64_t)ctx->K * ctx->dim;
+    hipLaunchKernelGGL(transpose_dict_kernel, dim3((unsigned)((kd + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->D.ptr, ctx->dim, ctx->K, (float *)ctx->Dt.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    ctx->gram_valid = true;
+    return KSVD_OK;
+}

@@ -1,0 +1,73 @@
+// Internal declarations of the gfx950 K-SVD library: the context behind the
+// opaque ksvd_ctx handle and the launchers of every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ksvd_hip.h"
+
+// Largest supported ompIter of the device path.
+constexpr int KSVD_MAX_SUPPORT = 16;
+// Largest patch dimension (pw * ph) of the device path.
+constexpr int KSVD_MAX_DIM = 1024;
+
+// Device counters, one slot each.
+enum { CNT_DUP = 0, CNT_DEGEN = 1, CNT_EIG_IT = 2, CNT_SLOTS = 8 };
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct ksvd_ctx {
+    ksvd_params prm{};
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+
+    // frame and patch grid: the buffer is n rows of length m
+    int m = 0, n = 0;
+    int dim = 0;
+    int Pr = 0, Pc = 0;
+    int64_t P = 0;
+    int K = 0, kmax = 0;
+
+    // state
+    bool has_image = false, has_patches = false, has_dict = false, has_codes = false;
+    bool has_rebuild = false;
+    bool gram_valid = false;
+
+    // device buffers
+    DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
+    DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
+    DevBuf rsum, rweight, rout, rpart, rlam;
+
+    ksvd_timings tm{};
+    std::string err;
+    void *timeline = nullptr;  // event bookkeeping owned by capi.hip
+};
+
+// Set ctx->err and return code.
+int ksvd_fail(ksvd_ctx *ctx, int code, const std::string &msg);
+// Grow a device buffer to at least `bytes` (contents not preserved).
+int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes);
+
+#define KSVD_HIP_CHECK(ctx, call)                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return ksvd_fail((ctx), KSVD_ERR_HIP,                                                    \
+                             std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+// ---- launchers (each enqueues on ctx->stream and returns KSVD_OK or fails) --
+int launch_gather(ksvd_ctx *ctx);
+int launch_init_dict(ksvd_ctx *ctx);
+int launch_dict_gram(ksvd_ctx *ctx);
+int launch_omp(ksvd_ctx *ctx);
+int launch_update(ksvd_ctx *ctx);
+int launch_rebuild(ksvd_ctx *ctx, float sigma2);

@@ -1,0 +1,433 @@
+// Batched Orthogonal Matching Pursuit on gfx950.
+//
+// omp_mfma_kernel: persistent, one workgroup per CU, four wavefronts.  The
+// whole dictionary (K x dim FP32, atom-major, row pitch LD = 4Q + 4 floats so
+// that the 16-byte LDS reads of one wavefront hit distinct banks) sits in LDS
+// for the kernel's lifetime.  Each wavefront owns a tile of 16 patches:
+//   * lane (col, g), col = lane % 16, g = lane / 16, holds quarter g of patch
+//     col: elements g*Q .. g*Q+Q-1 of y and of the residual r;
+//   * the correlation C (K x 16) = D^T R is v_mfma_f32_16x16x4_f32 with the
+//     reduction index permuted so that MFMA number kk takes element g*Q + kk
+//     from lane group g (A from LDS, B straight from r's registers);
+//   * |C| argmax per patch: in-lane scan in ascending atom order, then two
+//     cross-lane steps with the lowest index winning ties (cublasIsamax);
+//   * the support Gram grows by one Cholesky row per step (entries from the
+//     dictionary Gram G = D^T D), then two triangular solves give x; all four
+//     lanes of a patch carry identical copies of S, L, x;
+//   * r = y - D_S x and ||r|| end the step; ||r|| < 1e-3 retires the patch.
+// Semantics follow CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402).
+#include "ksvd_internal.hpp"
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int OMP_WAVES = 4;
+
+// Cross-lane sum over the four lanes that hold one patch (lanes col, col+16,
+// col+32, col+48).  Every lane ends with the same bits.
+__device__ __forceinline__ float patch_sum(float v)
+{
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+template <int Q, int KT, int KM>
+__global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
+    const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
+    const float *__restrict__ G, int kmax, int ntiles, int *__restrict__ idx_out,
+    float *__restrict__ coef_out, int *__restrict__ cnt_out, unsigned long long *__restrict__ counters)
+{
+    constexpr int LD = 4 * Q + 4;
+    constexpr int KP = 16 * KT;
+    extern __shared__ float Ds[];  // KP x LD, zero beyond (K, dim)
+    for (int q = threadIdx.x; q < KP * LD; q += blockDim.x) {
+        const int a = q / LD, e = q - a * LD;
+        Ds[q] = (a < K && e < dim) ? D[(int64_t)a * dim + e] : 0.f;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const int e0 = g * Q;
+    unsigned long long ndup = 0, ndeg = 0;
+
+    for (int tile = blockIdx.x * OMP_WAVES + wave; tile < ntiles; tile += gridDim.x * OMP_WAVES) {
+        const int64_t p = (int64_t)tile * 16 + col;
+        const bool valid = p < P;
+        float y[Q], r[Q];
+#pragma unroll
+        for (int kk = 0; kk < Q; kk++) {
+            const int e = e0 + kk;
+            y[kk] = (valid && e < dim) ? Y[p * dim + e] : 0.f;
+            r[kk] = y[kk];
+        }
+        int S[KM];
+        float x[KM], bv[KM], L[KM * KM];
+#pragma unroll
+        for (int i = 0; i < KM; i++) { S[i] = -1; x[i] = 0.f; bv[i] = 0.f; }
+#pragma unroll
+        for (int i = 0; i < KM * KM; i++) L[i] = 0.f;
+        int s = 0;
+        bool active = valid;
+
+        for (int t = 0; t < kmax; t++) {
+            if (__ballot(active) == 0) break;
+            // ---- correlation C = D^T R on MFMA --------------------------------
+            f4 acc[KT];
+#pragma unroll
+            for (int tt = 0; tt < KT; tt++) acc[tt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < Q; kb += 4) {
+                f4 av[KT];
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++)
+                    av[tt] = *(const f4 *)(Ds + (tt * 16 + col) * LD + e0 + kb);
+#pragma unroll
+                for (int sub = 0; sub < 4; sub++)
+#pragma unroll
+                    for (int tt = 0; tt < KT; tt++)
+                        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], r[kb + sub], acc[tt], 0, 0, 0);
+            }
+            // ---- first index of max |c| -------------------------------------
+            float best = -1.f;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int tt = 0; tt < KT; tt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = fabsf(acc[tt][i]);
+                    if (v > best) { best = v; bi = tt * 16 + 4 * g + i; }
+                }
+#pragma unroll
+            for (int off = 16; off <= 32; off <<= 1) {
+                const float ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            const int j = bi;
+            // ---- b_j = d_j . y ------------------------------------------------
+            float bp = 0.f;
+            {
+                const float *dj = Ds + (j < KP ? j : 0) * LD + e0;
+#pragma unroll
+                for (int kb = 0; kb < Q; kb += 4) {
+                    const f4 dv = *(const f4 *)(dj + kb);
+                    bp = fmaf(dv[0], y[kb + 0], bp);
+                    bp = fmaf(dv[1], y[kb + 1], bp);
+                    bp = fmaf(dv[2], y[kb + 2], bp);
+                    bp = fmaf(dv[3], y[kb + 3], bp);
+                }
+            }
+            bp = patch_sum(bp);
+            // ---- support bookkeeping and Cholesky row ------------------------
+            bool dup = false;
+#pragma unroll
+            for (int i = 0; i < KM; i++) dup |= (i < s) && (S[i] == j);
+            if (active && dup) {
+                active = false;
+                if (g == 0) ndup++;
+            }
+            // a non-finite residual leaves no atom with a finite |c|: retire
+            if (active && j >= K) {
+                active = false;
+                if (g == 0) ndeg++;
+            }
+            if (active) {
+                float w[KM];
+                float piv = G[(int64_t)j * K + j];
+#pragma unroll
+                for (int i = 0; i < KM; i++) {
+                    w[i] = 0.f;
+                    if (i < s) {
+                        float gi = G[(int64_t)S[i] * K + j];
+#pragma unroll
+                        for (int q = 0; q < i; q++) gi = fmaf(-L[i * KM + q], w[q], gi);
+                        w[i] = gi / L[i * KM + i];
+                        piv = fmaf(-w[i], w[i], piv);
+                    }
+                }
+                if (!(piv > 0.f)) {
+                    active = false;
+                    if (g == 0) ndeg++;
+                } else {
+                    const float lss = sqrtf(piv);
+#pragma unroll
+                    for (int i = 0; i < KM; i++)
+                        if (i == s) {
+#pragma unroll
+                            for (int q = 0; q < i; q++) L[i * KM + q] = w[q];
+                            L[i * KM + i] = lss;
+                            S[i] = j;
+                            bv[i] = bp;
+                        }
+                    s++;
+                    float z[KM];
+#pragma unroll
+                    for (int i = 0; i < KM; i++) {
+                        z[i] = 0.f;
+                        if (i < s) {
+                            float v = bv[i];
+#pragma unroll
+                            for (int q = 0; q < i; q++) v = fmaf(-L[i * KM + q], z[q], v);
+                            z[i] = v / L[i * KM + i];
+                        }
+                    }
+#pragma unroll
+                    for (int i = KM - 1; i >= 0; i--) {
+                        if (i < s) {
+                            float v = z[i];
+#pragma unroll
+                            for (int q = i + 1; q < KM; q++)
+                                if (q < s) v = fmaf(-L[q * KM + i], x[q], v);
+                            x[i] = v / L[i * KM + i];
+                        }
+                    }
+                }
+            }
+            // ---- residual and its norm ---------------------------------------
+            float nr = 0.f;
+            if (active) {
+#pragma unroll
+                for (int kb = 0; kb < Q; kb += 4) {
+                    f4 a4 = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int i = 0; i < KM; i++)
+                        if (i < s) {
+                            const f4 dv = *(const f4 *)(Ds + S[i] * LD + e0 + kb);
+                            a4[0] = fmaf(dv[0], x[i], a4[0]);
+                            a4[1] = fmaf(dv[1], x[i], a4[1]);
+                            a4[2] = fmaf(dv[2], x[i], a4[2]);
+                            a4[3] = fmaf(dv[3], x[i], a4[3]);
+                        }
+#pragma unroll
+                    for (int c4 = 0; c4 < 4; c4++) {
+                        const float rv = y[kb + c4] - a4[c4];
+                        r[kb + c4] = rv;
+                        nr = fmaf(rv, rv, nr);
+                    }
+                }
+            }
+            nr = patch_sum(nr);
+            if (active && sqrtf(nr) < 1e-3f) active = false;
+        }
+        if (valid && g == 0) {
+            cnt_out[p] = s;
+#pragma unroll
+            for (int i = 0; i < KM; i++)
+                if (i < kmax) {
+                    idx_out[p * kmax + i] = i < s ? S[i] : -1;
+                    coef_out[p * kmax + i] = i < s ? x[i] : 0.f;
+                }
+        }
+    }
+    if (ndup) atomicAdd(&counters[CNT_DUP], ndup);
+    if (ndeg) atomicAdd(&counters[CNT_DEGEN], ndeg);
+}
+
+// Fallback for shapes the MFMA kernel does not cover (dim > 144, K > 256 or
+// ompIter > 8): one wavefront per patch, the residual spread over the lanes
+// (element lane + 64 j), correlations from the transposed dictionary so that
+// lanes read consecutive atoms.  Same semantics, no MFMA.
+template <int RJ, int KM>
+__global__ __launch_bounds__(256) void omp_generic_kernel(
+    const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D,
+    const float *__restrict__ Dt, int K, const float *__restrict__ G, int kmax,
+    int *__restrict__ idx_out, float *__restrict__ coef_out, int *__restrict__ cnt_out,
+    unsigned long long *__restrict__ counters)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long ndup = 0, ndeg = 0;
+    for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < P; p += (int64_t)gridDim.x * 4) {
+        float y[RJ], r[RJ];
+#pragma unroll
+        for (int jj = 0; jj < RJ; jj++) {
+            const int e = lane + 64 * jj;
+            y[jj] = e < dim ? Y[p * dim + e] : 0.f;
+            r[jj] = y[jj];
+        }
+        int S[KM];
+        float x[KM], bv[KM], L[KM * KM];
+#pragma unroll
+        for (int i = 0; i < KM; i++) { S[i] = -1; x[i] = 0.f; bv[i] = 0.f; }
+#pragma unroll
+        for (int i = 0; i < KM * KM; i++) L[i] = 0.f;
+        int s = 0;
+        for (int t = 0; t < kmax; t++) {
+            float best = -1.f;
+            int bi = 0x7fffffff;
+            for (int a0 = 0; a0 < K; a0 += 64) {
+                const int a = a0 + lane;
+                float c = 0.f;
+#pragma unroll
+                for (int jj = 0; jj < RJ; jj++)
+                    for (int l = 0; l < 64; l++) {
+                        const int e = 64 * jj + l;
+                        if (e >= dim) break;
+                        const float re = __shfl(r[jj], l);
+                        if (a < K) c = fmaf(Dt[(int64_t)e * K + a], re, c);
+                    }
+                if (a < K) {
+                    const float v = fabsf(c);
+                    if (v > best) { best = v; bi = a; }
+                }
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const float ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            const int j = bi;
+            float bp = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < RJ; jj++) {
+                const int e = lane + 64 * jj;
+                if (e < dim && j < K) bp = fmaf(D[(int64_t)j * dim + e], y[jj], bp);
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) bp += __shfl_xor(bp, off);
+            bool dup = false;
+#pragma unroll
+            for (int i = 0; i < KM; i++) dup |= (i < s) && (S[i] == j);
+            if (dup) { if (lane == 0) ndup++; break; }
+            if (j >= K) { if (lane == 0) ndeg++; break; }
+            float w[KM];
+            float piv = G[(int64_t)j * K + j];
+#pragma unroll
+            for (int i = 0; i < KM; i++) {
+                w[i] = 0.f;
+                if (i < s) {
+                    float gi = G[(int64_t)S[i] * K + j];
+#pragma unroll
+                    for (int q = 0; q < i; q++) gi = fmaf(-L[i * KM + q], w[q], gi);
+                    w[i] = gi / L[i * KM + i];
+                    piv = fmaf(-w[i], w[i], piv);
+                }
+            }
+            if (!(piv > 0.f)) { if (lane == 0) ndeg++; break; }
+            const float lss = sqrtf(piv);
+#pragma unroll
+            for (int i = 0; i < KM; i++)
+                if (i == s) {
+#pragma unroll
+                    for (int q = 0; q < i; q++) L[i * KM + q] = w[q];
+                    L[i * KM + i] = lss;
+                    S[i] = j;
+                    bv[i] = bp;
+                }
+            s++;
+            float z[KM];
+#pragma unroll
+            for (int i = 0; i < KM; i++) {
+                z[i] = 0.f;
+                if (i < s) {
+                    float v = bv[i];
+#pragma unroll
+                    for (int q = 0; q < i; q++) v = fmaf(-L[i * KM + q], z[q], v);
+                    z[i] = v / L[i * KM + i];
+                }
+            }
+#pragma unroll
+            for (int i = KM - 1; i >= 0; i--)
+                if (i < s) {
+                    float v = z[i];
+#pragma unroll
+                    for (int q = i + 1; q < KM; q++)
+                        if (q < s) v = fmaf(-L[q * KM + i], x[q], v);
+                    x[i] = v / L[i * KM + i];
+                }
+            float nr = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < RJ; jj++) {
+                const int e = lane + 64 * jj;
+                float a = 0.f;
+#pragma unroll
+                for (int i = 0; i < KM; i++)
+                    if (i < s && e < dim) a = fmaf(D[(int64_t)S[i] * dim + e], x[i], a);
+                r[jj] = y[jj] - a;
+                nr = fmaf(r[jj], r[jj], nr);
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) nr += __shfl_xor(nr, off);
+            if (sqrtf(nr) < 1e-3f) break;
+        }
+        if (lane == 0) {
+            cnt_out[p] = s;
+            for (int i = 0; i < kmax; i++) {
+                int si = -1;
+                float xi = 0.f;
+#pragma unroll
+                for (int q = 0; q < KM; q++)
+                    if (q == i && q < s) { si = S[q]; xi = x[q]; }
+                idx_out[p * kmax + i] = si;
+                coef_out[p * kmax + i] = xi;
+            }
+        }
+    }
+    if (ndup) atomicAdd(&counters[CNT_DUP], ndup);
+    if (ndeg) atomicAdd(&counters[CNT_DEGEN], ndeg);
+}
+
+template <int Q>
+int launch_mfma(ksvd_ctx *ctx)
+{
+    constexpr int KT = 16, KM = 8;
+    constexpr int LD = 4 * Q + 4;
+    const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
+    auto kern = omp_mfma_kernel<Q, KT, KM>;
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int ntiles = (int)((ctx->P + 15) / 16);
+    int grid = (ntiles + OMP_WAVES - 1) / OMP_WAVES;
+    if (grid > ctx->num_cus) grid = ctx->num_cus;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * OMP_WAVES), lds, ctx->stream, (const float *)ctx->Y.ptr,
+                       ctx->dim, ctx->P, (const float *)ctx->D.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
+                       ntiles, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                       (unsigned long long *)ctx->counters.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+template <int RJ>
+int launch_generic(ksvd_ctx *ctx)
+{
+    const int64_t waves = ctx->P;
+    int64_t grid = (waves + 3) / 4;
+    if (grid > 8 * ctx->num_cus) grid = 8 * ctx->num_cus;
+    if (ctx->kmax <= 8) {
+        hipLaunchKernelGGL((omp_generic_kernel<RJ, 8>), dim3((unsigned)grid), dim3(256), 0, ctx->stream,
+                           (const float *)ctx->Y.ptr, ctx->dim, ctx->P, (const float *)ctx->D.ptr,
+                           (const float *)ctx->Dt.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
+                           (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                           (unsigned long long *)ctx->counters.ptr);
+    } else {
+        hipLaunchKernelGGL((omp_generic_kernel<RJ, KSVD_MAX_SUPPORT>), dim3((unsigned)grid), dim3(256), 0,
+                           ctx->stream, (const float *)ctx->Y.ptr, ctx->dim, ctx->P, (const float *)ctx->D.ptr,
+                           (const float *)ctx->Dt.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
+                           (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                           (unsigned long long *)ctx->counters.ptr);
+    }
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+}  // namespace
+
+// True when the MFMA pursuit kernel covers the context's shapes.
+static bool mfma_ok(const ksvd_ctx *ctx) { return ctx->dim <= 144 && ctx->K <= 256 && ctx->kmax <= 8; }
+
+int launch_omp(ksvd_ctx *ctx)
+{
+    if (!ctx->gram_valid) {
+        int rc = launch_dict_gram(ctx);
+        if (rc) return rc;
+    }
+    if (mfma_ok(ctx)) return ctx->dim <= 64 ? launch_mfma<16>(ctx) : launch_mfma<36>(ctx);
+    const int rj = (ctx->dim + 63) / 64;
+    if (rj <= 1) return launch_generic<1>(ctx);
+    if (rj <= 2) return launch_generic<2>(ctx);
+    if (rj <= 4) return launch_generic<4>(ctx);
+    if (rj <= 8) return launch_generic<8>(ctx);
+    return launch_generic<16>(ctx);
+}

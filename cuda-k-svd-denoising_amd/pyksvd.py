@@ -1,0 +1,346 @@
+"""ctypes binding of libksvd_hip.so (the C ABI declared in include/ksvd_hip.h).
+
+Used by the tests, __graft_entry__ and bench.py.  The GPU path lives entirely
+in the shared library; this module only marshals numpy arrays.  If the library
+has not been built, loading fails loudly (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libksvd_hip.so")
+
+KSVD_OK = 0
+KSVD_ERR_ARG = -1
+KSVD_ERR_STATE = -2
+KSVD_ERR_HIP = -3
+KSVD_ERR_ALLOC = -4
+KSVD_ERR_UNSUPPORTED = -5
+
+PREC_FP32 = 0
+PREC_BF16 = 1
+GEOM_TRUE = 0
+GEOM_COMPAT = 1
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("patch_w", ctypes.c_int),
+        ("patch_h", ctypes.c_int),
+        ("slide_w", ctypes.c_int),
+        ("slide_h", ctypes.c_int),
+        ("atoms", ctypes.c_int),
+        ("omp_iter", ctypes.c_int),
+        ("ksvd_iter", ctypes.c_int),
+        ("min_omp_iter_batch", ctypes.c_int),
+        ("user_cap", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("geometry", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class Timings(ctypes.Structure):
+    _fields_ = [
+        ("gather_ms", ctypes.c_double),
+        ("init_ms", ctypes.c_double),
+        ("omp_ms", ctypes.c_double),
+        ("omp_kernel_ms", ctypes.c_double),
+        ("omp_calls", ctypes.c_int64),
+        ("update_ms", ctypes.c_double),
+        ("update_calls", ctypes.c_int64),
+        ("rebuild_ms", ctypes.c_double),
+        ("omp_dup_stops", ctypes.c_int64),
+        ("omp_degenerate", ctypes.c_int64),
+        ("eig_iters", ctypes.c_int64),
+    ]
+
+
+# name -> (restype, argtypes); every function include/ksvd_hip.h declares.
+_P = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+_IP = ctypes.POINTER(ctypes.c_int32)
+SIGNATURES = {
+    "ksvd_default_params": (None, [ctypes.POINTER(Params)]),
+    "ksvd_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_P)]),
+    "ksvd_destroy": (ctypes.c_int, [_P]),
+    "ksvd_last_error": (ctypes.c_char_p, [_P]),
+    "ksvd_set_image": (ctypes.c_int, [_P, _FP, ctypes.c_int, ctypes.c_int]),
+    "ksvd_set_image_device": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int]),
+    "ksvd_extract_patches": (ctypes.c_int, [_P]),
+    "ksvd_init_dictionary": (ctypes.c_int, [_P]),
+    "ksvd_omp": (ctypes.c_int, [_P]),
+    "ksvd_update_dictionary": (ctypes.c_int, [_P]),
+    "ksvd_iterate": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ksvd_run": (ctypes.c_int, [_P]),
+    "ksvd_rebuild": (ctypes.c_int, [_P, ctypes.c_float, _FP]),
+    "ksvd_get_rebuild_parts": (ctypes.c_int, [_P, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double)]),
+    "ksvd_rebuild_device_ptr": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "ksvd_get_dims": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "ksvd_get_patches": (ctypes.c_int, [_P, _FP]),
+    "ksvd_get_dictionary": (ctypes.c_int, [_P, _FP]),
+    "ksvd_set_dictionary": (ctypes.c_int, [_P, _FP]),
+    "ksvd_get_codes": (ctypes.c_int, [_P, _IP, _FP, _IP]),
+    "ksvd_set_codes": (ctypes.c_int, [_P, _IP, _FP, _IP]),
+    "ksvd_get_codes_dense": (ctypes.c_int, [_P, _FP]),
+    "ksvd_get_timings": (ctypes.c_int, [_P, ctypes.POINTER(Timings)]),
+    "ksvd_reset_timings": (ctypes.c_int, [_P]),
+    "ksvd_synchronize": (ctypes.c_int, [_P]),
+    "ksvd_synth_image": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, _FP, _FP]),
+    "ksvd_noise_variance": (ctypes.c_double, [_FP, ctypes.c_int, ctypes.c_int]),
+    "ksvd_normalize": (None, [_FP, ctypes.c_int64, ctypes.c_float, ctypes.c_float]),
+    "ksvd_psnr": (ctypes.c_double, [_FP, _FP, ctypes.c_int64]),
+    "ksvd_image_to_frame": (None, [_FP, ctypes.c_int, ctypes.c_int, _FP]),
+    "ksvd_frame_to_image": (None, [_FP, ctypes.c_int, ctypes.c_int, _FP]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libksvd_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` "
+                               "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+class KsvdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ksvd error {code}: {msg}")
+        self.code = code
+
+
+def default_params():
+    p = Params()
+    lib().ksvd_default_params(ctypes.byref(p))
+    return p
+
+
+class KSvd:
+    """One context of the HIP path: image -> patches -> K-SVD -> rebuild."""
+
+    def __init__(self, patch=(8, 8), slide=(2, 2), atoms=256, omp_iter=5, ksvd_iter=10, user_cap=1024,
+                 geometry=GEOM_TRUE, precision=PREC_FP32, device=0):
+        L = lib()
+        p = default_params()
+        p.patch_w, p.patch_h = patch
+        p.slide_w, p.slide_h = slide
+        p.atoms, p.omp_iter, p.ksvd_iter = atoms, omp_iter, ksvd_iter
+        p.user_cap, p.geometry, p.precision, p.device = user_cap, geometry, precision, device
+        self.params = p
+        h = _P()
+        rc = L.ksvd_create(ctypes.byref(p), ctypes.byref(h))
+        self._h = h
+        if rc != KSVD_OK:
+            msg = L.ksvd_last_error(h).decode() if h else ""
+            L.ksvd_destroy(h)
+            self._h = None
+            raise KsvdError(rc, msg)
+        self.kmax = omp_iter
+        self.K = atoms
+        self.dim = patch[0] * patch[1]
+
+    # -- plumbing ------------------------------------------------------------
+    def _check(self, rc):
+        if rc != KSVD_OK:
+            raise KsvdError(rc, lib().ksvd_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().ksvd_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stages --------------------------------------------------------------
+    def set_frame(self, buf, m, n):
+        """buf: the reference's frame buffer, n rows of length m (m*n floats)."""
+        b = np.ascontiguousarray(buf, dtype=np.float32).ravel()
+        assert b.size == m * n
+        self.m, self.n = m, n
+        self._check(lib().ksvd_set_image(self._h, _fp(b), m, n))
+        d, P, K, km = self.dims()
+        self.P = P
+
+    def set_picture(self, img):
+        """img: (H, W) row-major picture; transposed into the frame as Denoiser::loadImage does."""
+        img = np.asarray(img, dtype=np.float32)
+        H, W = img.shape
+        self.set_frame(np.ascontiguousarray(img.T), W, H)
+
+    def set_frame_device(self, dptr, m, n):
+        self.m, self.n = m, n
+        self._check(lib().ksvd_set_image_device(self._h, _P(dptr), m, n))
+        self.P = self.dims()[1]
+
+    def extract(self):
+        self._check(lib().ksvd_extract_patches(self._h))
+
+    def init_dictionary(self):
+        self._check(lib().ksvd_init_dictionary(self._h))
+
+    def omp(self):
+        self._check(lib().ksvd_omp(self._h))
+
+    def update(self):
+        self._check(lib().ksvd_update_dictionary(self._h))
+
+    def iterate(self, iters):
+        self._check(lib().ksvd_iterate(self._h, iters))
+
+    def run(self):
+        self._check(lib().ksvd_run(self._h))
+
+    def rebuild(self, sigma2, copy=True):
+        out = np.empty(self.m * self.n, dtype=np.float32) if copy else None
+        self._check(lib().ksvd_rebuild(self._h, ctypes.c_float(sigma2), _fp(out) if copy else None))
+        return out.reshape(self.n, self.m) if copy else None
+
+    def rebuild_picture(self, sigma2):
+        """Denoised (H, W) picture: the frame output transposed back."""
+        fr = self.rebuild(sigma2)
+        # buf[x*H + y] = picture[y, x] with m = W, n = H
+        return np.ascontiguousarray(fr.ravel().reshape(self.m, self.n).T)
+
+    def rebuild_parts(self):
+        N = self.m * self.n
+        s = np.empty(N, np.float32)
+        w = np.empty(N, np.float32)
+        lam = ctypes.c_float()
+        ds = ctypes.c_double()
+        self._check(lib().ksvd_get_rebuild_parts(self._h, _fp(s), _fp(w), ctypes.byref(lam), ctypes.byref(ds)))
+        return s.reshape(self.n, self.m), w.reshape(self.n, self.m), lam.value, ds.value
+
+    # -- accessors -----------------------------------------------------------
+    def dims(self):
+        d, P, K, km = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().ksvd_get_dims(self._h, ctypes.byref(d), ctypes.byref(P), ctypes.byref(K),
+                                        ctypes.byref(km)))
+        return d.value, P.value, K.value, km.value
+
+    def patches(self):
+        Y = np.empty((self.P, self.dim), np.float32)
+        self._check(lib().ksvd_get_patches(self._h, _fp(Y)))
+        return Y
+
+    def dictionary(self):
+        """(K, dim): row k is atom k (the column-major dim x K matrix)."""
+        D = np.empty((self.K, self.dim), np.float32)
+        self._check(lib().ksvd_get_dictionary(self._h, _fp(D)))
+        return D
+
+    def set_dictionary(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float32)
+        assert D.shape == (self.K, self.dim)
+        self._check(lib().ksvd_set_dictionary(self._h, _fp(D)))
+
+    def codes(self):
+        idx = np.empty((self.P, self.kmax), np.int32)
+        coef = np.empty((self.P, self.kmax), np.float32)
+        cnt = np.empty(self.P, np.int32)
+        self._check(lib().ksvd_get_codes(self._h, _ip(idx), _fp(coef), _ip(cnt)))
+        return idx, coef, cnt
+
+    def set_codes(self, idx, coef, cnt):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        coef = np.ascontiguousarray(coef, dtype=np.float32)
+        cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+        self._check(lib().ksvd_set_codes(self._h, _ip(idx), _fp(coef), _ip(cnt)))
+
+    def codes_dense(self):
+        """(P, K): row p is column p of the reference's K x P sparse code."""
+        X = np.empty((self.P, self.K), np.float32)
+        self._check(lib().ksvd_get_codes_dense(self._h, _fp(X)))
+        return X
+
+    def timings(self):
+        t = Timings()
+        self._check(lib().ksvd_get_timings(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Timings._fields_}
+
+    def reset_timings(self):
+        self._check(lib().ksvd_reset_timings(self._h))
+
+    def synchronize(self):
+        self._check(lib().ksvd_synchronize(self._h))
+
+
+# ---- host helpers ------------------------------------------------------------
+def synth_image(seed, W, H, sigma):
+    """(clean, noisy), each (H, W) float32."""
+    clean = np.empty((H, W), np.float32)
+    noisy = np.empty((H, W), np.float32)
+    rc = lib().ksvd_synth_image(ctypes.c_uint64(seed), W, H, ctypes.c_float(sigma), _fp(clean), _fp(noisy))
+    if rc != KSVD_OK:
+        raise KsvdError(rc, "synth_image")
+    return clean, noisy
+
+
+def noise_variance(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    return lib().ksvd_noise_variance(_fp(img), W, H)
+
+
+def normalize(img, lo=0.0, hi=255.0):
+    out = np.array(img, dtype=np.float32, copy=True, order="C")
+    lib().ksvd_normalize(_fp(out), out.size, ctypes.c_float(lo), ctypes.c_float(hi))
+    return out
+
+
+def psnr(ref, img):
+    a = np.ascontiguousarray(ref, dtype=np.float32)
+    b = np.ascontiguousarray(img, dtype=np.float32)
+    assert a.size == b.size
+    return lib().ksvd_psnr(_fp(a), _fp(b), a.size)
+
+
+def image_to_frame(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    H, W = img.shape
+    buf = np.empty(W * H, np.float32)
+    lib().ksvd_image_to_frame(_fp(img), W, H, _fp(buf))
+    return buf
+
+
+def frame_to_image(buf, W, H):
+    buf = np.ascontiguousarray(buf, dtype=np.float32).ravel()
+    img = np.empty((H, W), np.float32)
+    lib().ksvd_frame_to_image(_fp(buf), W, H, _fp(img))
+    return img
+
+
+def patch_count(m, n, pw, ph, sw, sh):
+    if pw > m or ph > n:
+        return 0
+    return ((n - ph) // sh + 1) * ((m - pw) // sw + 1)

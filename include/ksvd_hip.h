@@ -1,0 +1,199 @@
+/*
+ * ksvd_hip.h -- C ABI of the MI355X (gfx950) K-SVD denoising hot path.
+ *
+ * One opaque context per host thread; every call returns an int status
+ * (KSVD_OK = 0, negative on error) and keeps a per-context error string.
+ * No exceptions cross this boundary; no torch or HIP types appear in it.
+ * Host buffers passed in are caller-owned and copied; device memory is owned
+ * by the context and freed by ksvd_destroy.
+ *
+ * Frame convention.  The image buffer is the reference's inputMatrix host
+ * buffer after Denoiser::loadImage (denoiser.cpp:37-62): the CImg image
+ * transposed, read as n rows of length m (m = image width W, n = image height
+ * H).  Patches are cut in that frame exactly as CudaKSvdDenoiser::
+ * createPatches walks it (cuda_svd_denoiser.cu:231-240), and ksvd_rebuild
+ * returns the denoised buffer in the same frame (before the final transpose
+ * of cuda_svd_denoiser.cu:489-490).
+ *
+ * The functions below replace, one for one, the stages of the reference's
+ * CUDA_K_GESVDJ pipeline; the reference interface each one stands in for is
+ * cited beside it (paths relative to the reference repository).
+ */
+#ifndef KSVD_HIP_H
+#define KSVD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    KSVD_OK = 0,
+    KSVD_ERR_ARG = -1,      /* invalid argument or parameter */
+    KSVD_ERR_STATE = -2,    /* call out of order (e.g. omp before a dictionary) */
+    KSVD_ERR_HIP = -3,      /* HIP runtime error (message in ksvd_last_error) */
+    KSVD_ERR_ALLOC = -4,    /* device allocation failed */
+    KSVD_ERR_UNSUPPORTED = -5
+};
+
+/* ---- numeric forms ------------------------------------------------------ */
+enum {
+    KSVD_PREC_FP32 = 0,     /* plain FP32: the drop-in for CUDA_K_GESVDJ */
+    KSVD_PREC_BF16 = 1      /* BF16 correlation on MFMA, FP32 re-score/LS/SVD */
+};
+enum {
+    KSVD_GEOM_TRUE = 0,     /* rebuild walks the gather's patch frame */
+    KSVD_GEOM_COMPAT = 1    /* rebuild walks the reference's swapped frame */
+};
+
+/* Parameters: the per-file keys of config.json (batch_denoiser.cpp:80-94,
+ * Denoiser fields denoisingLib.h:44-56) plus the backend's own knobs. */
+typedef struct ksvd_params {
+    int patch_w;            /* patchWidthDim  (default 8)  */
+    int patch_h;            /* patchHeightDim (default 8)  */
+    int slide_w;            /* slidingWidth   (default 2)  */
+    int slide_h;            /* slidingHeight  (default 2)  */
+    int atoms;              /* atoms          (default 256) */
+    int omp_iter;           /* ompIter: support-size limit (default 5) */
+    int ksvd_iter;          /* ksvdIter       (default 10) */
+    int min_omp_iter_batch; /* minOmpIterBatch: accepted and ignored (HBM holds
+                               every patch; cublas_matrix_omp.cu:130-165) */
+    int user_cap;           /* users per atom that enter the rank-1 update:
+                               1024 reproduces the reference's <<<1,1024>>>
+                               launches (cuda_svd_denoiser.cu:295-325); 0 = all */
+    int precision;          /* KSVD_PREC_* */
+    int geometry;           /* KSVD_GEOM_* */
+    int device;             /* HIP device ordinal */
+} ksvd_params;
+
+/* Per-phase device time (HIP events on the context's stream) and counters. */
+typedef struct ksvd_timings {
+    double gather_ms;       /* patch gather kernel */
+    double init_ms;         /* dictionary init kernel */
+    double omp_ms;          /* sum over OMP kernel launches */
+    double omp_kernel_ms;   /* the OMP pursuit kernel alone (roofline) */
+    int64_t omp_calls;
+    double update_ms;       /* sum over dictionary-update sweeps */
+    int64_t update_calls;
+    double rebuild_ms;      /* overlap-add + lambda + blend */
+    int64_t omp_dup_stops;  /* pursuits ended by a repeated atom pick */
+    int64_t omp_degenerate; /* pursuits ended by a non-positive pivot */
+    int64_t eig_iters;      /* power iterations spent in rank-1 updates */
+} ksvd_timings;
+
+typedef struct ksvd_ctx ksvd_ctx;
+
+/* Fill p with the reference defaults (denoisingLib.h:44-53) + user_cap 1024. */
+void ksvd_default_params(ksvd_params *p);
+
+/* Create a context on p->device.  Replaces the backend construction done by
+ * Denoiser::factory (denoiser.cpp:108-127) and CuBlasMatrixOps::init /
+ * SvdCudaEngine::init (cublas_matrix_ops.cu:22-34, svd_cuda_engine.cu:17-44). */
+int ksvd_create(const ksvd_params *p, ksvd_ctx **out);
+
+/* Free every device buffer and the stream.  Replaces CuBlasMatrixOps::
+ * finalize + SvdCudaEngine::finalize (cuda_svd_denoiser.cu:209-210). */
+int ksvd_destroy(ksvd_ctx *ctx);
+
+/* Last error message of ctx ("" if none); static text when ctx is NULL. */
+const char *ksvd_last_error(const ksvd_ctx *ctx);
+
+/* Upload the image buffer (host memory, m*n floats, n rows of length m).
+ * Replaces Matrix::copyOnDevice of the noisy patches' source
+ * (cuda_svd_denoiser.cu:246-247, matrix.cu:124-131). */
+int ksvd_set_image(ksvd_ctx *ctx, const float *buf, int m, int n);
+
+/* Same, from device memory already resident in HBM (device-to-device copy). */
+int ksvd_set_image_device(ksvd_ctx *ctx, const float *dbuf, int m, int n);
+
+/* Gather all patches into Y (dim x P, patch-contiguous) on the device.
+ * Replaces CudaKSvdDenoiser::createPatches (cuda_svd_denoiser.cu:221-256). */
+int ksvd_extract_patches(ksvd_ctx *ctx);
+
+/* D = Y[:, P/2 .. P/2+K-1], each column divided by its norm.  Replaces
+ * initDictionary + normalizeDictKernel (cuda_svd_denoiser.cu:52-63,261-280). */
+int ksvd_init_dictionary(ksvd_ctx *ctx);
+
+/* One batched OMP over every patch with the current dictionary.  Replaces
+ * CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402) and the pinv built by
+ * CuSolverGeSvdABatch (cusolver_gesvda_batch_engine.cu:117-143). */
+int ksvd_omp(ksvd_ctx *ctx);
+
+/* One Gauss-Seidel sweep of rank-1 atom updates over the current codes.
+ * Replaces CudaKSvdDenoiser::updateDictionary (cuda_svd_denoiser.cu:285-362)
+ * with its CuBlasMatrixMult (cublas_matrix_mult.cu:41-75) and CuSolverGeSvdJ
+ * (cusolver_gesvdj_engine.cu:46-69) calls. */
+int ksvd_update_dictionary(ksvd_ctx *ctx);
+
+/* iters x (ksvd_omp, ksvd_update_dictionary): the body of the loop of
+ * CudaKSvdDenoiser::kSvd (cuda_svd_denoiser.cu:374-397). */
+int ksvd_iterate(ksvd_ctx *ctx, int iters);
+
+/* The whole of CudaKSvdDenoiser::kSvd (cuda_svd_denoiser.cu:368-408):
+ * ksvd_iter iterations, then the final OMP. */
+int ksvd_run(ksvd_ctx *ctx);
+
+/* Overlap-add of D X, lambda from sigma2 (the noise variance that
+ * Denoiser::loadImage estimates, denoiser.cpp:56), blend, and copy of the
+ * m*n denoised buffer to host memory `out` (NULL: keep it on the device).
+ * Replaces CudaKSvdDenoiser::createImage (cuda_svd_denoiser.cu:413-498) up to
+ * its final transpose.  Pixels no patch covers keep their input value. */
+int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out);
+
+/* After ksvd_rebuild: the overlap-add sums and weights (m*n each, host) and
+ * lambda; any pointer may be NULL. */
+int ksvd_get_rebuild_parts(ksvd_ctx *ctx, float *sum, float *weight, float *lambda, double *dsum);
+
+/* Device pointer of the last rebuild's output (m*n floats, frame layout). */
+int ksvd_rebuild_device_ptr(ksvd_ctx *ctx, const float **dptr);
+
+/* Shapes: patch dimension, patch count, atoms, support limit. */
+int ksvd_get_dims(ksvd_ctx *ctx, int *dim, int64_t *patches, int *atoms, int *kmax);
+
+/* Copies between host memory and the context's matrices (the getters and
+ * setters the reference reaches through Matrix::copyOnHost/copyOnDevice,
+ * matrix.cu:124-142). */
+int ksvd_get_patches(ksvd_ctx *ctx, float *Y);              /* dim x P */
+int ksvd_get_dictionary(ksvd_ctx *ctx, float *D);           /* dim x K, column-major */
+int ksvd_set_dictionary(ksvd_ctx *ctx, const float *D);
+/* Compact codes: for patch p, cnt[p] <= kmax pairs (idx, coef) at p*kmax + i,
+ * in pursuit order; dense X[idx, p] = coef. */
+int ksvd_get_codes(ksvd_ctx *ctx, int32_t *idx, float *coef, int32_t *cnt);
+int ksvd_set_codes(ksvd_ctx *ctx, const int32_t *idx, const float *coef, const int32_t *cnt);
+/* Dense K x P column-major X, the layout of the reference's sparseCode. */
+int ksvd_get_codes_dense(ksvd_ctx *ctx, float *X);
+
+int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t);
+int ksvd_reset_timings(ksvd_ctx *ctx);
+int ksvd_synchronize(ksvd_ctx *ctx);
+
+/* ---- host-side helpers (no GPU) ---------------------------------------- */
+
+/* Seeded synthetic test image, row-major img[y*W + x]: six oriented
+ * sinusoidal gratings (period 4-64 px), twenty constant rectangles, affine
+ * rescale to [0,255] (clean), plus i.i.d. N(0, sigma^2) noise (noisy,
+ * unclipped).  Either output may be NULL. */
+int ksvd_synth_image(uint64_t seed, int W, int H, float sigma, float *clean, float *noisy);
+
+/* CImg<float>::variance_noise() with its default method (scaled Laplacian,
+ * then the MAD estimator), the estimate Denoiser::loadImage takes
+ * (denoiser.cpp:56).  img is row-major W x H. */
+double ksvd_noise_variance(const float *img, int W, int H);
+
+/* CImg<float>::normalize(lo, hi) in place (denoiser.cpp:75). */
+void ksvd_normalize(float *img, int64_t count, float lo, float hi);
+
+/* CImg PSNR with the default max 255 (denoiser.cpp:94-96). */
+double ksvd_psnr(const float *ref, const float *img, int64_t count);
+
+/* Reference frame <-> image: buf[x*H + y] = img[y*W + x] and back. */
+void ksvd_image_to_frame(const float *img, int W, int H, float *buf);
+void ksvd_frame_to_image(const float *buf, int W, int H, float *img);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KSVD_HIP_H */

@@ -1,0 +1,254 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars (from the plain-FP32 contract):
+  * gather, dictionary init, rebuild sums/weights: bit-exact;
+  * OMP: identical support sets on >= 99.9 % of patches, coefficients within
+    1e-4 relative (to the patch's coefficient norm) on those patches;
+  * dictionary update: sign-fixed atoms and coefficients within 1e-4
+    relative for atoms whose spectral gap lambda2/lambda1 < 0.9;
+  * end to end: PSNR within 0.02 dB of the FP32 oracle.
+Every test prints its error against the FP64 oracle as well.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = dict(W=128, H=128, sigma=20.0, patch=(8, 8), slide=(2, 2), atoms=256, iters=5, k=5, seed=1)
+CFG2 = dict(W=512, H=512, sigma=25.0, patch=(12, 12), slide=(3, 3), atoms=256, iters=10, k=5, seed=2)
+
+
+def make(kl, cfg, **kw):
+    clean, noisy = kl.synth_image(cfg["seed"], cfg["W"], cfg["H"], cfg["sigma"])
+    ks = kl.KSvd(patch=cfg["patch"], slide=cfg["slide"], atoms=cfg["atoms"], omp_iter=cfg["k"],
+                 ksvd_iter=cfg["iters"], **kw)
+    ks.set_picture(noisy)
+    return clean, noisy, ks
+
+
+def frame_of(kl, noisy):
+    return kl.image_to_frame(noisy)
+
+
+def support_agreement(idx_a, cnt_a, idx_b, cnt_b):
+    same = np.zeros(len(cnt_a), bool)
+    for p in range(len(cnt_a)):
+        same[p] = cnt_a[p] == cnt_b[p] and set(idx_a[p, :cnt_a[p]]) == set(idx_b[p, :cnt_b[p]])
+    return same
+
+
+def coef_rel_err(idx_a, coef_a, idx_b, coef_b, cnt, sel):
+    errs = []
+    for p in np.nonzero(sel)[0]:
+        c = cnt[p]
+        a = dict(zip(idx_a[p, :c], coef_a[p, :c]))
+        b = dict(zip(idx_b[p, :c], coef_b[p, :c]))
+        va = np.array([a[j] for j in sorted(a)], np.float64)
+        vb = np.array([b[j] for j in sorted(a)], np.float64)
+        errs.append(np.max(np.abs(va - vb)) / max(np.linalg.norm(vb), 1e-30))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("cfg", [CFG1, CFG2], ids=["128sq_8x8", "512sq_12x12"])
+def test_gather_and_init_bitexact(kl, orc, cfg):
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    W, H = cfg["W"], cfg["H"]
+    (pw, ph), (sw, sh) = cfg["patch"], cfg["slide"]
+    Yo = orc.extract(frame_of(kl, noisy), W, H, pw, ph, sw, sh)
+    Yg = ks.patches()
+    np.testing.assert_array_equal(Yg, Yo)
+    Do = orc.init_dict(Yo, cfg["atoms"])
+    np.testing.assert_array_equal(ks.dictionary(), Do)
+    ks.close()
+
+
+def test_gather_rectangular(kl, orc):
+    rng = np.random.default_rng(11)
+    m, n, pw, ph, sw, sh = 37, 23, 5, 7, 3, 2
+    frame = rng.uniform(0, 255, (n, m)).astype(np.float32)
+    ks = kl.KSvd(patch=(pw, ph), slide=(sw, sh), atoms=16, omp_iter=3)
+    ks.set_frame(frame, m, n)
+    ks.extract()
+    np.testing.assert_array_equal(ks.patches(), orc.extract(frame, m, n, pw, ph, sw, sh))
+    ks.close()
+
+
+@pytest.mark.parametrize("cfg", [CFG1, CFG2], ids=["128sq_8x8", "512sq_12x12"])
+def test_omp_parity(kl, orc, cfg):
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    Y = ks.patches()
+    D = ks.dictionary()
+    io, co, no, st = orc.omp(Y, D, cfg["k"], real="f32")
+    i64, c64, n64, _ = orc.omp(Y, D.astype(np.float64), cfg["k"], real="f64")
+    same = support_agreement(idx, cnt, io, no)
+    frac = same.mean()
+    err = coef_rel_err(idx, coef, io, co, cnt, same)
+    same64 = support_agreement(idx, cnt, i64, n64)
+    err64 = coef_rel_err(idx, coef, i64, c64, cnt, same & same64)
+    print(f"\nOMP {cfg['W']}^2: supports equal {frac * 100:.4f}% of {len(cnt)}; coef rel err max "
+          f"{err.max():.2e} (vs FP64 oracle: supports {same64.mean() * 100:.4f}%, coef {err64.max():.2e})")
+    assert frac >= 0.999
+    assert err.max() <= 1e-4
+    t = ks.timings()
+    assert t["omp_dup_stops"] == st[0] or abs(t["omp_dup_stops"] - st[0]) <= 0.001 * len(cnt)
+    ks.close()
+
+
+def test_omp_generic_kernel_parity(kl, orc):
+    # ompIter 10 is beyond the MFMA kernel's support limit -> generic kernel
+    cfg = dict(CFG1, k=10, atoms=300, W=96, H=96, seed=5)
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    io, co, no, _ = orc.omp(ks.patches(), ks.dictionary(), 10, real="f32")
+    same = support_agreement(idx, cnt, io, no)
+    err = coef_rel_err(idx, coef, io, co, cnt, same)
+    print(f"\ngeneric OMP: supports equal {same.mean() * 100:.3f}%, coef rel err {err.max():.2e}")
+    assert same.mean() >= 0.999 and err.max() <= 1e-4
+    ks.close()
+
+
+def signfix(D):
+    s = np.sign(D[np.arange(D.shape[0]), np.argmax(np.abs(D), axis=1)])
+    s[s == 0] = 1
+    return D * s[:, None], s
+
+
+@pytest.mark.parametrize("cfg,cap", [(CFG1, 1024), (CFG2, 1024), (CFG1, 0)], ids=["128sq_cap", "512sq_cap",
+                                                                                 "128sq_nocap"])
+def test_update_parity(kl, orc, cfg, cap):
+    clean, noisy, ks = make(kl, cfg, user_cap=cap)
+    ks.extract()
+    ks.init_dictionary()
+    Y = ks.patches()
+    D0 = ks.dictionary()
+    io, co, no, _ = orc.omp(Y, D0, cfg["k"], real="f32")
+    ks.set_codes(io, co, no)
+    ks.update()
+    Dg = ks.dictionary()
+    _, cg, _ = ks.codes()
+    Do, coo, gap, _ = orc.update(Y, D0, io, co, no, cap=cap, real="f32", want_gap=True)
+    D64, c64, _, _ = orc.update(Y, D0.astype(np.float64), io, co, no, cap=cap, real="f64")
+    good = (gap >= 0) & (gap < 0.9)
+    Dgs, sg = signfix(Dg)
+    Dos, so = signfix(Do)
+    D64s, s64 = signfix(D64)
+    atom_err = np.max(np.abs(Dgs - Dos), axis=1)
+    atom_err64 = np.max(np.abs(Dgs - D64s), axis=1)
+    # coefficients, sign-fixed with their atom's sign, relative to the atom's coefficient scale
+    P, kmax = io.shape
+    cerr = np.zeros(cfg["atoms"])
+    for k in np.nonzero(good)[0]:
+        sel = (io == k) & (np.arange(kmax)[None, :] < no[:, None])
+        a = cg[sel] * sg[k]
+        b = coo[sel] * so[k]
+        cerr[k] = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
+    print(f"\nupdate {cfg['W']}^2 cap={cap}: {good.sum()} atoms with gap<0.9; atom err max {atom_err[good].max():.2e}"
+          f" (FP64 {atom_err64[good].max():.2e}); coef rel err max {cerr[good].max():.2e}; worst-gap atoms "
+          f"err {atom_err[~good].max() if (~good).any() else 0:.2e}")
+    assert good.sum() >= 0.9 * (gap >= 0).sum()
+    assert atom_err[good].max() <= 1e-4
+    assert cerr[good].max() <= 1e-4
+    ks.close()
+
+
+def test_rebuild_bitexact_sums(kl, orc):
+    cfg = CFG2
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    D = ks.dictionary()
+    s2 = kl.noise_variance(noisy)
+    out = ks.rebuild(s2)
+    sg, wg, lamg, dsg = ks.rebuild_parts()
+    W, H = cfg["W"], cfg["H"]
+    (pw, ph), (sw, sh) = cfg["patch"], cfg["slide"]
+    oo, so, wo, dso, lamo = orc.rebuild(idx, coef, cnt, D, frame_of(kl, noisy), W, H, pw, ph, sw, sh, s2)
+    np.testing.assert_array_equal(wg, wo)
+    np.testing.assert_array_equal(sg, so)
+    assert abs(dsg - dso) <= 1e-9 * dso
+    print(f"\nrebuild: lambda gpu {lamg!r} oracle {lamo!r}; uncovered px {(wo == 0).sum()}")
+    assert np.float32(lamg) == np.float32(lamo)
+    np.testing.assert_array_equal(out, oo)
+    # uncovered pixels (512 - 12 is not a multiple of 3) keep the noisy value
+    fr = frame_of(kl, noisy).reshape(H, W)
+    assert (wo == 0).sum() == 2044
+    np.testing.assert_array_equal(out[wo == 0], fr[wo == 0])
+    ks.close()
+
+
+def run_gpu(kl, cfg):
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    ks.run()
+    s2 = kl.noise_variance(noisy)
+    img = ks.rebuild_picture(s2)
+    D = ks.dictionary()
+    codes = ks.codes()
+    t = ks.timings()
+    ks.close()
+    return clean, noisy, img, D, codes, t
+
+
+def run_oracle(kl, orc, cfg, real):
+    clean, noisy = kl.synth_image(cfg["seed"], cfg["W"], cfg["H"], cfg["sigma"])
+    W, H = cfg["W"], cfg["H"]
+    (pw, ph), (sw, sh) = cfg["patch"], cfg["slide"]
+    fr = kl.image_to_frame(noisy)
+    Y = orc.extract(fr, W, H, pw, ph, sw, sh, real=real)
+    D0 = orc.init_dict(Y, cfg["atoms"], real=real)
+    D, idx, coef, cnt, _ = orc.ksvd(Y, D0, cfg["k"], cfg["iters"], real=real)
+    out, *_ = orc.rebuild(idx, coef, cnt, D, fr, W, H, pw, ph, sw, sh, kl.noise_variance(noisy), real=real)
+    return kl.frame_to_image(out.astype(np.float32), W, H)
+
+
+def test_end_to_end_psnr_128(kl, orc):
+    cfg = CFG1
+    clean, noisy, img, D, codes, t = run_gpu(kl, cfg)
+    o32 = run_oracle(kl, orc, cfg, "f32")
+    o64 = run_oracle(kl, orc, cfg, "f64")
+    pg, p32, p64 = (kl.psnr(clean, kl.normalize(x)) for x in (img, o32, o64))
+    rg, r32, r64 = (kl.psnr(clean, x) for x in (img, o32, o64))
+    print(f"\nPSNR (normalized, as the reference reports) gpu {pg:.4f} oracle32 {p32:.4f} oracle64 {p64:.4f};"
+          f" raw gpu {rg:.4f} / {r32:.4f} / {r64:.4f}; noisy {kl.psnr(clean, noisy):.4f}")
+    assert abs(pg - p32) <= 0.02
+    assert abs(rg - r32) <= 0.02
+    assert rg > kl.psnr(clean, noisy) + 3
+
+
+def test_deterministic_twice(kl):
+    cfg = dict(CFG1, iters=2)
+    a = run_gpu(kl, cfg)
+    b = run_gpu(kl, cfg)
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[3], b[3])
+    for x, y in zip(a[4], b[4]):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_state_and_argument_errors(kl):
+    ks = kl.KSvd(patch=(8, 8), slide=(2, 2), atoms=256)
+    with pytest.raises(kl.KsvdError) as e:
+        ks.omp()
+    assert e.value.code == kl.KSVD_ERR_STATE
+    _, noisy = kl.synth_image(3, 24, 24, 10.0)
+    ks.set_picture(noisy)  # 81 patches < P/2 + 256
+    ks.extract()
+    with pytest.raises(kl.KsvdError) as e:
+        ks.init_dictionary()
+    assert e.value.code == kl.KSVD_ERR_ARG
+    with pytest.raises(kl.KsvdError) as e:
+        ks.set_picture(np.zeros((4, 4), np.float32))
+    assert e.value.code == kl.KSVD_ERR_ARG
+    ks.close()
