@@ -1,0 +1,150 @@
+// denoising: the reference's pipeline interface (src/lib/denoisingLib/
+// denoisingLib.h:13-122) with the gfx950 backend behind it.
+//
+// What stays as in the reference: the DenoiserType tag and its string map,
+// Denoiser::factory(type, input, output), denoising() with status 0 / -1 load
+// / -2 denoise / -3 save, getTimeElapsed(), getPsnr(), the protected
+// per-image parameters with the same defaults, and BatchDenoiser::factory
+// (config.json) with getDenoiserList().
+// What changes: HIP_K_SVD (plain FP32) and its precision / learned variants
+// are new tags; the CUDA_K_* tags run the same HIP rank-1 engine (their SVD
+// choice only changes how the top singular triple is computed); an unknown
+// tag is an error instead of a silent CUDA_K_GESVD.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "baseUtilityLib.h"
+
+namespace img {
+struct Image;
+}
+
+namespace denoising {
+
+class Denoiser;
+class HipKSvdDenoiser;
+class BatchDenoiser;
+
+enum DenoiserType {
+    CUDA_K_GESVD,
+    CUDA_K_GESVDJ,
+    CUDA_K_GESVDA,
+    HIP_K_SVD,               // plain FP32 on gfx950: the drop-in beside CUDA_K_GESVDJ
+    HIP_K_SVD_BF16,          // BF16 correlation on MFMA, FP32 re-score / LS / SVD
+    HIP_K_SVD_BF16_LEARNED,  // BF16 + learned support proposer, OMP fallback
+    DENOISER_TYPE_UNKNOWN
+};
+
+struct DenoiserTypeMap : public std::map<std::string, DenoiserType> {
+    DenoiserTypeMap()
+    {
+        (*this)["CUDA_K_GESVD"] = CUDA_K_GESVD;
+        (*this)["CUDA_K_GESVDJ"] = CUDA_K_GESVDJ;
+        (*this)["CUDA_K_GESVDA"] = CUDA_K_GESVDA;
+        (*this)["HIP_K_SVD"] = HIP_K_SVD;
+        (*this)["HIP_K_SVD_BF16"] = HIP_K_SVD_BF16;
+        (*this)["HIP_K_SVD_BF16_LEARNED"] = HIP_K_SVD_BF16_LEARNED;
+    }
+    // The reference's operator[] default-inserts CUDA_K_GESVD for a typo
+    // (batch_denoiser.cpp:96); lookup() reports it instead.
+    DenoiserType lookup(const std::string &name) const
+    {
+        auto it = find(name);
+        return it == end() ? DENOISER_TYPE_UNKNOWN : it->second;
+    }
+};
+
+}  // namespace denoising
+
+class denoising::Denoiser {
+public:
+    virtual ~Denoiser();
+    static Denoiser *factory(denoising::DenoiserType, std::string inputFile, std::string outputFile);
+    virtual signed char denoising() = 0;
+    baseUtl::TimeElapsed *getTimeElapsed();
+    std::vector<double> *getPsnr();
+    // Message of the last failure ("" if none).
+    const std::string &getError() const { return error; }
+
+protected:
+    int patchWidthDim = 8;
+    int patchHeightDim = 8;
+    int slidingWidth = 2;
+    int slidingHeight = 2;
+    int subImageWidthDim = 0;
+    int subImageHeightDim = 0;
+    int ompIter = 5;
+    int atoms = 256;
+    int iter = 10;
+    int minOmpIterBatch = 0;
+    float sigma = 0;  // noise variance estimated at load (denoiser.cpp:56)
+    bool speckle = false;
+    bool bw = true;
+    std::string refImage;
+    // backend knobs (optional config keys userCap, geometry, device)
+    int userCap = 1024;
+    int geometry = 0;
+    int device = 0;
+    baseUtl::Matrix *inputMatrix = nullptr;
+    baseUtl::Matrix *outputMatrix = nullptr;
+    baseUtl::TimeElapsed *timeElapsed = nullptr;
+    std::vector<double> *psnr = nullptr;
+    std::string error;
+
+    Denoiser();
+    virtual bool loadImage();
+    virtual bool saveImage();
+    virtual bool internalDenoising() = 0;
+
+private:
+    std::string inputFile, outputFile;
+    img::Image *inputImage = nullptr;
+
+    friend BatchDenoiser;
+};
+
+class denoising::HipKSvdDenoiser : public denoising::Denoiser {
+public:
+    ~HipKSvdDenoiser() override;
+    signed char denoising() override;
+    // Throughput of the last run: patches * ksvdIter / time of the K-SVD
+    // iterations (device events), and the patch count.
+    double patchItersPerSec() const { return patchIters; }
+    long long patches() const { return nPatches; }
+
+protected:
+    bool loadImage() override;
+    bool saveImage() override;
+    bool internalDenoising() override;
+
+private:
+    DenoiserType type = HIP_K_SVD;
+    double patchIters = 0;
+    long long nPatches = 0;
+
+    HipKSvdDenoiser();
+    // One frame buffer (n rows of length m) through the whole K-SVD path;
+    // out receives the denoised buffer in the same frame.
+    bool denoiseFrame(const float *buf, int m, int n, float noiseVar, std::vector<float> &out);
+
+    friend Denoiser;
+    friend BatchDenoiser;
+};
+
+class denoising::BatchDenoiser {
+public:
+    ~BatchDenoiser();
+    std::vector<Denoiser *> getDenoiserList();
+    // nullptr (and a message on stderr) when the file cannot be parsed, a
+    // required key is missing, or a type tag is unknown.
+    static BatchDenoiser *factory(std::string jsonFile);
+
+protected:
+    std::vector<Denoiser *> denoisers;
+
+private:
+    BatchDenoiser();
+};
