@@ -243,6 +243,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
         hipStreamSynchronize(ctx->stream);
     }
     drop_timeline(ctx);
+    drop_update_graph(ctx);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf,    &ctx->rsum,    &ctx->rweight,
@@ -295,6 +296,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->rout, sizeof(float) * (size_t)N},
         {&ctx->rpart, sizeof(double) * (size_t)((N + 255) / 256)},
     };
+    drop_update_graph(ctx);  // buffers may move
     for (auto &e : plan) {
         rc = ksvd_reserve(ctx, *e.b, e.bytes);
         if (rc) return rc;

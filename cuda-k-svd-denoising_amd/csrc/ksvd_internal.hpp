@@ -49,6 +49,8 @@ struct ksvd_ctx {
     ksvd_timings tm{};
     std::string err;
     void *timeline = nullptr;  // event bookkeeping owned by capi.hip
+    void *upd_graph = nullptr;  // captured dictionary sweep (hipGraph_t)
+    void *upd_exec = nullptr;   // its executable (hipGraphExec_t)
 };
 
 // Set ctx->err and return code.
@@ -70,4 +72,5 @@ int launch_init_dict(ksvd_ctx *ctx);
 int launch_dict_gram(ksvd_ctx *ctx);
 int launch_omp(ksvd_ctx *ctx);
 int launch_update(ksvd_ctx *ctx);
+void drop_update_graph(ksvd_ctx *ctx);
 int launch_rebuild(ksvd_ctx *ctx, float sigma2);

@@ -15,6 +15,8 @@
 //   upd_eig       top eigenvector of G by power iteration in LDS, started
 //                 from the previous atom; written to D[:,k];
 //   upd_coef      x_{k,p} = e_t . d_k (= s v_t) for t < cap, 0 beyond.
+#include <cstdlib>
+
 #include "ksvd_internal.hpp"
 
 namespace {
@@ -296,7 +298,9 @@ __global__ __launch_bounds__(256) void upd_coef_kernel(int k, int dim, const flo
 
 }  // namespace
 
-int launch_update(ksvd_ctx *ctx)
+// Enqueue one sweep: user lists, then per atom build -> Gram -> eigenvector ->
+// coefficients.  Everything is stream-ordered; no host synchronisation.
+static int enqueue_update(ksvd_ctx *ctx)
 {
     const int K = ctx->K, kmax = ctx->kmax, dim = ctx->dim;
     const int64_t P = ctx->P;
@@ -316,8 +320,6 @@ int launch_update(ksvd_ctx *ctx)
     KSVD_HIP_CHECK(ctx, hipGetLastError());
 
     const size_t eig_lds = sizeof(float) * ((size_t)dim * dim + dim + 1024);
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)upd_eig_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)eig_lds));
     const int nbt = (dim + 15) / 16;
     float *E = (float *)ctx->E.ptr;
     float *Gk = (float *)ctx->wbuf.ptr;
@@ -337,6 +339,47 @@ int launch_update(ksvd_ctx *ctx)
                            (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr, cap, (const float *)E);
     }
     KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+void drop_update_graph(ksvd_ctx *ctx)
+{
+    if (ctx->upd_exec) hipGraphExecDestroy((hipGraphExec_t)ctx->upd_exec);
+    if (ctx->upd_graph) hipGraphDestroy((hipGraph_t)ctx->upd_graph);
+    ctx->upd_exec = nullptr;
+    ctx->upd_graph = nullptr;
+}
+
+// The sweep's 4K + 4 launches are captured once per buffer layout into a
+// HIP graph and replayed (set KSVD_NO_GRAPH=1 to launch them one by one).
+int launch_update(ksvd_ctx *ctx)
+{
+    const size_t eig_lds = sizeof(float) * ((size_t)ctx->dim * ctx->dim + ctx->dim + 1024);
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)upd_eig_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)eig_lds));
+    const char *ng = getenv("KSVD_NO_GRAPH");
+    const bool no_graph = ng && ng[0] && ng[0] != '0';
+    if (no_graph) {
+        int rc = enqueue_update(ctx);
+        if (rc) return rc;
+    } else {
+        if (!ctx->upd_exec) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            KSVD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_update(ctx);
+            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return ksvd_fail(ctx, KSVD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+            KSVD_HIP_CHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            ctx->upd_graph = g;
+            ctx->upd_exec = ge;
+        }
+        KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
+    }
     ctx->gram_valid = false;
     return KSVD_OK;
 }

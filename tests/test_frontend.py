@@ -120,6 +120,6 @@ def test_denoising_cli_end_to_end(denoising_bin, tmp_path, kl):
     assert r.stdout.count("PSNR after:") == 2
     before = [float(l.split(":")[1]) for l in r.stdout.splitlines() if "PSNR before:" in l]
     after = [float(l.split(":")[1]) for l in r.stdout.splitlines() if "PSNR after:" in l]
-    assert abs(before[0] - kl.psnr(clean, noisy)) < 1e-6
+    assert abs(before[0] - kl.psnr(clean, noisy)) < 1e-3  # printed with 6 significant digits
     assert all(a > 0 for a in after)
     assert os.path.exists(tmp_path / "8_8_noisy.pfm") and os.path.exists(tmp_path / "6_6_noisy.pfm")

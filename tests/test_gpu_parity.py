@@ -252,3 +252,32 @@ def test_state_and_argument_errors(kl):
         ks.set_picture(np.zeros((4, 4), np.float32))
     assert e.value.code == kl.KSVD_ERR_ARG
     ks.close()
+
+
+def test_update_graph_replays_match_eager(kl, orc, monkeypatch):
+    """The dictionary sweep is captured once into a HIP graph and replayed;
+    every replay (with eager work and synchronisations in between) must give
+    the bits of the eagerly launched sweep."""
+    cfg = CFG1
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    Y = ks.patches()
+    D0 = ks.dictionary()
+    io, co, no, _ = orc.omp(Y, D0, cfg["k"], real="f32")
+    monkeypatch.setenv("KSVD_NO_GRAPH", "1")
+    ks.set_codes(io, co, no)
+    ks.update()
+    D_eager = ks.dictionary()
+    c_eager = ks.codes()[1]
+    monkeypatch.setenv("KSVD_NO_GRAPH", "0")
+    for rep in range(3):
+        ks.set_dictionary(D0)
+        ks.set_codes(io, co, no)
+        ks.omp()                       # eager work between replays
+        ks.set_codes(io, co, no)
+        ks.synchronize()
+        ks.update()                    # graph replay
+        np.testing.assert_array_equal(ks.dictionary(), D_eager, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(ks.codes()[1], c_eager, err_msg=f"replay {rep}")
+    ks.close()
