@@ -1,6 +1,7 @@
 // C ABI of the gfx950 K-SVD library (declarations and reference citations in
 // include/ksvd_hip.h).  Host orchestration only: buffer sizing, state checks,
 // launches on the context's stream, HIP-event timing of every phase.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -246,7 +247,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
     drop_update_graph(ctx);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
-                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf,    &ctx->rsum,    &ctx->rweight,
+                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->wcol, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth,    &ctx->rsum,    &ctx->rweight,
                       &ctx->rout, &ctx->rpart,  &ctx->rlam};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
@@ -276,6 +277,9 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
     const int64_t nb = (P + 255) / 256;
     const int64_t ecols = p.user_cap > 0 ? p.user_cap : P;
+    // columns of the sweep's table: sum over atoms of min(users, cap) <= both
+    // P * kmax (every support entry) and K * cap
+    const int64_t tcols = p.user_cap > 0 ? std::min<int64_t>(P * kmax, (int64_t)K * p.user_cap) : P * kmax;
     struct {
         DevBuf *b;
         size_t bytes;
@@ -291,6 +295,11 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->users, sizeof(int) * (size_t)P * kmax},
         {&ctx->uslot, sizeof(int) * (size_t)P * kmax},
         {&ctx->E, sizeof(float) * (size_t)dim * ecols},
+        {&ctx->wcol, sizeof(float) * (size_t)(p.user_cap > 0 ? p.user_cap : P)},
+        {&ctx->moff, sizeof(int) * (size_t)(K + 1)},
+        {&ctx->tab_p, sizeof(int) * (size_t)tcols},
+        {&ctx->tab_self, sizeof(int) * (size_t)tcols},
+        {&ctx->tab_oth, sizeof(int) * 2 * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
         {&ctx->rsum, sizeof(float) * (size_t)N},
         {&ctx->rweight, sizeof(float) * (size_t)N},
         {&ctx->rout, sizeof(float) * (size_t)N},

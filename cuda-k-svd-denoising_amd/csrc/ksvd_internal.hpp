@@ -43,7 +43,8 @@ struct ksvd_ctx {
 
     // device buffers
     DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
-    DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
+    DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf, wcol;
+    DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
     DevBuf rsum, rweight, rout, rpart, rlam;
 
     ksvd_timings tm{};

@@ -296,6 +296,218 @@ __global__ __launch_bounds__(256) void upd_coef_kernel(int k, int dim, const flo
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Column table of one sweep, built in parallel before it: for atom k and its
+// t-th capped user (t < m_k = min(c_k, cap)), the patch, the coefficient slot
+// of atom k, and the (atom, coefficient slot) pairs of the patch's other
+// support entries.  Only coefficient VALUES change during the sweep, so the
+// table holds for all of it and the sweep's loads stop depending on each
+// other.  moff[k] = sum_{j<k} m_j.
+__global__ __launch_bounds__(1024) void cols_offsets_kernel(const int *__restrict__ ucount, int K, int cap,
+                                                            int *__restrict__ moff)
+{
+    __shared__ int sh[1024];
+    const int a = threadIdx.x;
+    int v = a < K ? ucount[a] : 0;
+    if (cap > 0 && v > cap) v = cap;
+    sh[a] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int t = a >= off ? sh[a - off] : 0;
+        __syncthreads();
+        sh[a] += t;
+        __syncthreads();
+    }
+    if (a < K) moff[a] = sh[a] - v;
+    if (a == 0) moff[K] = sh[1023];
+}
+
+__global__ __launch_bounds__(256) void cols_table_kernel(int K, int kmax, int cap, const int *__restrict__ idx,
+                                                         const int *__restrict__ cnt, const int *__restrict__ uoff,
+                                                         const int *__restrict__ moff, const int *__restrict__ users,
+                                                         const int *__restrict__ uslot, int *__restrict__ tab_p,
+                                                         int *__restrict__ tab_self, int2 *__restrict__ tab_oth)
+{
+    const int k = blockIdx.y;
+    const int c = uoff[k + 1] - uoff[k];
+    const int m = (cap > 0 && c > cap) ? cap : c;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x) {
+        const int pos = uoff[k] + t;
+        const int p = users[pos], my = uslot[pos];
+        const int64_t col = moff[k] + t;
+        tab_p[col] = p;
+        tab_self[col] = p * kmax + my;
+        const int cp = cnt[p];
+        int na = 0;
+        for (int i = 0; i < cp; i++)
+            if (i != my) tab_oth[col * (kmax - 1) + na++] = make_int2(idx[(int64_t)p * kmax + i], p * kmax + i);
+        for (; na < kmax - 1; na++) tab_oth[col * (kmax - 1) + na] = make_int2(-1, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The whole Gauss-Seidel sweep in one workgroup (1024 threads, one CU): the
+// atoms are sequential and cross-workgroup synchronisation costs more than
+// an atom's work, so the sweep stays on one CU and only uses __syncthreads.
+//   * build: thread (tc, e) = (tid / dim, tid % dim) computes element e of
+//     columns tc, tc + cpt, ... (cpt = 1024 / dim columns at a time) from the
+//     column table: every load of a column is independent of the others;
+//   * columns t < lds_cols stay in LDS, the rest in an L2-resident buffer;
+//   * a power step: wave w takes blocks of UNR consecutive columns, lane l
+//     holds elements l + 64 j; w_t = e_t . u by wave reduction, z += w_t e_t
+//     in per-lane accumulators, z reduced over the 16 waves in fixed order;
+//   * stop when two successive unit iterates differ by less than sqrt(tol2)
+//     (never before the second); the last step's w_t = e_t . u are the new
+//     coefficients (s v_t), u the new atom.
+__device__ __forceinline__ float block_sum16(float v, float *red)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < 16; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+constexpr int UNR = 4;  // columns per wave per pass step
+
+template <int RJ>
+__device__ __forceinline__ void power_pass(const float *__restrict__ Es, const float *__restrict__ Eg, int lds_cols,
+                                           int m, int dim, const float *__restrict__ u, float *__restrict__ ws,
+                                           float (&zacc)[RJ])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < RJ; j++) zacc[j] = 0.f;
+    float uv[RJ];
+#pragma unroll
+    for (int j = 0; j < RJ; j++) {
+        const int e = lane + 64 * j;
+        uv[j] = e < dim ? u[e] : 0.f;
+    }
+    for (int t0 = wave * UNR; t0 < m; t0 += 16 * UNR) {
+        float ev[UNR][RJ];
+        float dot[UNR];
+#pragma unroll
+        for (int q = 0; q < UNR; q++) {
+            const int t = t0 + q;
+            const float *src = t < lds_cols ? Es + (int64_t)t * dim : Eg + (int64_t)(t - lds_cols) * dim;
+            dot[q] = 0.f;
+#pragma unroll
+            for (int j = 0; j < RJ; j++) {
+                const int e = lane + 64 * j;
+                ev[q][j] = (t < m && e < dim) ? src[e] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < UNR; q++)
+#pragma unroll
+            for (int j = 0; j < RJ; j++) dot[q] = fmaf(ev[q][j], uv[j], dot[q]);
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+            for (int q = 0; q < UNR; q++) dot[q] += __shfl_xor(dot[q], off);
+#pragma unroll
+        for (int q = 0; q < UNR; q++) {
+            if (lane == 0 && t0 + q < m) ws[t0 + q] = dot[q];
+#pragma unroll
+            for (int j = 0; j < RJ; j++) zacc[j] = fmaf(dot[q], ev[q][j], zacc[j]);
+        }
+    }
+}
+
+template <int RJ>
+__global__ __launch_bounds__(1024, 1) void sweep_kernel(
+    const float *__restrict__ Y, int dim, float *__restrict__ D, int K, int kmax, float *__restrict__ coef,
+    const int *__restrict__ uoff, const int *__restrict__ users, const int *__restrict__ uslot, int cap,
+    const int *__restrict__ moff, const int *__restrict__ tab_p, const int *__restrict__ tab_self,
+    const int2 *__restrict__ tab_oth, float *__restrict__ Eg, float *__restrict__ ws, int lds_cols, int maxit,
+    float tol2, unsigned long long *__restrict__ counters)
+{
+    extern __shared__ float sm[];
+    float *u = sm;                         // dim
+    float *zpart = u + dim;                // 16 x dim
+    float *red = zpart + 16 * dim;         // 16
+    float *Es = red + 16;                  // lds_cols x dim
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cpt = 1024 / dim;            // columns built at a time
+    const int tc = tid / dim, te = tid - tc * dim;
+    const int no = kmax - 1;
+    unsigned long long total_it = 0;
+
+    for (int k = 0; k < K; k++) {
+        const int base = uoff[k];
+        const int c = uoff[k + 1] - base;
+        if (c <= 0) continue;
+        const int m = (cap > 0 && c > cap) ? cap : c;
+        const int64_t cb = moff[k];
+        for (int e = tid; e < dim; e += 1024) u[e] = D[(int64_t)k * dim + e];
+        // ---- build E ------------------------------------------------------
+        if (tc < cpt) {
+            for (int t = tc; t < m; t += 2 * cpt) {
+                const int t2 = t + cpt;
+                const bool two = t2 < m;
+                const int p1 = tab_p[cb + t];
+                const int p2 = two ? tab_p[cb + t2] : p1;
+                float v1 = Y[(int64_t)p1 * dim + te];
+                float v2 = Y[(int64_t)p2 * dim + te];
+                for (int i = 0; i < no; i++) {
+                    const int2 o1 = tab_oth[(cb + t) * no + i];
+                    const int2 o2 = two ? tab_oth[(cb + t2) * no + i] : make_int2(-1, 0);
+                    if (o1.x >= 0) v1 = fmaf(-D[(int64_t)o1.x * dim + te], coef[o1.y], v1);
+                    if (o2.x >= 0) v2 = fmaf(-D[(int64_t)o2.x * dim + te], coef[o2.y], v2);
+                }
+                if (t < lds_cols) Es[(int64_t)t * dim + te] = v1;
+                else Eg[(int64_t)(t - lds_cols) * dim + te] = v1;
+                if (two) {
+                    if (t2 < lds_cols) Es[(int64_t)t2 * dim + te] = v2;
+                    else Eg[(int64_t)(t2 - lds_cols) * dim + te] = v2;
+                }
+            }
+        }
+        __syncthreads();
+        float zacc[RJ];
+        power_pass<RJ>(Es, Eg, lds_cols, m, dim, u, ws, zacc);
+        int it = 0;
+        for (;;) {
+#pragma unroll
+            for (int j = 0; j < RJ; j++) {
+                const int e = lane + 64 * j;
+                if (e < dim) zpart[wave * dim + e] = zacc[j];
+            }
+            __syncthreads();
+            float z = 0.f;
+            if (tid < dim)
+                for (int w = 0; w < 16; w++) z += zpart[w * dim + tid];
+            const float nz = sqrtf(block_sum16(tid < dim ? z * z : 0.f, red));
+            const float un = (tid < dim && nz > 0.f) ? z / nz : 0.f;
+            const float du = tid < dim ? un - u[tid] : 0.f;
+            const float d2 = block_sum16(du * du, red);
+            it++;
+            if (nz == 0.f) break;                  // E = 0: keep d_k, coefficients are 0
+            if (it >= 2 && d2 < tol2) break;       // ws holds e_t . u for the current u
+            if (it > maxit) break;
+            if (tid < dim) u[tid] = un;
+            __syncthreads();
+            power_pass<RJ>(Es, Eg, lds_cols, m, dim, u, ws, zacc);
+        }
+        total_it += it;
+        __syncthreads();
+        // ---- write back d_k and the coefficients ------------------------------
+        for (int e = tid; e < dim; e += 1024) D[(int64_t)k * dim + e] = u[e];
+        for (int t = tid; t < m; t += 1024) coef[tab_self[cb + t]] = ws[t];
+        for (int t = m + tid; t < c; t += 1024) {
+            const int pos = base + t;
+            coef[(int64_t)users[pos] * kmax + uslot[pos]] = 0.f;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) atomicAdd(&counters[CNT_EIG_IT], total_it);
+}
+
 }  // namespace
 
 // Enqueue one sweep: user lists, then per atom build -> Gram -> eigenvector ->
@@ -319,11 +531,45 @@ static int enqueue_update(ksvd_ctx *ctx)
                        (int *)ctx->uslot.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
 
-    const size_t eig_lds = sizeof(float) * ((size_t)dim * dim + dim + 1024);
-    const int nbt = (dim + 15) / 16;
-    float *E = (float *)ctx->E.ptr;
-    float *Gk = (float *)ctx->wbuf.ptr;
     const float tol = 1e-6f;
+    float *E = (float *)ctx->E.ptr;
+    const char *mk = getenv("KSVD_MULTI_KERNEL");
+    if (!(mk && mk[0] && mk[0] != '0')) {
+        int *moff = (int *)ctx->moff.ptr;
+        hipLaunchKernelGGL(cols_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const int *)ctx->ucount.ptr, K,
+                           cap, moff);
+        hipLaunchKernelGGL(cols_table_kernel, dim3(4, K), dim3(256), 0, ctx->stream, K, kmax, cap,
+                           (const int *)ctx->idx.ptr, (const int *)ctx->cnt.ptr, (const int *)ctx->uoff.ptr,
+                           (const int *)moff, (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr,
+                           (int *)ctx->tab_p.ptr, (int *)ctx->tab_self.ptr, (int2 *)ctx->tab_oth.ptr);
+        const size_t fixed = sizeof(float) * ((size_t)dim * 17 + 16);
+        const size_t budget = 156 * 1024;
+        const int lds_cols = (int)((budget - fixed) / (sizeof(float) * dim));
+        const size_t lds = fixed + sizeof(float) * (size_t)lds_cols * dim;
+        const int rj = (dim + 63) / 64;
+#define KSVD_SWEEP(RJ)                                                                                         \
+    do {                                                                                                       \
+        auto kern = sweep_kernel<RJ>;                                                                          \
+        KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                (int)lds));                                                    \
+        hipLaunchKernelGGL(kern, dim3(1), dim3(1024), lds, ctx->stream, (const float *)ctx->Y.ptr, dim,        \
+                           (float *)ctx->D.ptr, K, kmax, (float *)ctx->coef.ptr, (const int *)ctx->uoff.ptr,   \
+                           (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr, cap, (const int *)moff,   \
+                           (const int *)ctx->tab_p.ptr, (const int *)ctx->tab_self.ptr,                        \
+                           (const int2 *)ctx->tab_oth.ptr, E, (float *)ctx->wcol.ptr, lds_cols, 300,           \
+                           tol * tol, (unsigned long long *)ctx->counters.ptr);                                \
+    } while (0)
+        if (rj <= 1) KSVD_SWEEP(1);
+        else if (rj <= 2) KSVD_SWEEP(2);
+        else if (rj <= 3) KSVD_SWEEP(3);
+        else KSVD_SWEEP(4);
+#undef KSVD_SWEEP
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        return KSVD_OK;
+    }
+    const int nbt = (dim + 15) / 16;
+    float *Gk = (float *)ctx->wbuf.ptr;
+    const size_t eig_lds = sizeof(float) * ((size_t)dim * dim + dim + 1024);
     for (int k = 0; k < K; k++) {
         hipLaunchKernelGGL(upd_build_kernel, dim3(BUILD_BLOCKS), dim3(256), 0, ctx->stream, k,
                            (const float *)ctx->Y.ptr, dim, (const float *)ctx->D.ptr, kmax, (const int *)ctx->idx.ptr,
