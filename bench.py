@@ -174,6 +174,7 @@ def main():
                      "flops_per_launch": flops_launch},
         "phase_ms_per_step": {"omp": round(tm["omp_ms"] / args.steps, 4),
                               "update": round(upd_avg_ms, 4)},
+        "power_iters_per_atom": round(tm["eig_iters"] / max(tm["update_calls"], 1) / wl["atoms"], 2),
         "images_per_s": round(1.0 / T_img, 3),
         "psnr": {"noisy": round(psnr_noisy, 4), "denoised_normalized": round(psnr_out, 4),
                  "denoised_raw": round(psnr_raw, 4)},
