@@ -15,6 +15,7 @@
 //   upd_eig       top eigenvector of G by power iteration in LDS, started
 //                 from the previous atom; written to D[:,k];
 //   upd_coef      x_{k,p} = e_t . d_k (= s v_t) for t < cap, 0 beyond.
+#include <algorithm>
 #include <cstdlib>
 
 #include "ksvd_internal.hpp"
@@ -372,7 +373,8 @@ __device__ __forceinline__ float block_sum16(float v, float *red)
     return s;
 }
 
-constexpr int UNR = 4;  // columns per wave per pass step
+constexpr int UNR = 8;  // columns per wave per pass step
+constexpr int BU = 8;   // columns per thread per build round
 
 template <int RJ>
 __device__ __forceinline__ void power_pass(const float *__restrict__ Es, const float *__restrict__ Eg, int lds_cols,
@@ -424,18 +426,21 @@ __global__ __launch_bounds__(1024, 1) void sweep_kernel(
     const float *__restrict__ Y, int dim, float *__restrict__ D, int K, int kmax, float *__restrict__ coef,
     const int *__restrict__ uoff, const int *__restrict__ users, const int *__restrict__ uslot, int cap,
     const int *__restrict__ moff, const int *__restrict__ tab_p, const int *__restrict__ tab_self,
-    const int2 *__restrict__ tab_oth, float *__restrict__ Eg, float *__restrict__ ws, int lds_cols, int maxit,
-    float tol2, unsigned long long *__restrict__ counters)
+    const int2 *__restrict__ tab_oth, float *__restrict__ Eg, float *__restrict__ ws, int lds_cols, int mch,
+    int maxit, float tol2, unsigned long long *__restrict__ counters)
 {
     extern __shared__ float sm[];
     float *u = sm;                         // dim
     float *zpart = u + dim;                // 16 x dim
     float *red = zpart + 16 * dim;         // 16
     float *Es = red + 16;                  // lds_cols x dim
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    int *mp = (int *)(Es + (size_t)lds_cols * dim);   // mch
+    int *ma = mp + mch;                                // mch x no
+    float *mc = (float *)(ma + (size_t)mch * no);      // mch x no
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cpt = 1024 / dim;            // columns built at a time
     const int tc = tid / dim, te = tid - tc * dim;
-    const int no = kmax - 1;
     unsigned long long total_it = 0;
 
     for (int k = 0; k < K; k++) {
@@ -446,27 +451,55 @@ __global__ __launch_bounds__(1024, 1) void sweep_kernel(
         const int64_t cb = moff[k];
         for (int e = tid; e < dim; e += 1024) u[e] = D[(int64_t)k * dim + e];
         // ---- build E ------------------------------------------------------
-        if (tc < cpt) {
-            for (int t = tc; t < m; t += 2 * cpt) {
-                const int t2 = t + cpt;
-                const bool two = t2 < m;
-                const int p1 = tab_p[cb + t];
-                const int p2 = two ? tab_p[cb + t2] : p1;
-                float v1 = Y[(int64_t)p1 * dim + te];
-                float v2 = Y[(int64_t)p2 * dim + te];
+        // (a) one thread per column stages the patch, the other atoms and
+        //     their current coefficients in LDS (one round of loads);
+        // (b) thread (tc, te) then computes element te of BU columns at once
+        //     from LDS addresses: all their loads are in flight together.
+        for (int c0 = 0; c0 < m; c0 += mch) {
+            const int nc = min(mch, m - c0);
+            for (int t = tid; t < nc; t += 1024) {
+                const int64_t col = cb + c0 + t;
+                mp[t] = tab_p[col];
                 for (int i = 0; i < no; i++) {
-                    const int2 o1 = tab_oth[(cb + t) * no + i];
-                    const int2 o2 = two ? tab_oth[(cb + t2) * no + i] : make_int2(-1, 0);
-                    if (o1.x >= 0) v1 = fmaf(-D[(int64_t)o1.x * dim + te], coef[o1.y], v1);
-                    if (o2.x >= 0) v2 = fmaf(-D[(int64_t)o2.x * dim + te], coef[o2.y], v2);
-                }
-                if (t < lds_cols) Es[(int64_t)t * dim + te] = v1;
-                else Eg[(int64_t)(t - lds_cols) * dim + te] = v1;
-                if (two) {
-                    if (t2 < lds_cols) Es[(int64_t)t2 * dim + te] = v2;
-                    else Eg[(int64_t)(t2 - lds_cols) * dim + te] = v2;
+                    const int2 o = tab_oth[col * no + i];
+                    ma[t * no + i] = o.x;
+                    mc[t * no + i] = o.x >= 0 ? coef[o.y] : 0.f;
                 }
             }
+            __syncthreads();
+            if (tc < cpt) {
+                for (int t = tc; t < nc; t += BU * cpt) {
+                    float v[BU];
+#pragma unroll
+                    for (int b = 0; b < BU; b++) {
+                        const int tt = t + b * cpt;
+                        v[b] = tt < nc ? Y[(int64_t)mp[tt] * dim + te] : 0.f;
+                    }
+                    for (int i = 0; i < no; i++) {
+                        float dv[BU];
+#pragma unroll
+                        for (int b = 0; b < BU; b++) {
+                            const int tt = t + b * cpt;
+                            const int a = tt < nc ? ma[tt * no + i] : -1;
+                            dv[b] = a >= 0 ? D[(int64_t)a * dim + te] : 0.f;
+                        }
+#pragma unroll
+                        for (int b = 0; b < BU; b++) {
+                            const int tt = t + b * cpt;
+                            if (tt < nc) v[b] = fmaf(-dv[b], mc[tt * no + i], v[b]);
+                        }
+                    }
+#pragma unroll
+                    for (int b = 0; b < BU; b++) {
+                        const int tt = c0 + t + b * cpt;
+                        if (t + b * cpt < nc) {
+                            if (tt < lds_cols) Es[(int64_t)tt * dim + te] = v[b];
+                            else Eg[(int64_t)(tt - lds_cols) * dim + te] = v[b];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
         }
         __syncthreads();
         float zacc[RJ];
@@ -547,10 +580,15 @@ static int enqueue_update(ksvd_ctx *ctx)
                            (const int *)ctx->idx.ptr, (const int *)ctx->cnt.ptr, (const int *)ctx->uoff.ptr,
                            (const int *)moff, (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr,
                            (int *)ctx->tab_p.ptr, (int *)ctx->tab_self.ptr, (int2 *)ctx->tab_oth.ptr);
+        // LDS: u, 16 partial z, reductions | E columns kept on chip | column
+        // metadata of up to mch columns (patch, other atoms, coefficients)
+        const int no = kmax > 1 ? kmax - 1 : 1;
         const size_t fixed = sizeof(float) * ((size_t)dim * 17 + 16);
+        const size_t meta_per_col = sizeof(int) + (sizeof(int) + sizeof(float)) * (size_t)no;
+        const int mch = (int)std::min<size_t>(1024, (40 * 1024) / meta_per_col);
         const size_t budget = 156 * 1024;
-        const int lds_cols = (int)((budget - fixed) / (sizeof(float) * dim));
-        const size_t lds = fixed + sizeof(float) * (size_t)lds_cols * dim;
+        const int lds_cols = (int)((budget - fixed - meta_per_col * mch) / (sizeof(float) * dim));
+        const size_t lds = fixed + sizeof(float) * (size_t)lds_cols * dim + meta_per_col * mch;
         const int rj = (dim + 63) / 64;
 #define KSVD_SWEEP(RJ)                                                                                         \
     do {                                                                                                       \
@@ -561,7 +599,7 @@ static int enqueue_update(ksvd_ctx *ctx)
                            (float *)ctx->D.ptr, K, kmax, (float *)ctx->coef.ptr, (const int *)ctx->uoff.ptr,   \
                            (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr, cap, (const int *)moff,   \
                            (const int *)ctx->tab_p.ptr, (const int *)ctx->tab_self.ptr,                        \
-                           (const int2 *)ctx->tab_oth.ptr, E, (float *)ctx->wcol.ptr, lds_cols, 300,           \
+                           (const int2 *)ctx->tab_oth.ptr, E, (float *)ctx->wcol.ptr, lds_cols, mch, 300,      \
                            tol * tol, (unsigned long long *)ctx->counters.ptr);                                \
     } while (0)
         if (rj <= 1) KSVD_SWEEP(1);
