@@ -294,7 +294,8 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->uoff, sizeof(int) * (size_t)(K + 1)},
         {&ctx->users, sizeof(int) * (size_t)P * kmax},
         {&ctx->uslot, sizeof(int) * (size_t)P * kmax},
-        {&ctx->E, sizeof(float) * (size_t)dim * ecols},
+        // E columns: pitch up to 192 floats (the tile sweep pads columns to 4Q)
+        {&ctx->E, sizeof(float) * (size_t)std::max(dim, 192) * ecols},
         {&ctx->wcol, sizeof(float) * (size_t)(p.user_cap > 0 ? p.user_cap : P)},
         {&ctx->moff, sizeof(int) * (size_t)(K + 1)},
         {&ctx->tab_p, sizeof(int) * (size_t)tcols},

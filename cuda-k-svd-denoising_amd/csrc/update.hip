@@ -541,6 +541,193 @@ __global__ __launch_bounds__(1024, 1) void sweep_kernel(
     if (tid == 0) atomicAdd(&counters[CNT_EIG_IT], total_it);
 }
 
+
+// ---------------------------------------------------------------------------
+// Tile form of the sweep (dim <= 4Q): 512 threads, eight wavefronts; a
+// wavefront handles tiles of 16 columns, lane (col, g) = (lane % 16, lane / 16)
+// holding quarter g (elements g*Q .. g*Q+Q-1) of column col.  A column's dot
+// product needs two cross-lane steps; z accumulates per lane and is folded
+// across the 16 lanes of a quarter once per pass.  Columns are stored with
+// pitch 4Q (zero padded): the first lds_cols in LDS, the rest in L2.
+template <int Q>
+__device__ __forceinline__ void tile_fold_z(float (&zacc)[Q], float *zpart_w, int lane)
+{
+    const int col = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < Q; kk++) {
+        float v = zacc[kk];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        zacc[kk] = v;
+    }
+    if (col == 0)
+#pragma unroll
+        for (int kk = 0; kk < Q; kk++) zpart_w[g * Q + kk] = zacc[kk];
+}
+
+template <int Q>
+__device__ __forceinline__ void tile_pass(const float *__restrict__ Es, const float *__restrict__ Eg, int lds_cols,
+                                          int m, const float *__restrict__ u, float *__restrict__ ws,
+                                          float *zpart_w)
+{
+    constexpr int LDE = 4 * Q;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    float uq[Q], zacc[Q];
+#pragma unroll
+    for (int kk = 0; kk < Q; kk++) {
+        uq[kk] = u[g * Q + kk];
+        zacc[kk] = 0.f;
+    }
+    const int ntile = (m + 15) >> 4;
+    for (int tl = wave; tl < ntile; tl += 8) {
+        const int c = tl * 16 + col;
+        float ev[Q];
+        if (c < m) {
+            const float *src = c < lds_cols ? Es + (size_t)c * LDE : Eg + (size_t)(c - lds_cols) * LDE;
+#pragma unroll
+            for (int kk = 0; kk < Q; kk += 4) {
+                const float4 v4 = *(const float4 *)(src + g * Q + kk);
+                ev[kk] = v4.x;
+                ev[kk + 1] = v4.y;
+                ev[kk + 2] = v4.z;
+                ev[kk + 3] = v4.w;
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < Q; kk++) ev[kk] = 0.f;
+        }
+        float d = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < Q; kk++) d = fmaf(ev[kk], uq[kk], d);
+        d += __shfl_xor(d, 16);
+        d += __shfl_xor(d, 32);
+        if (g == 0 && c < m) ws[c] = d;
+#pragma unroll
+        for (int kk = 0; kk < Q; kk++) zacc[kk] = fmaf(d, ev[kk], zacc[kk]);
+    }
+    tile_fold_z<Q>(zacc, zpart_w, lane);
+}
+
+__device__ __forceinline__ float block_sum8(float v, float *red)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < 8; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+template <int Q>
+__global__ __launch_bounds__(512, 1) void sweep_tile_kernel(
+    const float *__restrict__ Y, int dim, float *__restrict__ D, int K, int kmax, float *__restrict__ coef,
+    const int *__restrict__ uoff, const int *__restrict__ users, const int *__restrict__ uslot, int cap,
+    const int *__restrict__ moff, const int *__restrict__ tab_p, const int *__restrict__ tab_self,
+    const int2 *__restrict__ tab_oth, float *__restrict__ Eg, float *__restrict__ ws, int lds_cols, int mch,
+    int maxit, float tol2, unsigned long long *__restrict__ counters)
+{
+    constexpr int LDE = 4 * Q;
+    extern __shared__ float sm[];
+    float *u = sm;                          // LDE
+    float *zpart = u + LDE;                 // 8 x LDE
+    float *red = zpart + 8 * LDE;           // 16
+    float *Es = red + 16;                   // lds_cols x LDE
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    int *mp = (int *)(Es + (size_t)lds_cols * LDE);   // mch
+    int *ma = mp + mch;                                // mch x no
+    float *mc = (float *)(ma + (size_t)mch * no);      // mch x no
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    unsigned long long total_it = 0;
+
+    for (int k = 0; k < K; k++) {
+        const int base = uoff[k];
+        const int c = uoff[k + 1] - base;
+        if (c <= 0) continue;
+        const int m = (cap > 0 && c > cap) ? cap : c;
+        const int64_t cb = moff[k];
+        for (int e = tid; e < LDE; e += 512) u[e] = e < dim ? D[(int64_t)k * dim + e] : 0.f;
+        // ---- build E in chunks of mch columns --------------------------------
+        for (int c0 = 0; c0 < m; c0 += mch) {
+            const int nc = min(mch, m - c0);
+            for (int t = tid; t < nc; t += 512) {
+                const int64_t cl = cb + c0 + t;
+                mp[t] = tab_p[cl];
+                for (int i = 0; i < no; i++) {
+                    const int2 o = tab_oth[cl * no + i];
+                    ma[t * no + i] = o.x;
+                    mc[t * no + i] = o.x >= 0 ? coef[o.y] : 0.f;
+                }
+            }
+            __syncthreads();
+            const int ntile = (nc + 15) >> 4;
+            for (int tl = wave; tl < ntile; tl += 8) {
+                const int t = tl * 16 + col;         // column within the chunk
+                if (t < nc) {
+                    const int p = mp[t];
+                    float ev[Q];
+#pragma unroll
+                    for (int kk = 0; kk < Q; kk++) {
+                        const int e = g * Q + kk;
+                        ev[kk] = e < dim ? Y[(int64_t)p * dim + e] : 0.f;
+                    }
+                    for (int i = 0; i < no; i++) {
+                        const int a = ma[t * no + i];
+                        if (a < 0) continue;
+                        const float x = mc[t * no + i];
+#pragma unroll
+                        for (int kk = 0; kk < Q; kk++) {
+                            const int e = g * Q + kk;
+                            if (e < dim) ev[kk] = fmaf(-D[(int64_t)a * dim + e], x, ev[kk]);
+                        }
+                    }
+                    const int cc = c0 + t;
+                    float *dst = cc < lds_cols ? Es + (size_t)cc * LDE : Eg + (size_t)(cc - lds_cols) * LDE;
+#pragma unroll
+                    for (int kk = 0; kk < Q; kk += 4)
+                        *(float4 *)(dst + g * Q + kk) = make_float4(ev[kk], ev[kk + 1], ev[kk + 2], ev[kk + 3]);
+                }
+            }
+            __syncthreads();
+        }
+        tile_pass<Q>(Es, Eg, lds_cols, m, u, ws, zpart + wave * LDE);
+        int it = 0;
+        for (;;) {
+            __syncthreads();
+            float z = 0.f;
+            if (tid < dim)
+                for (int w = 0; w < 8; w++) z += zpart[w * LDE + tid];
+            const float nz = sqrtf(block_sum8(tid < dim ? z * z : 0.f, red));
+            const float un = (tid < dim && nz > 0.f) ? z / nz : 0.f;
+            const float du = tid < dim ? un - u[tid] : 0.f;
+            const float d2 = block_sum8(du * du, red);
+            it++;
+            if (nz == 0.f) break;
+            if (it >= 2 && d2 < tol2) break;
+            if (it > maxit) break;
+            if (tid < dim) u[tid] = un;
+            __syncthreads();
+            tile_pass<Q>(Es, Eg, lds_cols, m, u, ws, zpart + wave * LDE);
+        }
+        total_it += it;
+        __syncthreads();
+        for (int e = tid; e < dim; e += 512) D[(int64_t)k * dim + e] = u[e];
+        for (int t = tid; t < m; t += 512) coef[tab_self[cb + t]] = ws[t];
+        for (int t = m + tid; t < c; t += 512) {
+            const int pos = base + t;
+            coef[(int64_t)users[pos] * kmax + uslot[pos]] = 0.f;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) atomicAdd(&counters[CNT_EIG_IT], total_it);
+    (void)lane;
+}
+
 }  // namespace
 
 // Enqueue one sweep: user lists, then per atom build -> Gram -> eigenvector ->
@@ -580,6 +767,35 @@ static int enqueue_update(ksvd_ctx *ctx)
                            (const int *)ctx->idx.ptr, (const int *)ctx->cnt.ptr, (const int *)ctx->uoff.ptr,
                            (const int *)moff, (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr,
                            (int *)ctx->tab_p.ptr, (int *)ctx->tab_self.ptr, (int2 *)ctx->tab_oth.ptr);
+        const int no_t = kmax > 1 ? kmax - 1 : 1;
+        const size_t meta_col = sizeof(int) + (sizeof(int) + sizeof(float)) * (size_t)no_t;
+        const int mch_t = (int)std::min<size_t>(1024, (40 * 1024) / meta_col);
+#define KSVD_TILE(QQ)                                                                                          \
+    do {                                                                                                       \
+        constexpr int LDE = 4 * QQ;                                                                            \
+        const size_t fixed_t = sizeof(float) * ((size_t)LDE * 9 + 16);                                         \
+        int lc = (int)((156 * 1024 - fixed_t - meta_col * mch_t) / (sizeof(float) * LDE));                     \
+        lc &= ~15;                                                                                             \
+        const size_t lds_t = fixed_t + sizeof(float) * (size_t)lc * LDE + meta_col * mch_t;                    \
+        auto kern = sweep_tile_kernel<QQ>;                                                                     \
+        KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                (int)lds_t));                                                  \
+        hipLaunchKernelGGL(kern, dim3(1), dim3(512), lds_t, ctx->stream, (const float *)ctx->Y.ptr, dim,       \
+                           (float *)ctx->D.ptr, K, kmax, (float *)ctx->coef.ptr, (const int *)ctx->uoff.ptr,   \
+                           (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr, cap, (const int *)moff,   \
+                           (const int *)ctx->tab_p.ptr, (const int *)ctx->tab_self.ptr,                        \
+                           (const int2 *)ctx->tab_oth.ptr, E, (float *)ctx->wcol.ptr, lc, mch_t, 300,          \
+                           tol * tol, (unsigned long long *)ctx->counters.ptr);                                \
+        KSVD_HIP_CHECK(ctx, hipGetLastError());                                                                \
+        return KSVD_OK;                                                                                        \
+    } while (0)
+        const char *nt = getenv("KSVD_SWEEP_ELEMENT");
+        if (!(nt && nt[0] && nt[0] != '0')) {
+            if (dim <= 64) KSVD_TILE(16);
+            if (dim <= 144) KSVD_TILE(36);
+            if (dim <= 192) KSVD_TILE(48);
+        }
+#undef KSVD_TILE
         // LDS: u, 16 partial z, reductions | E columns kept on chip | column
         // metadata of up to mch columns (patch, other atoms, coefficients)
         const int no = kmax > 1 ? kmax - 1 : 1;
