@@ -3,6 +3,7 @@
 // launches on the context's stream, HIP-event timing of every phase.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -232,6 +233,7 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
     if (rc) return rc;
     rc = ksvd_reserve(ctx, ctx->rlam, 16);
     if (rc) return rc;
+
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return KSVD_OK;
 }
@@ -247,7 +249,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
     drop_update_graph(ctx);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
-                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->wcol, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth,    &ctx->rsum,    &ctx->rweight,
+                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
                       &ctx->rout, &ctx->rpart,  &ctx->rlam};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
@@ -296,11 +298,14 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->uslot, sizeof(int) * (size_t)P * kmax},
         // E columns: pitch up to 192 floats (the tile sweep pads columns to 4Q)
         {&ctx->E, sizeof(float) * (size_t)std::max(dim, 192) * ecols},
-        {&ctx->wcol, sizeof(float) * (size_t)(p.user_cap > 0 ? p.user_cap : P)},
         {&ctx->moff, sizeof(int) * (size_t)(K + 1)},
         {&ctx->tab_p, sizeof(int) * (size_t)tcols},
         {&ctx->tab_self, sizeof(int) * (size_t)tcols},
-        {&ctx->tab_oth, sizeof(int) * 2 * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
+        {&ctx->tab_oth, sizeof(int) * 4 * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
+        {&ctx->srank, sizeof(int) * (size_t)P * kmax},
+        {&ctx->E2, sizeof(float) * (size_t)std::max(dim, 192) * ecols},
+        {&ctx->gpart, sweep_gpart_bytes(dim)},
+        {&ctx->ticket, 256},
         {&ctx->rsum, sizeof(float) * (size_t)N},
         {&ctx->rweight, sizeof(float) * (size_t)N},
         {&ctx->rout, sizeof(float) * (size_t)N},

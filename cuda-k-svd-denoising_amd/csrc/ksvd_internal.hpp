@@ -43,8 +43,9 @@ struct ksvd_ctx {
 
     // device buffers
     DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
-    DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf, wcol;
+    DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
     DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
+    DevBuf srank, E2, gpart, ticket;        // per-atom sweep kernels (sweep.hip)
     DevBuf rsum, rweight, rout, rpart, rlam;
 
     ksvd_timings tm{};
@@ -74,4 +75,5 @@ int launch_dict_gram(ksvd_ctx *ctx);
 int launch_omp(ksvd_ctx *ctx);
 int launch_update(ksvd_ctx *ctx);
 void drop_update_graph(ksvd_ctx *ctx);
+size_t sweep_gpart_bytes(int dim);
 int launch_rebuild(ksvd_ctx *ctx, float sigma2);

@@ -1,0 +1,344 @@
+// The Gauss-Seidel dictionary sweep (cuda_svd_denoiser.cu:309-361) as two
+// kernels per atom, spread over many CUs, with no grid-wide barrier.
+//
+// K1(k), 16 workgroups x 1024 threads:
+//   (a) the new coefficients of atom k-1 for its users: x_t = e_t . d_{k-1}
+//       for its first min(c, cap) users, 0 for the rest;
+//   (b) the restricted residual E_k of atom k, a slice of columns per
+//       workgroup: e_t = y_p - sum_{j != k} d_j x_{j,p}.  Atom k-1's
+//       coefficient for p is not read from memory (part (a) of this same
+//       launch is writing it) but recomputed as e_{k-1,r} . d_{k-1} from the
+//       stored E_{k-1} (r = rank of p among atom k-1's users), with the same
+//       code as (a), so both agree bit for bit;
+//   (c) the partial Gram E_slice E_slice^T of the slice on MFMA
+//       (v_mfma_f32_16x16x4_f32), upper 16x16 tiles only.
+// K2(k), one workgroup per upper Gram tile: sums the partial tiles in a fixed
+//   order into G = E_k E_k^T; the last workgroup to finish (ticket counter)
+//   loads G into LDS and takes its top eigenvector by power iteration from
+//   d_k, which becomes the new atom d_k.
+// A final K1(K) writes atom K-1's coefficients.  E_k alternates between two
+// buffers so that K1(k+1) reads E_k while writing E_{k+1}.
+#include <algorithm>
+
+#include "ksvd_internal.hpp"
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int B1 = 16;      // K1 workgroups
+constexpr int T1 = 1024;    // K1 threads
+constexpr int CHUNK = 64;   // E columns per K1 chunk (held in LDS)
+constexpr int T2 = 256;     // K2 threads
+constexpr int MAX_TPW = 5;  // Gram tiles per wavefront in K1 (16 waves -> 80 tiles, dpad <= 192)
+
+struct SweepArgs {
+    const float *Y;
+    float *D;
+    float *coef;
+    const int *uoff;
+    const int *users;
+    const int *uslot;
+    const int *moff;
+    const int *tab_p;
+    const int *tab_self;
+    const int4 *tab_oth;
+    float *E0, *E1;
+    float *Gpart;
+    float *G;
+    unsigned *ticket;
+    unsigned long long *counters;
+    int dim, dpad, K, kmax, cap;
+    int nbt, ntiles;
+    int maxit;
+    float tol2;
+};
+
+__device__ __forceinline__ int atom_m(const int *uoff, int k, int cap, int *c_out)
+{
+    const int c = uoff[k + 1] - uoff[k];
+    *c_out = c;
+    return (cap > 0 && c > cap) ? cap : c;
+}
+
+// e . u in element order (one thread).  Used for the stored coefficients
+// and for the recomputed ones, which must agree exactly.
+__device__ __forceinline__ float col_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
+{
+    float s = 0.f;
+    for (int q = 0; q < dim; q++) s = fmaf(e[q], u[q], s);
+    return s;
+}
+
+// upper tile index -> (ba, bb), ba <= bb, row-major over the upper triangle
+__device__ __forceinline__ void tile_coords(int ti, int nbt, int *ba, int *bb)
+{
+    int a = 0, rem = ti;
+    while (rem >= nbt - a) {
+        rem -= nbt - a;
+        a++;
+    }
+    *ba = a;
+    *bb = a + rem;
+}
+
+__global__ void sweep_reset_kernel(unsigned *ticket) { *ticket = 0u; }
+
+__global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
+{
+    extern __shared__ float sm[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int dim = a.dim, dpad = a.dpad, kmax = a.kmax;
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    // ---- (a) coefficients of atom k-1 --------------------------------------
+    int cprev = 0, mprev = 0;
+    const float *Eprev = nullptr, *uprev = nullptr;
+    if (k >= 1) {
+        mprev = atom_m(a.uoff, k - 1, a.cap, &cprev);
+        Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
+        uprev = a.D + (int64_t)(k - 1) * dim;
+        if (cprev > 0) {
+            const int64_t cbp = a.moff[k - 1];
+            const int basep = a.uoff[k - 1];
+            for (int t = blockIdx.x * T1 + tid; t < cprev; t += gridDim.x * T1) {
+                if (t < mprev) {
+                    a.coef[a.tab_self[cbp + t]] = col_dot(Eprev + (int64_t)t * dpad, uprev, dim);
+                } else {
+                    const int pos = basep + t;
+                    a.coef[(int64_t)a.users[pos] * kmax + a.uslot[pos]] = 0.f;
+                }
+            }
+        }
+    }
+    if (k >= a.K) return;
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    if (m == 0) return;
+    const int cw = (m + B1 - 1) / B1;
+    const int t_begin = blockIdx.x * cw;
+    const int t_end = min(m, t_begin + cw);
+    if (t_begin >= t_end) return;
+    float *Ecur = (k & 1) ? a.E1 : a.E0;
+    const int64_t cb = a.moff[k];
+
+    float *Es = sm;                                   // CHUNK x dpad
+    int *mp = (int *)(Es + CHUNK * dpad);             // CHUNK
+    int *ma = mp + CHUNK;                             // CHUNK x no
+    float *mc = (float *)(ma + CHUNK * no);           // CHUNK x no
+
+    // MFMA accumulators of this wavefront's Gram tiles, kept across chunks
+    f4 acc[MAX_TPW];
+#pragma unroll
+    for (int q = 0; q < MAX_TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+    const int cpt = T1 / dim;
+    const int tc = tid / dim, te = tid - tc * dim;
+
+    for (int c0 = t_begin; c0 < t_end; c0 += CHUNK) {
+        const int nc = min(CHUNK, t_end - c0);
+        for (int q = tid; q < CHUNK * dpad; q += T1) Es[q] = 0.f;
+        // metadata: patch, other atoms, their coefficients (atom k-1's recomputed)
+        if (tid < nc) {
+            const int64_t col = cb + c0 + tid;
+            mp[tid] = a.tab_p[col];
+            for (int i = 0; i < no; i++) {
+                const int4 o = a.tab_oth[col * no + i];
+                float x = 0.f;
+                if (o.x >= 0) {
+                    if (o.x == k - 1 && o.z >= 0)
+                        x = o.z < mprev ? col_dot(Eprev + (int64_t)o.z * dpad, uprev, dim) : 0.f;
+                    else
+                        x = a.coef[o.y];
+                }
+                ma[tid * no + i] = o.x;
+                mc[tid * no + i] = x;
+            }
+        }
+        __syncthreads();
+        // element-parallel build: thread (tc, te), 8 columns in flight
+        constexpr int BU = 8;
+        if (tc < cpt) {
+            for (int t = tc; t < nc; t += BU * cpt) {
+                float v[BU];
+#pragma unroll
+                for (int b = 0; b < BU; b++) {
+                    const int tt = t + b * cpt;
+                    v[b] = tt < nc ? a.Y[(int64_t)mp[tt] * dim + te] : 0.f;
+                }
+                for (int i = 0; i < no; i++) {
+                    float dv[BU];
+#pragma unroll
+                    for (int b = 0; b < BU; b++) {
+                        const int tt = t + b * cpt;
+                        const int at = tt < nc ? ma[tt * no + i] : -1;
+                        dv[b] = at >= 0 ? a.D[(int64_t)at * dim + te] : 0.f;
+                    }
+#pragma unroll
+                    for (int b = 0; b < BU; b++) {
+                        const int tt = t + b * cpt;
+                        if (tt < nc) v[b] = fmaf(-dv[b], mc[tt * no + i], v[b]);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < BU; b++) {
+                    const int tt = t + b * cpt;
+                    if (tt < nc) {
+                        Es[tt * dpad + te] = v[b];
+                        Ecur[(int64_t)(c0 + tt) * dpad + te] = v[b];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // partial Gram of the chunk: tile (ba, bb) += E_chunk[ba] E_chunk[bb]^T
+        const int col16 = lane & 15, grp = lane >> 4;
+#pragma unroll
+        for (int q = 0; q < MAX_TPW; q++) {
+            const int ti = wave + 16 * q;
+            if (ti < a.ntiles) {
+                int ba, bb;
+                tile_coords(ti, a.nbt, &ba, &bb);
+                for (int kk = 0; kk < CHUNK / 4; kk++) {
+                    const int t = 4 * kk + grp;
+                    const float av = Es[t * dpad + ba * 16 + col16];
+                    const float bv = Es[t * dpad + bb * 16 + col16];
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // write this workgroup's partial tiles
+    float *gp = a.Gpart + (size_t)blockIdx.x * a.ntiles * 256;
+#pragma unroll
+    for (int q = 0; q < MAX_TPW; q++) {
+        const int ti = wave + 16 * q;
+        if (ti < a.ntiles)
+            *(f4 *)(gp + (size_t)ti * 256 + lane * 4) = acc[q];
+    }
+}
+
+__device__ __forceinline__ float block_sum_t2(float v, float *red)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const float s = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
+{
+    extern __shared__ float sm[];
+    __shared__ float red[4];
+    __shared__ int is_last;
+    const int tid = threadIdx.x;
+    const int dim = a.dim;
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    if (m == 0) return;
+    const int cw = (m + B1 - 1) / B1;
+    const int nb_used = (m + cw - 1) / cw;
+    // ---- sum the partial tiles in workgroup order ----------------------------
+    const int ti = blockIdx.x;
+    int ba, bb;
+    tile_coords(ti, a.nbt, &ba, &bb);
+    float s = 0.f;
+    for (int b = 0; b < nb_used; b++) s += a.Gpart[((size_t)b * a.ntiles + ti) * 256 + tid];
+    {
+        const int l = tid >> 2, i = tid & 3;           // MFMA output layout
+        const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
+        if (ga < dim && gb < dim) {
+            a.G[(size_t)ga * dim + gb] = s;
+            a.G[(size_t)gb * dim + ga] = s;
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned t = atomicAdd(a.ticket, 1u);
+        is_last = ((t + 1u) % (unsigned)a.ntiles) == 0u;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    // ---- last workgroup: top eigenvector of G by power iteration --------------
+    float *Gs = sm;              // dim x dim (symmetric: column t read as row t)
+    float *u = Gs + dim * dim;   // dim
+    for (int q = tid; q < dim * dim; q += T2) Gs[q] = a.G[q];
+    for (int e = tid; e < dim; e += T2) u[e] = a.D[(int64_t)k * dim + e];
+    __syncthreads();
+    int it = 0;
+    for (;;) {
+        float z = 0.f;
+        if (tid < dim)
+            for (int q = 0; q < dim; q++) z = fmaf(Gs[q * dim + tid], u[q], z);
+        const float nz = sqrtf(block_sum_t2(tid < dim ? z * z : 0.f, red));
+        if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
+        const float un = tid < dim ? z / nz : 0.f;
+        const float du = tid < dim ? un - u[tid] : 0.f;
+        const float d2 = block_sum_t2(du * du, red);
+        if (tid < dim) u[tid] = un;
+        __syncthreads();
+        it++;
+        if ((it >= 2 && d2 < a.tol2) || it >= a.maxit) break;
+    }
+    for (int e = tid; e < dim; e += T2) a.D[(int64_t)k * dim + e] = u[e];
+    if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
+}
+
+}  // namespace
+
+int launch_sweep(ksvd_ctx *ctx, float tol)
+{
+    const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
+    const int dpad = (dim + 15) / 16 * 16;
+    const int nbt = dpad / 16;
+    const int ntiles = nbt * (nbt + 1) / 2;
+    if (ntiles > 16 * MAX_TPW) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension too large for the sweep");
+    SweepArgs a;
+    a.Y = (const float *)ctx->Y.ptr;
+    a.D = (float *)ctx->D.ptr;
+    a.coef = (float *)ctx->coef.ptr;
+    a.uoff = (const int *)ctx->uoff.ptr;
+    a.users = (const int *)ctx->users.ptr;
+    a.uslot = (const int *)ctx->uslot.ptr;
+    a.moff = (const int *)ctx->moff.ptr;
+    a.tab_p = (const int *)ctx->tab_p.ptr;
+    a.tab_self = (const int *)ctx->tab_self.ptr;
+    a.tab_oth = (const int4 *)ctx->tab_oth.ptr;
+    a.E0 = (float *)ctx->E.ptr;
+    a.E1 = (float *)ctx->E2.ptr;
+    a.Gpart = (float *)ctx->gpart.ptr;
+    a.G = (float *)ctx->wbuf.ptr;
+    a.ticket = (unsigned *)ctx->ticket.ptr;
+    a.counters = (unsigned long long *)ctx->counters.ptr;
+    a.dim = dim;
+    a.dpad = dpad;
+    a.K = K;
+    a.kmax = kmax;
+    a.cap = ctx->prm.user_cap;
+    a.nbt = nbt;
+    a.ntiles = ntiles;
+    a.maxit = 300;
+    a.tol2 = tol * tol;
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad + sizeof(int) * CHUNK + (sizeof(int) + sizeof(float)) * CHUNK * no;
+    const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim);
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, a.ticket);
+    for (int k = 0; k <= K; k++) {
+        hipLaunchKernelGGL(k1_kernel, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(ntiles), dim3(T2), lds2, ctx->stream, a, k);
+    }
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+size_t sweep_gpart_bytes(int dim)
+{
+    const int nbt = (dim + 15) / 16;
+    return sizeof(float) * (size_t)B1 * (nbt * (nbt + 1) / 2) * 256;
+}
