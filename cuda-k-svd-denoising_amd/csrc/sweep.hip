@@ -29,7 +29,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int B1 = 16;      // K1 workgroups
 constexpr int T1 = 1024;    // K1 threads
 constexpr int CHUNK = 64;   // E columns per K1 chunk (held in LDS)
-constexpr int T2 = 256;     // K2 threads
+constexpr int T2 = 576;     // K2 threads (9 wavefronts: 4 lanes per row of a 144-row Gram)
 constexpr int MAX_TPW = 5;  // Gram tiles per wavefront in K1 (16 waves -> 80 tiles, dpad <= 192)
 
 struct SweepArgs {
@@ -61,12 +61,26 @@ __device__ __forceinline__ int atom_m(const int *uoff, int k, int cap, int *c_ou
     return (cap > 0 && c > cap) ? cap : c;
 }
 
-// e . u in element order (one thread).  Used for the stored coefficients
-// and for the recomputed ones, which must agree exactly.
-__device__ __forceinline__ float col_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
+// e . u by one wavefront: lane l sums elements l, l+64, l+128, ... in order,
+// then a fixed butterfly.  Used for the stored coefficients and for the
+// recomputed ones, which therefore agree exactly.
+__device__ __forceinline__ float wave_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
 {
+    const int lane = threadIdx.x & 63;
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = lane + 64 * j;
+        v[j] = q < dim ? e[q] : 0.f;
+    }
     float s = 0.f;
-    for (int q = 0; q < dim; q++) s = fmaf(e[q], u[q], s);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = lane + 64 * j;
+        if (q < dim) s = fmaf(v[j], u[q], s);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     return s;
 }
 
@@ -84,6 +98,7 @@ __device__ __forceinline__ void tile_coords(int ti, int nbt, int *ba, int *bb)
 
 __global__ void sweep_reset_kernel(unsigned *ticket) { *ticket = 0u; }
 
+template <int TPW>
 __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
 {
     extern __shared__ float sm[];
@@ -100,13 +115,14 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
         if (cprev > 0) {
             const int64_t cbp = a.moff[k - 1];
             const int basep = a.uoff[k - 1];
-            for (int t = blockIdx.x * T1 + tid; t < cprev; t += gridDim.x * T1) {
-                if (t < mprev) {
-                    a.coef[a.tab_self[cbp + t]] = col_dot(Eprev + (int64_t)t * dpad, uprev, dim);
-                } else {
-                    const int pos = basep + t;
-                    a.coef[(int64_t)a.users[pos] * kmax + a.uslot[pos]] = 0.f;
-                }
+            const int nwaves = gridDim.x * (T1 / 64);
+            for (int t = blockIdx.x * (T1 / 64) + wave; t < mprev; t += nwaves) {
+                const float x = wave_dot(Eprev + (int64_t)t * dpad, uprev, dim);
+                if (lane == 0) a.coef[a.tab_self[cbp + t]] = x;
+            }
+            for (int t = mprev + blockIdx.x * T1 + tid; t < cprev; t += gridDim.x * T1) {
+                const int pos = basep + t;
+                a.coef[(int64_t)a.users[pos] * kmax + a.uslot[pos]] = 0.f;
             }
         }
     }
@@ -125,32 +141,45 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
     int *mp = (int *)(Es + CHUNK * dpad);             // CHUNK
     int *ma = mp + CHUNK;                             // CHUNK x no
     float *mc = (float *)(ma + CHUNK * no);           // CHUNK x no
+    int *mr = (int *)(mc + CHUNK * no);               // CHUNK x no: rank to recompute, or -1
 
     // MFMA accumulators of this wavefront's Gram tiles, kept across chunks
-    f4 acc[MAX_TPW];
+    f4 acc[TPW];
 #pragma unroll
-    for (int q = 0; q < MAX_TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
     const int cpt = T1 / dim;
     const int tc = tid / dim, te = tid - tc * dim;
 
     for (int c0 = t_begin; c0 < t_end; c0 += CHUNK) {
         const int nc = min(CHUNK, t_end - c0);
         for (int q = tid; q < CHUNK * dpad; q += T1) Es[q] = 0.f;
-        // metadata: patch, other atoms, their coefficients (atom k-1's recomputed)
+        // metadata: patch, other atoms, their coefficients.  Atom k-1's is
+        // recomputed (entry flagged with its rank, then one wavefront each).
         if (tid < nc) {
             const int64_t col = cb + c0 + tid;
             mp[tid] = a.tab_p[col];
             for (int i = 0; i < no; i++) {
                 const int4 o = a.tab_oth[col * no + i];
                 float x = 0.f;
+                int lazy = -1;
                 if (o.x >= 0) {
-                    if (o.x == k - 1 && o.z >= 0)
-                        x = o.z < mprev ? col_dot(Eprev + (int64_t)o.z * dpad, uprev, dim) : 0.f;
-                    else
+                    if (o.x == k - 1 && o.z >= 0) {
+                        if (o.z < mprev) lazy = o.z;
+                    } else {
                         x = a.coef[o.y];
+                    }
                 }
                 ma[tid * no + i] = o.x;
                 mc[tid * no + i] = x;
+                mr[tid * no + i] = lazy;
+            }
+        }
+        __syncthreads();
+        for (int q = wave; q < nc * no; q += T1 / 64) {
+            const int r = mr[q];
+            if (r >= 0) {
+                const float x = wave_dot(Eprev + (int64_t)r * dpad, uprev, dim);
+                if (lane == 0) mc[q] = x;
             }
         }
         __syncthreads();
@@ -192,7 +221,7 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
         // partial Gram of the chunk: tile (ba, bb) += E_chunk[ba] E_chunk[bb]^T
         const int col16 = lane & 15, grp = lane >> 4;
 #pragma unroll
-        for (int q = 0; q < MAX_TPW; q++) {
+        for (int q = 0; q < TPW; q++) {
             const int ti = wave + 16 * q;
             if (ti < a.ntiles) {
                 int ba, bb;
@@ -210,7 +239,7 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
     // write this workgroup's partial tiles
     float *gp = a.Gpart + (size_t)blockIdx.x * a.ntiles * 256;
 #pragma unroll
-    for (int q = 0; q < MAX_TPW; q++) {
+    for (int q = 0; q < TPW; q++) {
         const int ti = wave + 16 * q;
         if (ti < a.ntiles)
             *(f4 *)(gp + (size_t)ti * 256 + lane * 4) = acc[q];
@@ -223,7 +252,9 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
     for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    const float s = (red[0] + red[1]) + (red[2] + red[3]);
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < T2 / 64; w++) s += red[w];
     __syncthreads();
     return s;
 }
@@ -231,7 +262,7 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
 __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 {
     extern __shared__ float sm[];
-    __shared__ float red[4];
+    __shared__ float red[T2 / 64];
     __shared__ int is_last;
     const int tid = threadIdx.x;
     const int dim = a.dim;
@@ -244,9 +275,13 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     const int ti = blockIdx.x;
     int ba, bb;
     tile_coords(ti, a.nbt, &ba, &bb);
-    float s = 0.f;
-    for (int b = 0; b < nb_used; b++) s += a.Gpart[((size_t)b * a.ntiles + ti) * 256 + tid];
-    {
+    if (tid < 256) {
+        float pv[B1];
+#pragma unroll
+        for (int b = 0; b < B1; b++) pv[b] = b < nb_used ? a.Gpart[((size_t)b * a.ntiles + ti) * 256 + tid] : 0.f;
+        float s = 0.f;
+#pragma unroll
+        for (int b = 0; b < B1; b++) s += pv[b];
         const int l = tid >> 2, i = tid & 3;           // MFMA output layout
         const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
         if (ga < dim && gb < dim) {
@@ -266,20 +301,41 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     // ---- last workgroup: top eigenvector of G by power iteration --------------
     float *Gs = sm;              // dim x dim (symmetric: column t read as row t)
     float *u = Gs + dim * dim;   // dim
-    for (int q = tid; q < dim * dim; q += T2) Gs[q] = a.G[q];
+    {
+        // all of G in flight at once: up to 12 float4 per thread
+        const int n4 = (dim * dim) / 4;
+        float4 gv[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            const int q = tid + j * T2;
+            gv[j] = q < n4 ? ((const float4 *)a.G)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            const int q = tid + j * T2;
+            if (q < n4) ((float4 *)Gs)[q] = gv[j];
+        }
+        for (int q = 4 * n4 + tid; q < dim * dim; q += T2) Gs[q] = a.G[q];
+    }
     for (int e = tid; e < dim; e += T2) u[e] = a.D[(int64_t)k * dim + e];
     __syncthreads();
+    const int row = tid >> 2, part = tid & 3;     // 4 lanes per row
+    const int qlen = (dim + 3) / 4;
+    const int q0 = part * qlen, q1 = min(dim, q0 + qlen);
     int it = 0;
     for (;;) {
         float z = 0.f;
-        if (tid < dim)
-            for (int q = 0; q < dim; q++) z = fmaf(Gs[q * dim + tid], u[q], z);
-        const float nz = sqrtf(block_sum_t2(tid < dim ? z * z : 0.f, red));
+        if (row < dim)
+            for (int q = q0; q < q1; q++) z = fmaf(Gs[q * dim + row], u[q], z);
+        z += __shfl_xor(z, 1);
+        z += __shfl_xor(z, 2);
+        const bool owner = row < dim && part == 0;
+        const float nz = sqrtf(block_sum_t2(owner ? z * z : 0.f, red));
         if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
-        const float un = tid < dim ? z / nz : 0.f;
-        const float du = tid < dim ? un - u[tid] : 0.f;
+        const float un = owner ? z / nz : 0.f;
+        const float du = owner ? un - u[row] : 0.f;
         const float d2 = block_sum_t2(du * du, red);
-        if (tid < dim) u[tid] = un;
+        if (owner) u[row] = un;
         __syncthreads();
         it++;
         if ((it >= 2 && d2 < a.tol2) || it >= a.maxit) break;
@@ -324,13 +380,19 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.maxit = 300;
     a.tol2 = tol * tol;
     const int no = kmax > 1 ? kmax - 1 : 1;
-    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad + sizeof(int) * CHUNK + (sizeof(int) + sizeof(float)) * CHUNK * no;
+    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad + sizeof(int) * CHUNK + (2 * sizeof(int) + sizeof(float)) * CHUNK * no;
     const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim);
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    const int tpw = (ntiles + 15) / 16;
+    const void *k1 = tpw <= 1 ? (const void *)k1_kernel<1> : tpw <= 2 ? (const void *)k1_kernel<2>
+                   : tpw <= 3 ? (const void *)k1_kernel<3> : (const void *)k1_kernel<5>;
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
     KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
     hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, a.ticket);
     for (int k = 0; k <= K; k++) {
-        hipLaunchKernelGGL(k1_kernel, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        if (tpw <= 1) hipLaunchKernelGGL(k1_kernel<1>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        else if (tpw <= 2) hipLaunchKernelGGL(k1_kernel<2>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        else if (tpw <= 3) hipLaunchKernelGGL(k1_kernel<3>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        else hipLaunchKernelGGL(k1_kernel<5>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(ntiles), dim3(T2), lds2, ctx->stream, a, k);
     }
     KSVD_HIP_CHECK(ctx, hipGetLastError());
