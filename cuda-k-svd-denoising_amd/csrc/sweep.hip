@@ -1,7 +1,8 @@
 // The Gauss-Seidel dictionary sweep (cuda_svd_denoiser.cu:309-361) as two
-// kernels per atom, spread over many CUs, with no grid-wide barrier.
+// kernels per atom, spread over CUs, with no grid-wide barrier (a kernel
+// boundary is the only cross-workgroup synchronisation).
 //
-// K1(k), 16 workgroups x 1024 threads:
+// K1(k), B1 workgroups x 1024 threads:
 //   (a) the new coefficients of atom k-1 for its users: x_t = e_t . d_{k-1}
 //       for its first min(c, cap) users, 0 for the rest;
 //   (b) the restricted residual E_k of atom k, a slice of columns per
@@ -12,10 +13,9 @@
 //       code as (a), so both agree bit for bit;
 //   (c) the partial Gram E_slice E_slice^T of the slice on MFMA
 //       (v_mfma_f32_16x16x4_f32), upper 16x16 tiles only.
-// K2(k), one workgroup per upper Gram tile: sums the partial tiles in a fixed
-//   order into G = E_k E_k^T; the last workgroup to finish (ticket counter)
-//   loads G into LDS and takes its top eigenvector by power iteration from
-//   d_k, which becomes the new atom d_k.
+// K2(k), one workgroup x 1024 threads: sums the B1 partial Grams in a fixed
+//   order straight into LDS (G = E_k E_k^T) and takes its top eigenvector by
+//   power iteration from d_k; that vector becomes the new atom d_k.
 // A final K1(K) writes atom K-1's coefficients.  E_k alternates between two
 // buffers so that K1(k+1) reads E_k while writing E_{k+1}.
 #include <algorithm>
@@ -26,10 +26,10 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int B1 = 16;      // K1 workgroups
+constexpr int B1 = 8;       // K1 workgroups
 constexpr int T1 = 1024;    // K1 threads
 constexpr int CHUNK = 64;   // E columns per K1 chunk (held in LDS)
-constexpr int T2 = 576;     // K2 threads (9 wavefronts: 4 lanes per row of a 144-row Gram)
+constexpr int T2 = 1024;    // K2 threads (4 lanes per row of the Gram)
 constexpr int MAX_TPW = 5;  // Gram tiles per wavefront in K1 (16 waves -> 80 tiles, dpad <= 192)
 
 struct SweepArgs {
@@ -46,7 +46,6 @@ struct SweepArgs {
     float *E0, *E1;
     float *Gpart;
     float *G;
-    unsigned *ticket;
     unsigned long long *counters;
     int dim, dpad, K, kmax, cap;
     int nbt, ntiles;
@@ -96,8 +95,6 @@ __device__ __forceinline__ void tile_coords(int ti, int nbt, int *ba, int *bb)
     *bb = a + rem;
 }
 
-__global__ void sweep_reset_kernel(unsigned *ticket) { *ticket = 0u; }
-
 template <int TPW>
 __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
 {
@@ -115,10 +112,39 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
         if (cprev > 0) {
             const int64_t cbp = a.moff[k - 1];
             const int basep = a.uoff[k - 1];
-            const int nwaves = gridDim.x * (T1 / 64);
-            for (int t = blockIdx.x * (T1 / 64) + wave; t < mprev; t += nwaves) {
-                const float x = wave_dot(Eprev + (int64_t)t * dpad, uprev, dim);
-                if (lane == 0) a.coef[a.tab_self[cbp + t]] = x;
+            // wavefront w of workgroup b takes columns t = g*W + w for its
+            // share of g; the columns' loads are issued before any use
+            const int W = gridDim.x * (T1 / 64);
+            const int w0 = blockIdx.x * (T1 / 64) + wave;
+            constexpr int CPW = 8;
+            for (int g0 = 0; g0 * W < mprev; g0 += CPW) {
+                float ev[CPW][4], uv[4];
+                int slot[CPW];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int q = lane + 64 * j;
+                    uv[j] = q < dim ? uprev[q] : 0.f;
+                }
+#pragma unroll
+                for (int g = 0; g < CPW; g++) {
+                    const int t = (g0 + g) * W + w0;
+                    slot[g] = t < mprev ? a.tab_self[cbp + t] : -1;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int q = lane + 64 * j;
+                        ev[g][j] = (t < mprev && q < dim) ? Eprev[(int64_t)t * dpad + q] : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < CPW; g++) {
+                    float sdot = 0.f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (lane + 64 * j < dim) sdot = fmaf(ev[g][j], uv[j], sdot);
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) sdot += __shfl_xor(sdot, off);
+                    if (lane == 0 && slot[g] >= 0) a.coef[slot[g]] = sdot;
+                }
             }
             for (int t = mprev + blockIdx.x * T1 + tid; t < cprev; t += gridDim.x * T1) {
                 const int pos = basep + t;
@@ -263,7 +289,6 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 {
     extern __shared__ float sm[];
     __shared__ float red[T2 / 64];
-    __shared__ int is_last;
     const int tid = threadIdx.x;
     const int dim = a.dim;
     int c;
@@ -271,55 +296,54 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     if (m == 0) return;
     const int cw = (m + B1 - 1) / B1;
     const int nb_used = (m + cw - 1) / cw;
-    // ---- sum the partial tiles in workgroup order ----------------------------
-    const int ti = blockIdx.x;
-    int ba, bb;
-    tile_coords(ti, a.nbt, &ba, &bb);
-    if (tid < 256) {
-        float pv[B1];
-#pragma unroll
-        for (int b = 0; b < B1; b++) pv[b] = b < nb_used ? a.Gpart[((size_t)b * a.ntiles + ti) * 256 + tid] : 0.f;
-        float s = 0.f;
-#pragma unroll
-        for (int b = 0; b < B1; b++) s += pv[b];
-        const int l = tid >> 2, i = tid & 3;           // MFMA output layout
-        const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
-        if (ga < dim && gb < dim) {
-            a.G[(size_t)ga * dim + gb] = s;
-            a.G[(size_t)gb * dim + ga] = s;
-        }
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned t = atomicAdd(a.ticket, 1u);
-        is_last = ((t + 1u) % (unsigned)a.ntiles) == 0u;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // ---- last workgroup: top eigenvector of G by power iteration --------------
     float *Gs = sm;              // dim x dim (symmetric: column t read as row t)
     float *u = Gs + dim * dim;   // dim
-    {
-        // all of G in flight at once: up to 12 float4 per thread
-        const int n4 = (dim * dim) / 4;
-        float4 gv[12];
+    // ---- G = sum of the partial Grams (workgroup order), straight into LDS ----
+    // float4 f of tile ti holds lane l = f % 64 of the MFMA output: rows
+    // 4*(l/16) .. +3, column l % 16 of the tile
+    const int nf4 = a.ntiles * 64;
+    for (int f0 = 0; f0 < nf4; f0 += 2 * T2) {
+        float4 acc2[2];
 #pragma unroll
-        for (int j = 0; j < 12; j++) {
-            const int q = tid + j * T2;
-            gv[j] = q < n4 ? ((const float4 *)a.G)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int h = 0; h < 2; h++) {
+            const int f = f0 + h * T2 + tid;
+            float4 pv[B1];
+#pragma unroll
+            for (int b = 0; b < B1; b++)
+                pv[b] = (f < nf4 && b < nb_used) ? ((const float4 *)a.Gpart)[(size_t)b * nf4 + f]
+                                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int b = 0; b < B1; b++) {
+                sacc.x += pv[b].x;
+                sacc.y += pv[b].y;
+                sacc.z += pv[b].z;
+                sacc.w += pv[b].w;
+            }
+            acc2[h] = sacc;
         }
 #pragma unroll
-        for (int j = 0; j < 12; j++) {
-            const int q = tid + j * T2;
-            if (q < n4) ((float4 *)Gs)[q] = gv[j];
+        for (int h = 0; h < 2; h++) {
+            const int f = f0 + h * T2 + tid;
+            if (f >= nf4) continue;
+            const int ti = f >> 6, l = f & 63;
+            int ba, bb;
+            tile_coords(ti, a.nbt, &ba, &bb);
+            const float vals[4] = {acc2[h].x, acc2[h].y, acc2[h].z, acc2[h].w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
+                if (ga < dim && gb < dim) {
+                    Gs[ga * dim + gb] = vals[i];
+                    Gs[gb * dim + ga] = vals[i];
+                }
+            }
         }
-        for (int q = 4 * n4 + tid; q < dim * dim; q += T2) Gs[q] = a.G[q];
     }
     for (int e = tid; e < dim; e += T2) u[e] = a.D[(int64_t)k * dim + e];
     __syncthreads();
-    const int row = tid >> 2, part = tid & 3;     // 4 lanes per row
+    // ---- top eigenvector by power iteration: 4 lanes per row -----------------
+    const int row = tid >> 2, part = tid & 3;
     const int qlen = (dim + 3) / 4;
     const int q0 = part * qlen, q1 = min(dim, q0 + qlen);
     int it = 0;
@@ -368,7 +392,6 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.E1 = (float *)ctx->E2.ptr;
     a.Gpart = (float *)ctx->gpart.ptr;
     a.G = (float *)ctx->wbuf.ptr;
-    a.ticket = (unsigned *)ctx->ticket.ptr;
     a.counters = (unsigned long long *)ctx->counters.ptr;
     a.dim = dim;
     a.dpad = dpad;
@@ -387,13 +410,12 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                    : tpw <= 3 ? (const void *)k1_kernel<3> : (const void *)k1_kernel<5>;
     KSVD_HIP_CHECK(ctx, hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
     KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-    hipLaunchKernelGGL(sweep_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, a.ticket);
     for (int k = 0; k <= K; k++) {
         if (tpw <= 1) hipLaunchKernelGGL(k1_kernel<1>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         else if (tpw <= 2) hipLaunchKernelGGL(k1_kernel<2>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         else if (tpw <= 3) hipLaunchKernelGGL(k1_kernel<3>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         else hipLaunchKernelGGL(k1_kernel<5>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
-        if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(ntiles), dim3(T2), lds2, ctx->stream, a, k);
+        if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(1), dim3(T2), lds2, ctx->stream, a, k);
     }
     KSVD_HIP_CHECK(ctx, hipGetLastError());
     return KSVD_OK;
