@@ -112,6 +112,8 @@ int check_params(ksvd_ctx *ctx, const ksvd_params *p)
     if (p->atoms <= 0 || p->atoms > 1024) return ksvd_fail(ctx, KSVD_ERR_ARG, "atoms must be in [1, 1024]");
     if (p->omp_iter <= 0 || p->omp_iter > KSVD_MAX_SUPPORT)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "ompIter must be in [1, 16]");
+    if (p->omp_iter > 8)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the dictionary sweep handles ompIter <= 8");
     if (p->ksvd_iter < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "ksvdIter must be >= 0");
     if (p->user_cap < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "user_cap must be >= 0");
     const int dim = p->patch_w * p->patch_h;
@@ -249,7 +251,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
     drop_update_graph(ctx);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
-                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
+                      &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
                       &ctx->rout, &ctx->rpart,  &ctx->rlam};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
@@ -303,6 +305,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->tab_self, sizeof(int) * (size_t)tcols},
         {&ctx->tab_oth, sizeof(int) * 4 * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
         {&ctx->srank, sizeof(int) * (size_t)P * kmax},
+        {&ctx->handx, sizeof(float) * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
         {&ctx->E2, sizeof(float) * (size_t)std::max(dim, 192) * ecols},
         {&ctx->gpart, sweep_gpart_bytes(dim)},
         {&ctx->ticket, 256},

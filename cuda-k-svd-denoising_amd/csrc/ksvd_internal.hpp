@@ -45,7 +45,7 @@ struct ksvd_ctx {
     DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
     DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
     DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
-    DevBuf srank, E2, gpart, ticket;        // per-atom sweep kernels (sweep.hip)
+    DevBuf srank, handx, E2, gpart, ticket; // per-atom sweep kernels (sweep.hip)
     DevBuf rsum, rweight, rout, rpart, rlam;
 
     ksvd_timings tm{};

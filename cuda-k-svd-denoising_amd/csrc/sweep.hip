@@ -30,6 +30,7 @@ constexpr int B1 = 8;       // K1 workgroups
 constexpr int T1 = 1024;    // K1 threads
 constexpr int CHUNK = 64;   // E columns per K1 chunk (held in LDS)
 constexpr int T2 = 1024;    // K2 threads (4 lanes per row of the Gram)
+constexpr int HAND_MAX = 4096;  // handoff entries per atom held in LDS
 constexpr int MAX_TPW = 5;  // Gram tiles per wavefront in K1 (16 waves -> 80 tiles, dpad <= 192)
 
 struct SweepArgs {
@@ -45,6 +46,7 @@ struct SweepArgs {
     const int4 *tab_oth;
     float *E0, *E1;
     float *Gpart;
+    float *handx;  // atom k-1's new coefficient for the table entries of atom k's columns
     float *G;
     unsigned long long *counters;
     int dim, dpad, K, kmax, cap;
@@ -95,7 +97,7 @@ __device__ __forceinline__ void tile_coords(int ti, int nbt, int *ba, int *bb)
     *bb = a + rem;
 }
 
-template <int TPW>
+template <int TPW, int NO, int BU>
 __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
 {
     extern __shared__ float sm[];
@@ -105,6 +107,7 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
     // ---- (a) coefficients of atom k-1 --------------------------------------
     int cprev = 0, mprev = 0;
     const float *Eprev = nullptr, *uprev = nullptr;
+    (void)no;
     if (k >= 1) {
         mprev = atom_m(a.uoff, k - 1, a.cap, &cprev);
         Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
             // share of g; the columns' loads are issued before any use
             const int W = gridDim.x * (T1 / 64);
             const int w0 = blockIdx.x * (T1 / 64) + wave;
-            constexpr int CPW = 8;
+            constexpr int CPW = 4;
             for (int g0 = 0; g0 * W < mprev; g0 += CPW) {
                 float ev[CPW][4], uv[4];
                 int slot[CPW];
@@ -164,10 +167,6 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
     const int64_t cb = a.moff[k];
 
     float *Es = sm;                                   // CHUNK x dpad
-    int *mp = (int *)(Es + CHUNK * dpad);             // CHUNK
-    int *ma = mp + CHUNK;                             // CHUNK x no
-    float *mc = (float *)(ma + CHUNK * no);           // CHUNK x no
-    int *mr = (int *)(mc + CHUNK * no);               // CHUNK x no: rank to recompute, or -1
 
     // MFMA accumulators of this wavefront's Gram tiles, kept across chunks
     f4 acc[TPW];
@@ -179,59 +178,42 @@ __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
     for (int c0 = t_begin; c0 < t_end; c0 += CHUNK) {
         const int nc = min(CHUNK, t_end - c0);
         for (int q = tid; q < CHUNK * dpad; q += T1) Es[q] = 0.f;
-        // metadata: patch, other atoms, their coefficients.  Atom k-1's is
-        // recomputed (entry flagged with its rank, then one wavefront each).
-        if (tid < nc) {
-            const int64_t col = cb + c0 + tid;
-            mp[tid] = a.tab_p[col];
-            for (int i = 0; i < no; i++) {
-                const int4 o = a.tab_oth[col * no + i];
-                float x = 0.f;
-                int lazy = -1;
-                if (o.x >= 0) {
-                    if (o.x == k - 1 && o.z >= 0) {
-                        if (o.z < mprev) lazy = o.z;
-                    } else {
-                        x = a.coef[o.y];
-                    }
-                }
-                ma[tid * no + i] = o.x;
-                mc[tid * no + i] = x;
-                mr[tid * no + i] = lazy;
-            }
-        }
         __syncthreads();
-        for (int q = wave; q < nc * no; q += T1 / 64) {
-            const int r = mr[q];
-            if (r >= 0) {
-                const float x = wave_dot(Eprev + (int64_t)r * dpad, uprev, dim);
-                if (lane == 0) mc[q] = x;
-            }
-        }
-        __syncthreads();
-        // element-parallel build: thread (tc, te), 8 columns in flight
-        constexpr int BU = 8;
+        // element-parallel build: thread (tc, te) takes element te of BU
+        // columns at once; each thread reads its columns' table entries
+        // directly (the same addresses across a column: broadcast), then the
+        // patch, the atoms and the coefficients, all in flight together.
+        // Atom k-1's coefficient comes from the handoff values that K2(k-1)
+        // computed for exactly these entries.
         if (tc < cpt) {
             for (int t = tc; t < nc; t += BU * cpt) {
-                float v[BU];
+                int pp[BU];
+                int4 oo[BU][NO];
 #pragma unroll
                 for (int b = 0; b < BU; b++) {
                     const int tt = t + b * cpt;
-                    v[b] = tt < nc ? a.Y[(int64_t)mp[tt] * dim + te] : 0.f;
+                    const int64_t col = cb + c0 + (tt < nc ? tt : 0);
+                    pp[b] = a.tab_p[col];
+#pragma unroll
+                    for (int i = 0; i < NO; i++)
+                        oo[b][i] = i < no ? a.tab_oth[col * no + i] : make_int4(-1, 0, -1, 0);
                 }
-                for (int i = 0; i < no; i++) {
-                    float dv[BU];
+                float v[BU];
+#pragma unroll
+                for (int b = 0; b < BU; b++) v[b] = a.Y[(int64_t)pp[b] * dim + te];
+#pragma unroll
+                for (int i = 0; i < NO; i++) {
+                    float dv[BU], xv[BU];
 #pragma unroll
                     for (int b = 0; b < BU; b++) {
                         const int tt = t + b * cpt;
-                        const int at = tt < nc ? ma[tt * no + i] : -1;
-                        dv[b] = at >= 0 ? a.D[(int64_t)at * dim + te] : 0.f;
+                        const int4 o = oo[b][i];
+                        const int64_t ent = (cb + c0 + tt) * no + i;
+                        dv[b] = o.x >= 0 ? a.D[(int64_t)o.x * dim + te] : 0.f;
+                        xv[b] = o.x < 0 ? 0.f : o.w ? a.handx[ent] : a.coef[o.y];
                     }
 #pragma unroll
-                    for (int b = 0; b < BU; b++) {
-                        const int tt = t + b * cpt;
-                        if (tt < nc) v[b] = fmaf(-dv[b], mc[tt * no + i], v[b]);
-                    }
+                    for (int b = 0; b < BU; b++) v[b] = fmaf(-dv[b], xv[b], v[b]);
                 }
 #pragma unroll
                 for (int b = 0; b < BU; b++) {
@@ -366,6 +348,42 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     }
     for (int e = tid; e < dim; e += T2) a.D[(int64_t)k * dim + e] = u[e];
     if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
+    // ---- handoff: atom k's new coefficient for every table entry of atom
+    // k+1's columns that names atom k (x = e_r . d_k for rank r < m, else 0),
+    // computed with the same wave-level dot as K1's coefficient pass ----------
+    if (k + 1 >= a.K) return;
+    int c1;
+    const int m1 = atom_m(a.uoff, k + 1, a.cap, &c1);
+    if (m1 == 0) return;
+    const int no = a.kmax > 1 ? a.kmax - 1 : 1;
+    const int64_t cb1 = a.moff[k + 1];
+    const int nent = m1 * no;
+    int *hlist = (int *)(u + dim);      // LDS list of entries to compute
+    __shared__ int hcount;
+    if (tid == 0) hcount = 0;
+    __syncthreads();
+    const float *Ek = (k & 1) ? a.E1 : a.E0;
+    for (int q = tid; q < nent; q += T2) {
+        const int4 o = a.tab_oth[cb1 * no + q];
+        if (o.x == k && o.z >= 0) {
+            if (o.z < m) {
+                const int slot = atomicAdd(&hcount, 1);
+                if (slot < HAND_MAX) hlist[slot] = q;
+            } else {
+                a.handx[cb1 * no + q] = 0.f;
+            }
+        }
+    }
+    __syncthreads();
+    const int nh = min(hcount, HAND_MAX);
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int h = wave; h < nh; h += T2 / 64) {
+        const int q = hlist[h];
+        const int r = a.tab_oth[cb1 * no + q].z;
+        const float x = wave_dot(Ek + (int64_t)r * a.dpad, u, dim);
+        if (lane == 0) a.handx[cb1 * no + q] = x;
+    }
+    if (hcount > HAND_MAX && tid == 0) atomicAdd(&a.counters[CNT_DEGEN + 2], 1ull);
 }
 
 }  // namespace
@@ -391,6 +409,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.E0 = (float *)ctx->E.ptr;
     a.E1 = (float *)ctx->E2.ptr;
     a.Gpart = (float *)ctx->gpart.ptr;
+    a.handx = (float *)ctx->handx.ptr;
     a.G = (float *)ctx->wbuf.ptr;
     a.counters = (unsigned long long *)ctx->counters.ptr;
     a.dim = dim;
@@ -403,18 +422,22 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.maxit = 300;
     a.tol2 = tol * tol;
     const int no = kmax > 1 ? kmax - 1 : 1;
-    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad + sizeof(int) * CHUNK + (2 * sizeof(int) + sizeof(float)) * CHUNK * no;
-    const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim);
+    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad;
+    (void)no;
+    const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim) + sizeof(int) * HAND_MAX;
+    // K1 variants: Gram tiles per wavefront x other support entries per column
     const int tpw = (ntiles + 15) / 16;
-    const void *k1 = tpw <= 1 ? (const void *)k1_kernel<1> : tpw <= 2 ? (const void *)k1_kernel<2>
-                   : tpw <= 3 ? (const void *)k1_kernel<3> : (const void *)k1_kernel<5>;
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    const bool wide = kmax > 5;
+    typedef void (*K1Fn)(SweepArgs, int);
+    K1Fn k1 = nullptr;
+    if (!wide) k1 = tpw <= 1 ? k1_kernel<1, 4, 4> : tpw <= 2 ? k1_kernel<2, 4, 3> : tpw <= 3 ? k1_kernel<3, 4, 3>
+                                                                                             : k1_kernel<5, 4, 2>;
+    else k1 = tpw <= 1 ? k1_kernel<1, 7, 2> : tpw <= 2 ? k1_kernel<2, 7, 2> : tpw <= 3 ? k1_kernel<3, 7, 2>
+                                                                                       : k1_kernel<5, 7, 2>;
+    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
     KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
     for (int k = 0; k <= K; k++) {
-        if (tpw <= 1) hipLaunchKernelGGL(k1_kernel<1>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
-        else if (tpw <= 2) hipLaunchKernelGGL(k1_kernel<2>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
-        else if (tpw <= 3) hipLaunchKernelGGL(k1_kernel<3>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
-        else hipLaunchKernelGGL(k1_kernel<5>, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
+        hipLaunchKernelGGL(k1, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(1), dim3(T2), lds2, ctx->stream, a, k);
     }
     KSVD_HIP_CHECK(ctx, hipGetLastError());

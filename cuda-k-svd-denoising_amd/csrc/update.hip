@@ -173,8 +173,12 @@ __global__ __launch_bounds__(256) void cols_table_kernel(int K, int kmax, int ca
         int na = 0;
         for (int i = 0; i < cp; i++)
             if (i != my)
-                tab_oth[col * (kmax - 1) + na++] =
-                    make_int4(idx[(int64_t)p * kmax + i], p * kmax + i, srank[(int64_t)p * kmax + i], 0);
+            {
+                // .w = 1: atom k-1's coefficient, which the sweep hands over
+                // from K2(k-1) instead of reading it
+                const int at = idx[(int64_t)p * kmax + i], rk = srank[(int64_t)p * kmax + i];
+                tab_oth[col * (kmax - 1) + na++] = make_int4(at, p * kmax + i, rk, (at == k - 1 && rk >= 0) ? 1 : 0);
+            }
         for (; na < kmax - 1; na++) tab_oth[col * (kmax - 1) + na] = make_int4(-1, 0, -1, 0);
     }
 }
