@@ -112,8 +112,7 @@ int check_params(ksvd_ctx *ctx, const ksvd_params *p)
     if (p->atoms <= 0 || p->atoms > 1024) return ksvd_fail(ctx, KSVD_ERR_ARG, "atoms must be in [1, 1024]");
     if (p->omp_iter <= 0 || p->omp_iter > KSVD_MAX_SUPPORT)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "ompIter must be in [1, 16]");
-    if (p->omp_iter > 8)
-        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the dictionary sweep handles ompIter <= 8");
+
     if (p->ksvd_iter < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "ksvdIter must be >= 0");
     if (p->user_cap < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "user_cap must be >= 0");
     const int dim = p->patch_w * p->patch_h;
@@ -389,6 +388,7 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 {
     if (!ctx) return KSVD_ERR_ARG;
     if (!ctx->has_codes || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "codes and dictionary required");
+    if (ctx->kmax > 8) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the dictionary sweep handles ompIter <= 8");
     int rc = set_device(ctx);
     if (rc) return rc;
     Span s;
