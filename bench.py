@@ -23,7 +23,13 @@ PKG = os.path.join(ROOT, "cuda-k-svd-denoising_amd")
 ORACLE = os.path.join(ROOT, "oracle")
 sys.path.insert(0, PKG)
 
-WORKLOAD = dict(W=512, H=512, sigma=25.0, patch=(12, 12), slide=(3, 3), atoms=256, omp_iter=5, seed=2)
+# The three square workloads (seeded synthetic images; the 512^2 one is the
+# default benchmark line, the others are selectable with --config).
+WORKLOADS = {
+    "128": dict(W=128, H=128, sigma=20.0, patch=(8, 8), slide=(2, 2), atoms=256, omp_iter=5, seed=1, iters=5),
+    "512": dict(W=512, H=512, sigma=25.0, patch=(12, 12), slide=(3, 3), atoms=256, omp_iter=5, seed=2, iters=10),
+    "2048": dict(W=2048, H=2048, sigma=20.0, patch=(12, 12), slide=(3, 3), atoms=256, omp_iter=5, seed=3, iters=10),
+}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -64,7 +70,9 @@ def calibrate_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--config", choices=sorted(WORKLOADS), default="512",
+                    help="image side of the workload (512: the benchmark line)")
+    ap.add_argument("--steps", type=int, default=None, help="K-SVD iterations timed (default: the config's)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -79,7 +87,9 @@ def main():
         sys.exit(2)
 
     import pyksvd as K
-    wl = WORKLOAD
+    wl = WORKLOADS[args.config]
+    if args.steps is None:
+        args.steps = wl["iters"]
     clean, noisy = K.synth_image(wl["seed"], wl["W"], wl["H"], wl["sigma"])
     frame = K.image_to_frame(noisy)
 
