@@ -7,9 +7,15 @@ sigma 25, 12 x 12 patches sliding 3 (27,889 patches of dim 144), 256 atoms,
 OMP limit 5.  Inputs are resident in HBM before the timed region; the timed
 region is exactly --steps iterations from the reference's dictionary init.
 
-Prints one JSON line.  The roofline object is for the OMP pursuit kernel
-(correlation on MFMA FP32), timed with HIP events on the library's stream;
-the cpu_baseline object times the CPU oracle (FP32, OpenMP) on one K-SVD
+Prints one JSON line.  `roofline` is for the kernel pair that dominates the
+step: one atom of the Gauss-Seidel dictionary sweep (k1_kernel builds the
+restricted residual and partial Grams, k2_kernel reduces them and takes the
+top eigenvector).  Its algorithmic bytes are those of the reference's update
+for the atom's m users (re-read each user's patch and code, write its new
+coefficient); its duration is the library's own HIP-event span of the
+dictionary update on the library's stream, divided by the atoms swept.
+`roofline_omp` is the same record for the OMP pursuit kernel (correlation on
+MFMA FP32).  `cpu_baseline` times the CPU oracle (FP32, OpenMP) on one K-SVD
 iteration of the same workload on this host.
 """
 import argparse
@@ -17,6 +23,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "cuda-k-svd-denoising_amd")
@@ -33,6 +41,14 @@ WORKLOADS = {
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+# the 512 x 512 workload (profiles/hbm_traffic_ksvd_512sq.md), scaled by what
+# the same counters read for a device copy of known size (FETCH x 2, WRITE x 1).
+MEASURED_TRAFFIC = {
+    "512": {"omp": (9367.4 * 2 + 1198.5) * 1024,
+            "sweep_atom": (485.4 * 2 + 578.9 + 196.1 * 2 + 0.9) * 1024},
+}
+
 
 def omp_flops_per_patch(n, K, k):
     """Algorithmic flops of one patch's pursuit (residual-correlation OMP):
@@ -47,6 +63,20 @@ def update_flops_per_patch(n, k):
 
 def bytes_per_patch(n, k):
     return 4 * n + 8 * k + k * (4 * n + 8 * k + 4)
+
+
+def update_bytes_per_user(n, k):
+    """Algorithmic HBM bytes of one (atom, user) column of the rank-1 update:
+    re-read the user's patch (4n) and code (8k), write its new coefficient."""
+    return 4 * n + 8 * k + 4
+
+
+def capped_users(idx, coef, cnt, K, cap):
+    """Sum over atoms of min(c_k, cap), c_k = patches whose code for atom k is
+    non-zero: the columns one dictionary sweep processes."""
+    mask = (np.arange(idx.shape[1])[None, :] < cnt[:, None]) & (coef != 0)
+    c = np.bincount(idx[mask], minlength=K)
+    return int(np.minimum(c, cap).sum())
 
 
 def calibrate_copy(mib):
@@ -145,6 +175,21 @@ def main():
     upd_avg_ms = tm["update_ms"] / max(tm["update_calls"], 1)
     ks.close()
 
+    # Columns swept per iteration: replay the same (deterministic) iterations
+    # step by step, outside any timed region, and count each sweep's users.
+    ks3 = make()
+    cols = 0
+    for _ in range(args.steps):
+        ks3.omp()
+        cols += capped_users(*ks3.codes(), wl["atoms"], 1024)
+        ks3.update()
+    ks3.close()
+    atoms_swept = args.steps * wl["atoms"]
+    atom_ms = tm["update_ms"] / atoms_swept
+    atom_bytes = cols / atoms_swept * update_bytes_per_user(n, kk)
+    atom_gbs = atom_bytes / (atom_ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(args.config, {})
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, ORACLE)
@@ -178,10 +223,17 @@ def main():
                                f"{wl['atoms']} atoms, OMP limit {kk}, user cap 1024",
                    "patches": P, "patch_dim": n, "atoms": wl["atoms"], "omp_iter": kk,
                    "ksvd_iter": args.steps},
-        "roofline": {"bound": "mfma", "kernel": "omp_mfma_kernel", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
-                     "traffic": None, "avg_launch_ms": round(omp_avg_ms, 5),
-                     "flops_per_launch": flops_launch},
+        "roofline": {"bound": "hbm", "kernel": "k1_kernel + k2_kernel (one atom of the dictionary sweep)",
+                     "achieved": round(atom_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(atom_gbs / PEAK_HBM_GBS, 6),
+                     "traffic": round(traffic["sweep_atom"]) if "sweep_atom" in traffic else None,
+                     "avg_launch_ms": round(atom_ms, 5), "bytes_per_launch": round(atom_bytes),
+                     "users_per_atom": round(cols / atoms_swept, 1)},
+        "roofline_omp": {"bound": "mfma", "kernel": "omp_mfma_kernel", "achieved": round(achieved, 3),
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                         "traffic": round(traffic["omp"]) if "omp" in traffic else None,
+                         "avg_launch_ms": round(omp_avg_ms, 5), "flops_per_launch": flops_launch},
         "phase_ms_per_step": {"omp": round(tm["omp_ms"] / args.steps, 4),
                               "update": round(upd_avg_ms, 4)},
         "power_iters_per_atom": round(tm["eig_iters"] / max(tm["update_calls"], 1) / wl["atoms"], 2),
