@@ -318,6 +318,9 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         rc = ksvd_reserve(ctx, *e.b, e.bytes);
         if (rc) return rc;
     }
+    // flags of the persistent sweep (its reset kernel keeps the sticky abort word)
+    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    KSVD_HIP_CHECK(ctx, hipMemset(ctx->ticket.ptr, 0, ctx->ticket.bytes));
     KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
     ctx->has_image = true;
     ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
@@ -384,9 +387,9 @@ int ksvd_omp(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
-int ksvd_update_dictionary(ksvd_ctx *ctx)
+// Enqueues one sweep; a sweep that gave up is reported by check_sweep.
+static int update_dictionary_async(ksvd_ctx *ctx)
 {
-    if (!ctx) return KSVD_ERR_ARG;
     if (!ctx->has_codes || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "codes and dictionary required");
     if (ctx->kmax > 8) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the dictionary sweep handles ompIter <= 8");
     int rc = set_device(ctx);
@@ -399,6 +402,14 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
+int ksvd_update_dictionary(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    int rc = update_dictionary_async(ctx);
+    if (rc) return rc;
+    return check_sweep(ctx);
+}
+
 int ksvd_iterate(ksvd_ctx *ctx, int iters)
 {
     if (!ctx) return KSVD_ERR_ARG;
@@ -406,10 +417,10 @@ int ksvd_iterate(ksvd_ctx *ctx, int iters)
     for (int i = 0; i < iters; i++) {
         int rc = ksvd_omp(ctx);
         if (rc) return rc;
-        rc = ksvd_update_dictionary(ctx);
+        rc = update_dictionary_async(ctx);
         if (rc) return rc;
     }
-    return KSVD_OK;
+    return iters > 0 ? check_sweep(ctx) : KSVD_OK;
 }
 
 int ksvd_run(ksvd_ctx *ctx)

@@ -45,7 +45,7 @@ struct ksvd_ctx {
     DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
     DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
     DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
-    DevBuf srank, handx, E2, gpart, ticket; // per-atom sweep kernels (sweep.hip)
+    DevBuf srank, handx, E2, gpart, ticket; // sweep kernels (sweep.hip); ticket: flags of the persistent sweep
     DevBuf rsum, rweight, rout, rpart, rlam;
 
     ksvd_timings tm{};
@@ -76,4 +76,6 @@ int launch_omp(ksvd_ctx *ctx);
 int launch_update(ksvd_ctx *ctx);
 void drop_update_graph(ksvd_ctx *ctx);
 size_t sweep_gpart_bytes(int dim);
+// Synchronises the stream; fails if a persistent sweep gave up waiting.
+int check_sweep(ksvd_ctx *ctx);
 int launch_rebuild(ksvd_ctx *ctx, float sigma2);

@@ -19,6 +19,7 @@
 // A final K1(K) writes atom K-1's coefficients.  E_k alternates between two
 // buffers so that K1(k+1) reads E_k while writing E_{k+1}.
 #include <algorithm>
+#include <cstdlib>
 
 #include "ksvd_internal.hpp"
 
@@ -267,6 +268,38 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
     return s;
 }
 
+// Top eigenvector of the symmetric dim x dim matrix Gs (LDS) by power
+// iteration from u (LDS, overwritten with the result), T2 threads, 4 lanes per
+// row.  Stops when two successive unit iterates differ by less than sqrt(tol2)
+// (after at least 2 steps) or after maxit steps; a zero product keeps u.
+// Returns the number of steps.
+__device__ int power_iteration(const float *Gs, float *u, int dim, float *red, float tol2, int maxit)
+{
+    const int tid = threadIdx.x;
+    const int row = tid >> 2, part = tid & 3;
+    const int qlen = (dim + 3) / 4;
+    const int q0 = part * qlen, q1 = min(dim, q0 + qlen);
+    int it = 0;
+    for (;;) {
+        float z = 0.f;
+        if (row < dim)
+            for (int q = q0; q < q1; q++) z = fmaf(Gs[q * dim + row], u[q], z);
+        z += __shfl_xor(z, 1);
+        z += __shfl_xor(z, 2);
+        const bool owner = row < dim && part == 0;
+        const float nz = sqrtf(block_sum_t2(owner ? z * z : 0.f, red));
+        if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
+        const float un = owner ? z / nz : 0.f;
+        const float du = owner ? un - u[row] : 0.f;
+        const float d2 = block_sum_t2(du * du, red);
+        if (owner) u[row] = un;
+        __syncthreads();
+        it++;
+        if ((it >= 2 && d2 < tol2) || it >= maxit) break;
+    }
+    return it;
+}
+
 __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 {
     extern __shared__ float sm[];
@@ -324,28 +357,7 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     }
     for (int e = tid; e < dim; e += T2) u[e] = a.D[(int64_t)k * dim + e];
     __syncthreads();
-    // ---- top eigenvector by power iteration: 4 lanes per row -----------------
-    const int row = tid >> 2, part = tid & 3;
-    const int qlen = (dim + 3) / 4;
-    const int q0 = part * qlen, q1 = min(dim, q0 + qlen);
-    int it = 0;
-    for (;;) {
-        float z = 0.f;
-        if (row < dim)
-            for (int q = q0; q < q1; q++) z = fmaf(Gs[q * dim + row], u[q], z);
-        z += __shfl_xor(z, 1);
-        z += __shfl_xor(z, 2);
-        const bool owner = row < dim && part == 0;
-        const float nz = sqrtf(block_sum_t2(owner ? z * z : 0.f, red));
-        if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
-        const float un = owner ? z / nz : 0.f;
-        const float du = owner ? un - u[row] : 0.f;
-        const float d2 = block_sum_t2(du * du, red);
-        if (owner) u[row] = un;
-        __syncthreads();
-        it++;
-        if ((it >= 2 && d2 < a.tol2) || it >= a.maxit) break;
-    }
+    const int it = power_iteration(Gs, u, dim, red, a.tol2, a.maxit);
     for (int e = tid; e < dim; e += T2) a.D[(int64_t)k * dim + e] = u[e];
     if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
     // ---- handoff: atom k's new coefficient for every table entry of atom
@@ -386,7 +398,409 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     if (hcount > HAND_MAX && tid == 0) atomicAdd(&a.counters[CNT_DEGEN + 2], 1ull);
 }
 
+// ===========================================================================
+// Persistent sweep on one XCD: the whole Gauss-Seidel sweep in one launch,
+// used when every atom's capped column count fits the helpers' LDS slices
+// (cap <= H * PS_CWMAX).
+//
+// The workgroups read the XCD they run on (hardware register XCC_ID); the
+// first H + 1 to arrive on XCD 0 take roles, one master and H helpers, and
+// the others leave at once.  All parties then share XCD 0's L2, so data
+// passes between them without an L2 write-back: the writer stores normally,
+// waits for its stores (vmcnt(0)) and sets a flag with an L1-bypassing
+// (sc1) store; the reader polls the flag and reads the data with sc1 loads.
+// Data that nobody changes during the sweep (patches, column table, users)
+// is read with ordinary loads.
+//
+// Phase k (atom k); helper h owns columns [h*cw, (h+1)*cw) of atom k's m
+// capped users, cw = ceil(m / H):
+//   P1 (before atom k-1 is final) builds its columns of E_k in LDS from every
+//      support entry except atom k-1's, once every helper has written atom
+//      k-2's final coefficients (cdone = k);
+//   W  waits until the master has published d_{k-1} (aflag = k);
+//   P2 for each column whose patch is among atom k-1's first m_{k-1} users
+//      (rank r), x = e_{k-1,r} . d_{k-1} -- the same arithmetic on the same
+//      values as P4's stored coefficient, so both agree bit for bit -- and
+//      e -= d_{k-1} x;
+//   P3 partial Gram of its columns on MFMA -> Gpart[h], its E_k columns ->
+//      global; done[h] = k + 1;
+//   P4 atom k-1's final coefficients for its share of atom k-1's users (0 for
+//      users beyond the cap); cdone[h] = k + 1.
+// Master, phase k: waits for done[*] = k + 1, sums the partial Grams in helper
+// order into LDS, runs the power iteration from d_k, writes d_k and sets
+// aflag = k + 1.  After phase K - 1 the helpers run P4 for atom K - 1.
+// Every wait is bounded: one that runs out sets an abort word (and a sticky
+// copy the host reads) which every other wait also polls, so all workgroups
+// leave; the host then reports the sweep as failed.
+constexpr int PS_T = 1024;              // threads per workgroup
+constexpr int PS_W = PS_T / 64;         // wavefronts per workgroup
+constexpr int PS_HMAX = 16;             // helpers at most
+constexpr int PS_CWMAX = 128;           // columns per helper slice
+constexpr int PS_CPW = PS_CWMAX / PS_W; // columns per wavefront in the dot passes
+constexpr int PS_NOMAX = 7;             // other support entries per column (ompIter <= 8)
+constexpr long PS_SPIN_MAX = 1L << 21;  // polls before a wait gives up
+enum { PSF_REG = 0, PSF_DONE = 16, PSF_CDONE = 32, PSF_AFLAG = 48, PSF_ABORT = 49, PSF_STICKY = 50, PSF_WORDS = 64 };
+
+__device__ __forceinline__ unsigned ld_flag(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_flag(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// L1-bypassing (sc1) load of a value another workgroup wrote
+__device__ __forceinline__ float ld_c(const float *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sc1 16-byte load, `off` bytes into a buffer of `bytes` bytes
+__device__ __forceinline__ f4 ld4_c(const float *base, unsigned bytes, unsigned off)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), 0, bytes, 0x00020000);
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 16));
+}
+
+// This workgroup's global stores reach the XCD's L2, then *flag = v.
+__device__ __forceinline__ void ps_publish(unsigned *flag, unsigned v)
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt untouched
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) st_flag(flag, v);
+}
+
+// Threads 0 .. count-1 each wait for flags[first + t] >= v.  Returns false
+// (for the whole workgroup) after an abort or a wait that ran out.
+__device__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *bail)
+{
+    if ((int)threadIdx.x < count) {
+        const unsigned *f = flags + first + threadIdx.x;
+        long spins = 0;
+        while (ld_flag(f) < v) {
+            __builtin_amdgcn_s_sleep(2);
+            if ((++spins & 127) == 0) {
+                if (ld_flag(flags + PSF_ABORT)) {
+                    *bail = 1;
+                    break;
+                }
+                if (spins > PS_SPIN_MAX) {
+                    st_flag(flags + PSF_STICKY, 1u);
+                    st_flag(flags + PSF_ABORT, 1u);
+                    *bail = 1;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const bool ok = *bail == 0;
+    __syncthreads();
+    return ok;
+}
+
+// x[g] = e_g . u for the PS_CPW columns col[g] of one wavefront (nullptr: no
+// column, x[g] meaningless; col0 is any valid column).  Lane l sums elements l, l+64, l+128 in that order with
+// fmaf, then a fixed butterfly; the loads of all columns are issued first.
+__device__ __forceinline__ void ps_dots(const float *const *col, const float *col0, const float *u, int dim,
+                                        float *x)
+{
+    const int lane = threadIdx.x & 63;
+    // every load is issued (a missing column reads col0, an element beyond dim
+    // reads element dim-1) so that none waits on a branch; those values are
+    // never used
+    float ev[PS_CPW][3];
+#pragma unroll
+    for (int g = 0; g < PS_CPW; g++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) ev[g][j] = ld_c((col[g] ? col[g] : col0) + min(lane + 64 * j, dim - 1));
+    float uv[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) uv[j] = lane + 64 * j < dim ? u[lane + 64 * j] : 0.f;
+#pragma unroll
+    for (int g = 0; g < PS_CPW; g++) {
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (lane + 64 * j < dim) s = fmaf(ev[g][j], uv[j], s);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        x[g] = s;
+    }
+}
+
+__device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm, float *red, int *bail)
+{
+    const int tid = threadIdx.x, dim = a.dim;
+    float *Gs = sm;             // dim x dim
+    float *u = Gs + dim * dim;  // dim
+    const int nf4 = a.ntiles * 64;
+    const unsigned gbytes = (unsigned)(H * nf4 * 16);
+    unsigned long long iters = 0;
+    for (int k = 0; k < a.K; k++) {
+        int c;
+        const int m = atom_m(a.uoff, k, a.cap, &c);
+        if (m > 0) {
+            if (!ps_wait(flags, PSF_DONE, H, (unsigned)(k + 1), bail)) return;
+            const int cw = (m + H - 1) / H;
+            const int nb = (m + cw - 1) / cw;
+            // G = sum of the nb partial Grams (helper order), both triangles
+            for (int f = tid; f < nf4; f += PS_T) {
+                f4 s = f4{0.f, 0.f, 0.f, 0.f};
+                for (int b0 = 0; b0 < nb; b0 += 8) {
+                    // unconditional: a load past the buffer's end returns 0;
+                    // partials past nb are read but not added
+                    f4 pv[8];
+#pragma unroll
+                    for (int b = 0; b < 8; b++) pv[b] = ld4_c(a.Gpart, gbytes, (unsigned)(((b0 + b) * nf4 + f) * 16));
+#pragma unroll
+                    for (int b = 0; b < 8; b++)
+                        if (b0 + b < nb) s += pv[b];
+                }
+                const int ti = f >> 6, l = f & 63;
+                int ba, bb;
+                tile_coords(ti, a.nbt, &ba, &bb);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
+                    if (ga < dim && gb < dim) {
+                        Gs[ga * dim + gb] = s[i];
+                        Gs[gb * dim + ga] = s[i];
+                    }
+                }
+            }
+            for (int e = tid; e < dim; e += PS_T) u[e] = ld_c(a.D + (int64_t)k * dim + e);
+            __syncthreads();
+            iters += (unsigned long long)power_iteration(Gs, u, dim, red, a.tol2, a.maxit);
+            for (int e = tid; e < dim; e += PS_T) a.D[(int64_t)k * dim + e] = u[e];
+        }
+        ps_publish(flags + PSF_AFLAG, (unsigned)(k + 1));
+    }
+    if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], iters);
+}
+
+template <int TPW, int NO, int BU>
+__device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, float *sm, int *bail)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int dim = a.dim, dpad = a.dpad, kmax = a.kmax, K = a.K;
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    float *Es = sm;                              // PS_CWMAX x dpad, columns of E_k
+    float *uprev = Es + PS_CWMAX * dpad;         // dpad: d_{k-1}
+    float *sx = uprev + dpad;                    // PS_CWMAX x PS_NOMAX coefficients
+    int *sat = (int *)(sx + PS_CWMAX * PS_NOMAX);  // PS_CWMAX x PS_NOMAX atoms (-1: none)
+    int *sp = sat + PS_CWMAX * PS_NOMAX;         // PS_CWMAX patches
+    int *hrank = sp + PS_CWMAX;                  // PS_CWMAX: rank among atom k-1's users, -1: none
+    for (int k = 0; k <= K; k++) {
+        // ---- P1: E_k without atom k-1 ---------------------------------------
+        int m = 0, nc = 0, tb = 0;
+        int cprev = 0, mprev = 0;
+        if (k >= 1) mprev = atom_m(a.uoff, k - 1, a.cap, &cprev);
+        if (k < K) {
+            int c;
+            m = atom_m(a.uoff, k, a.cap, &c);
+            if (k >= 1 && !ps_wait(flags, PSF_CDONE, H, (unsigned)k, bail)) return;
+            if (m > 0) {
+                const int cw = (m + H - 1) / H;
+                tb = h * cw;
+                nc = max(0, min(m, tb + cw) - tb);
+            }
+            if (nc > 0) {
+                const int64_t cb = a.moff[k] + tb;
+                for (int tl = tid; tl < nc; tl += PS_T) {
+                    const int64_t col = cb + tl;
+                    sp[tl] = a.tab_p[col];
+                    // all loads unconditional (entries past `no` reread the
+                    // last one and are then dropped), so none waits on a branch
+                    int4 o[NO];
+#pragma unroll
+                    for (int i = 0; i < NO; i++) o[i] = a.tab_oth[col * no + min(i, no - 1)];
+                    float xv[NO];
+#pragma unroll
+                    for (int i = 0; i < NO; i++) xv[i] = ld_c(a.coef + (o[i].x >= 0 ? o[i].y : 0));
+                    int hr = -1;
+#pragma unroll
+                    for (int i = 0; i < NO; i++) {
+                        int at = i < no ? o[i].x : -1;
+                        if (at == k - 1) {
+                            if (o[i].z >= 0 && o[i].z < mprev) hr = o[i].z;
+                            at = -1;
+                        }
+                        sat[tl * PS_NOMAX + i] = at;
+                        sx[tl * PS_NOMAX + i] = at >= 0 ? xv[i] : 0.f;
+                    }
+                    hrank[tl] = hr;
+                }
+                const int nc4 = (nc + 3) & ~3;
+                for (int q = tid; q < nc4 * dpad; q += PS_T) {
+                    const int tl = q / dpad, e = q - tl * dpad;
+                    if (tl >= nc || e >= dim) Es[q] = 0.f;
+                }
+                __syncthreads();
+                const int cpt = PS_T / dim;
+                const int tc = tid / dim, te = tid - tc * dim;
+                if (tc < cpt) {
+                    for (int t = tc; t < nc; t += BU * cpt) {
+                        float v[BU], dv[BU][NO];
+                        // unconditional loads: a column past nc rereads column
+                        // nc-1, an empty entry reads atom 0 (its coefficient
+                        // is 0, so it adds exactly nothing)
+#pragma unroll
+                        for (int b = 0; b < BU; b++) {
+                            const int tt = min(t + b * cpt, nc - 1);
+                            v[b] = a.Y[(int64_t)sp[tt] * dim + te];
+#pragma unroll
+                            for (int i = 0; i < NO; i++)
+                                dv[b][i] = ld_c(a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te);
+                        }
+#pragma unroll
+                        for (int b = 0; b < BU; b++) {
+                            const int tt = t + b * cpt;
+                            if (tt < nc) {
+#pragma unroll
+                                for (int i = 0; i < NO; i++) v[b] = fmaf(-dv[b][i], sx[tt * PS_NOMAX + i], v[b]);
+                                Es[tt * dpad + te] = v[b];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // ---- W: d_{k-1} -------------------------------------------------------
+        if (k >= 1) {
+            if (!ps_wait(flags, PSF_AFLAG, 1, (unsigned)k, bail)) return;
+            for (int e = tid; e < dpad; e += PS_T) uprev[e] = e < dim ? ld_c(a.D + (int64_t)(k - 1) * dim + e) : 0.f;
+        }
+        __syncthreads();
+        const float *Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
+        // ---- P2: atom k-1's share of E_k --------------------------------------
+        if (k >= 1 && nc > 0) {
+            const float *colp[PS_CPW];
+            int tls[PS_CPW];
+#pragma unroll
+            for (int g = 0; g < PS_CPW; g++) {
+                const int tl = wave + PS_W * g;
+                const int r = tl < nc ? hrank[tl] : -1;
+                tls[g] = r >= 0 ? tl : -1;
+                colp[g] = r >= 0 ? Eprev + (int64_t)r * dpad : nullptr;
+            }
+            float x[PS_CPW];
+            ps_dots(colp, Eprev, uprev, dim, x);
+#pragma unroll
+            for (int g = 0; g < PS_CPW; g++) {
+                if (tls[g] < 0) continue;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int e = lane + 64 * j;
+                    if (e < dim) Es[tls[g] * dpad + e] = fmaf(-uprev[e], x[g], Es[tls[g] * dpad + e]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- P3: partial Gram, E_k columns to global --------------------------
+        if (k < K) {
+            if (nc > 0) {
+                f4 acc[TPW];
+#pragma unroll
+                for (int q = 0; q < TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+                const int col16 = lane & 15, grp = lane >> 4;
+                const int nk = (nc + 3) >> 2;
+#pragma unroll
+                for (int q = 0; q < TPW; q++) {
+                    const int ti = wave + PS_W * q;
+                    if (ti < a.ntiles) {
+                        int ba, bb;
+                        tile_coords(ti, a.nbt, &ba, &bb);
+                        for (int kk = 0; kk < nk; kk++) {
+                            const int t = 4 * kk + grp;
+                            const float av = Es[t * dpad + ba * 16 + col16];
+                            const float bv = Es[t * dpad + bb * 16 + col16];
+                            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
+                        }
+                    }
+                }
+                float *gp = a.Gpart + (size_t)h * a.ntiles * 256;
+#pragma unroll
+                for (int q = 0; q < TPW; q++) {
+                    const int ti = wave + PS_W * q;
+                    if (ti < a.ntiles) *(f4 *)(gp + (size_t)ti * 256 + lane * 4) = acc[q];
+                }
+                float *Ecur = (k & 1) ? a.E1 : a.E0;
+                for (int q = tid; q < nc * dim; q += PS_T) {
+                    const int tl = q / dim, e = q - tl * dim;
+                    Ecur[(int64_t)(tb + tl) * dpad + e] = Es[tl * dpad + e];
+                }
+            }
+            ps_publish(flags + PSF_DONE + h, (unsigned)(k + 1));
+        }
+        // ---- P4: atom k-1's final coefficients --------------------------------
+        if (k >= 1 && cprev > 0) {
+            const int cwp = (mprev + H - 1) / H;
+            const int tbp = h * cwp;
+            const int ncp = max(0, min(mprev, tbp + cwp) - tbp);
+            const int64_t cbp = a.moff[k - 1] + tbp;
+            const float *colp[PS_CPW];
+#pragma unroll
+            for (int g = 0; g < PS_CPW; g++) {
+                const int tl = wave + PS_W * g;
+                colp[g] = tl < ncp ? Eprev + (int64_t)(tbp + tl) * dpad : nullptr;
+            }
+            float x[PS_CPW];
+            ps_dots(colp, Eprev, uprev, dim, x);
+            if (lane == 0) {
+#pragma unroll
+                for (int g = 0; g < PS_CPW; g++) {
+                    const int tl = wave + PS_W * g;
+                    if (tl < ncp) a.coef[a.tab_self[cbp + tl]] = x[g];
+                }
+            }
+            const int basep = a.uoff[k - 1];
+            for (int t = mprev + h * PS_T + tid; t < cprev; t += H * PS_T) {
+                const int pos = basep + t;
+                a.coef[(int64_t)a.users[pos] * kmax + a.uslot[pos]] = 0.f;
+            }
+        }
+        if (k < K) ps_publish(flags + PSF_CDONE + h, (unsigned)(k + 1));
+    }
+}
+
+template <int TPW, int NO, int BU>
+__global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *flags, int H)
+{
+    extern __shared__ float sm[];
+    __shared__ float red[PS_W];
+    __shared__ int s_role, s_bail;
+    if (threadIdx.x == 0) {
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+        s_role = (xcc == 0 && slot <= (unsigned)H) ? (int)slot : -1;
+        s_bail = 0;
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    if (role == 0) ps_master(a, flags, H, sm, red, &s_bail);
+    else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, sm, &s_bail);
+}
+
+// Clears the sweep's flags (all but the sticky abort word) before a launch.
+__global__ void ps_reset_kernel(unsigned *flags)
+{
+    const int i = threadIdx.x;
+    if (i < PSF_WORDS && i != PSF_STICKY) flags[i] = 0u;
+}
+
 }  // namespace
+
+static bool env_flag(const char *name)
+{
+    const char *v = getenv(name);
+    return v && v[0] && v[0] != '0';
+}
 
 int launch_sweep(ksvd_ctx *ctx, float tol)
 {
@@ -421,13 +835,38 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.ntiles = ntiles;
     a.maxit = 300;
     a.tol2 = tol * tol;
-    const int no = kmax > 1 ? kmax - 1 : 1;
-    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad;
-    (void)no;
-    const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim) + sizeof(int) * HAND_MAX;
-    // K1 variants: Gram tiles per wavefront x other support entries per column
     const int tpw = (ntiles + 15) / 16;
     const bool wide = kmax > 5;
+
+    // ---- persistent one-XCD sweep -------------------------------------------
+    const int cap = a.cap;
+    int H = 8;
+    if (const char *hv = getenv("KSVD_SWEEP_HELPERS")) H = std::max(1, std::min(PS_HMAX, atoi(hv)));
+    if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
+    const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL");
+    if (persistent) {
+        const size_t lds_helper = sizeof(float) * ((size_t)PS_CWMAX * dpad + dpad + (size_t)PS_CWMAX * PS_NOMAX) +
+                                  sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 2 * PS_CWMAX);
+        const size_t lds_master = sizeof(float) * ((size_t)dim * dim + dim);
+        const size_t lds = std::max(lds_helper, lds_master);
+        typedef void (*PsFn)(SweepArgs, unsigned *, int);
+        PsFn ps = nullptr;
+        if (!wide) ps = tpw <= 1 ? ps_sweep_kernel<1, 4, 4> : tpw <= 2 ? ps_sweep_kernel<2, 4, 4>
+                      : tpw <= 3 ? ps_sweep_kernel<3, 4, 4> : ps_sweep_kernel<5, 4, 4>;
+        else ps = tpw <= 1 ? ps_sweep_kernel<1, 7, 2> : tpw <= 2 ? ps_sweep_kernel<2, 7, 2>
+                 : tpw <= 3 ? ps_sweep_kernel<3, 7, 2> : ps_sweep_kernel<5, 7, 2>;
+        KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)ps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        unsigned *flags = (unsigned *)ctx->ticket.ptr;
+        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, flags);
+        // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on XCD 0
+        hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H);
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        return KSVD_OK;
+    }
+
+    // ---- two kernels per atom (any cap) --------------------------------------
+    const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad;
+    const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim) + sizeof(int) * HAND_MAX;
     typedef void (*K1Fn)(SweepArgs, int);
     K1Fn k1 = nullptr;
     if (!wide) k1 = tpw <= 1 ? k1_kernel<1, 4, 4> : tpw <= 2 ? k1_kernel<2, 4, 3> : tpw <= 3 ? k1_kernel<3, 4, 3>
@@ -444,8 +883,27 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     return KSVD_OK;
 }
 
+// Reads and clears the persistent sweep's sticky abort word (synchronises the
+// context's stream).  Returns KSVD_OK, or KSVD_ERR_HIP if a sweep gave up.
+int check_sweep(ksvd_ctx *ctx)
+{
+    unsigned v = 0;
+    unsigned *flags = (unsigned *)ctx->ticket.ptr;
+    if (!flags) return KSVD_OK;
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&v, flags + PSF_STICKY, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (v) {
+        KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STICKY, 0, sizeof(unsigned), ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        return ksvd_fail(ctx, KSVD_ERR_HIP,
+                         "dictionary sweep: its workgroups did not all run on one XCD (a wait ran out); "
+                         "set KSVD_SWEEP_TWO_KERNEL=1 to use the two-kernel sweep");
+    }
+    return KSVD_OK;
+}
+
 size_t sweep_gpart_bytes(int dim)
 {
     const int nbt = (dim + 15) / 16;
-    return sizeof(float) * (size_t)B1 * (nbt * (nbt + 1) / 2) * 256;
+    return sizeof(float) * (size_t)std::max(B1, PS_HMAX) * (nbt * (nbt + 1) / 2) * 256;
 }

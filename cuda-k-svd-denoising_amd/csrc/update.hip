@@ -21,15 +21,6 @@ namespace {
 
 constexpr int UB = 256;           // patches per users_* workgroup
 constexpr int UW = UB / 64;       // 64-bit mask words per atom per workgroup
-constexpr int BCOLS = 16;         // E columns per upd_build workgroup
-constexpr int BUILD_BLOCKS = 64;  // covers a 1024-user cap in one pass
-
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
-    return v;
-}
 
 __device__ void build_masks(unsigned long long *msk, const int *__restrict__ idx, const float *__restrict__ coef,
                             const int *__restrict__ cnt, int64_t P, int K, int kmax)
@@ -170,16 +161,17 @@ __global__ __launch_bounds__(256) void cols_table_kernel(int K, int kmax, int ca
         tab_p[col] = p;
         tab_self[col] = p * kmax + my;
         const int cp = cnt[p];
+        const int no = kmax > 1 ? kmax - 1 : 1;  // entries per column (one padding entry when kmax = 1)
         int na = 0;
         for (int i = 0; i < cp; i++)
             if (i != my)
             {
                 // .w = 1: atom k-1's coefficient, which the sweep hands over
-                // from K2(k-1) instead of reading it
+                // instead of reading it
                 const int at = idx[(int64_t)p * kmax + i], rk = srank[(int64_t)p * kmax + i];
-                tab_oth[col * (kmax - 1) + na++] = make_int4(at, p * kmax + i, rk, (at == k - 1 && rk >= 0) ? 1 : 0);
+                tab_oth[col * no + na++] = make_int4(at, p * kmax + i, rk, (at == k - 1 && rk >= 0) ? 1 : 0);
             }
-        for (; na < kmax - 1; na++) tab_oth[col * (kmax - 1) + na] = make_int4(-1, 0, -1, 0);
+        for (; na < no; na++) tab_oth[col * no + na] = make_int4(-1, 0, -1, 0);
     }
 }
 

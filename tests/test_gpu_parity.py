@@ -123,9 +123,17 @@ def signfix(D):
     return D * s[:, None], s
 
 
-@pytest.mark.parametrize("cfg,cap", [(CFG1, 1024), (CFG2, 1024), (CFG1, 0)], ids=["128sq_cap", "512sq_cap",
-                                                                                 "128sq_nocap"])
-def test_update_parity(kl, orc, cfg, cap):
+@pytest.mark.parametrize("cfg,cap,sweep", [
+    (CFG1, 1024, "persistent"), (CFG2, 1024, "persistent"), (CFG1, 0, "two_kernel"),
+    (CFG1, 1024, "two_kernel"), (CFG2, 1024, "two_kernel"),
+    (dict(CFG1, k=1), 1024, "persistent"), (dict(CFG1, k=8), 1024, "persistent"),
+    (dict(CFG1, k=8), 1024, "two_kernel"), (dict(CFG1, k=1), 1024, "two_kernel"),
+], ids=["128sq_cap", "512sq_cap", "128sq_nocap", "128sq_cap_2k", "512sq_cap_2k", "128sq_k1", "128sq_k8",
+        "128sq_k8_2k", "128sq_k1_2k"])
+def test_update_parity(kl, orc, cfg, cap, sweep, monkeypatch):
+    # the persistent one-XCD sweep runs whenever the cap allows; the
+    # two-kernel sweep is the one for no cap, or when forced
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1" if sweep == "two_kernel" else "0")
     clean, noisy, ks = make(kl, cfg, user_cap=cap)
     ks.extract()
     ks.init_dictionary()
@@ -152,7 +160,7 @@ def test_update_parity(kl, orc, cfg, cap):
         a = cg[sel] * sg[k]
         b = coo[sel] * so[k]
         cerr[k] = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30)
-    print(f"\nupdate {cfg['W']}^2 cap={cap}: {good.sum()} atoms with gap<0.9; atom err max {atom_err[good].max():.2e}"
+    print(f"\nupdate {cfg['W']}^2 k={cfg['k']} cap={cap} {sweep}: {good.sum()} atoms with gap<0.9; atom err max {atom_err[good].max():.2e}"
           f" (FP64 {atom_err64[good].max():.2e}); coef rel err max {cerr[good].max():.2e}; worst-gap atoms "
           f"err {atom_err[~good].max() if (~good).any() else 0:.2e}")
     assert good.sum() >= 0.9 * (gap >= 0).sum()
