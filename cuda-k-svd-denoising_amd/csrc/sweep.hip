@@ -98,6 +98,35 @@ __device__ __forceinline__ void tile_coords(int ti, int nbt, int *ba, int *bb)
     *bb = a + rem;
 }
 
+// DPP lane moves (VALU, no LDS round trip).  Rows whose bit is clear in
+// row_mask receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum of the 4 lanes of each quad, in every lane of the quad (fixed order).
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += dpp_f<0xB1, 0xF>(v);  // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E, 0xF>(v);  // quad_perm [2,3,0,1]
+    return v;
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order, returned to every
+// lane: quads, half rows, rows (mirrors), then rows 0+1 and 2+3 and the total
+// in lane 63 (row broadcasts), read back as a scalar.
+__device__ __forceinline__ float wave_sum_dpp(float v)
+{
+    v = quad_sum(v);
+    v += dpp_f<0x141, 0xF>(v);  // row_half_mirror
+    v += dpp_f<0x140, 0xF>(v);  // row_mirror
+    v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 template <int TPW, int NO, int BU>
 __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
 {
@@ -273,6 +302,62 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
 // row.  Stops when two successive unit iterates differ by less than sqrt(tol2)
 // (after at least 2 steps) or after maxit steps; a zero product keeps u.
 // Returns the number of steps.
+// Sum over the T2 threads of a workgroup (fixed order), returned to every
+// thread; `red` holds 2 x (T2 / 64) floats and `phase` alternates between
+// calls so that one barrier suffices.
+__device__ __forceinline__ float block_sum_dpp(float v, float *red, int &phase)
+{
+    v = wave_sum_dpp(v);
+    float *r = red + phase * (T2 / 64);
+    phase ^= 1;
+    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < T2 / 64; w++) s += r[w];
+    return s;
+}
+
+// Power iteration from u (LDS) on the symmetric Gs (LDS, pitch ldg), T2
+// threads, 4 per row, each holding its quarter of a row of G in registers
+// (QL >= ceil(dim / 4); u must hold 4*QL floats, zero past dim).  Stops when
+// two successive unit iterates differ by less than sqrt(tol2) (after at least
+// 2 steps) or after maxit steps; a zero product keeps u.  Returns the steps.
+template <int QL>
+__device__ int power_iteration_regs(const float *Gs, int ldg, float *u, int dim, float *red, float tol2, int maxit)
+{
+    const int tid = threadIdx.x;
+    const int row = tid >> 2, part = tid & 3;
+    const int ql = (dim + 3) / 4;
+    const int q0 = part * ql;
+    float g[QL];
+#pragma unroll
+    for (int i = 0; i < QL; i++) {
+        const int q = q0 + i;
+        g[i] = (row < dim && i < ql && q < dim) ? Gs[q * ldg + row] : 0.f;
+    }
+    const bool owner = row < dim && part == 0;
+    int phase = 0;
+    int it = 0;
+    for (;;) {
+        // terms past the thread's range have g = 0 and add exactly nothing
+        float z = 0.f;
+#pragma unroll
+        for (int i = 0; i < QL; i++) z = fmaf(g[i], u[q0 + i], z);
+        z = quad_sum(z);
+        const float nz = sqrtf(block_sum_dpp(owner ? z * z : 0.f, red, phase));
+        if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
+        const float un = owner ? z / nz : 0.f;
+        const float du = owner ? un - u[row] : 0.f;
+        const float d2 = block_sum_dpp(du * du, red, phase);
+        if (owner) u[row] = un;
+        __syncthreads();
+        it++;
+        if ((it >= 2 && d2 < tol2) || it >= maxit) break;
+    }
+    return it;
+}
+
 __device__ int power_iteration(const float *Gs, float *u, int dim, float *red, float tol2, int maxit)
 {
     const int tid = threadIdx.x;
@@ -432,7 +517,7 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 // Every wait is bounded: one that runs out sets an abort word (and a sticky
 // copy the host reads) which every other wait also polls, so all workgroups
 // leave; the host then reports the sweep as failed.
-constexpr int PS_T = 1024;              // threads per workgroup
+constexpr int PS_T = T2;                // threads per workgroup (the power iteration's block size)
 constexpr int PS_W = PS_T / 64;         // wavefronts per workgroup
 constexpr int PS_HMAX = 16;             // helpers at most
 constexpr int PS_CWMAX = 128;           // columns per helper slice
@@ -503,41 +588,70 @@ __device__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *
     return ok;
 }
 
-// x[g] = e_g . u for the PS_CPW columns col[g] of one wavefront (nullptr: no
-// column, x[g] meaningless; col0 is any valid column).  Lane l sums elements l, l+64, l+128 in that order with
-// fmaf, then a fixed butterfly; the loads of all columns are issued first.
-__device__ __forceinline__ void ps_dots(const float *const *col, const float *col0, const float *u, int dim,
+// x[g] = e_g . u for the CPW columns col[g] of one wavefront (nullptr: no
+// column, x[g] meaningless; col0 is any valid column); lane l holds u[l + 64j]
+// in uv[j] (0 past dim).  Lane l sums elements l, l+64, l+128 in that order
+// with fmaf, then wave_sum_dpp; the loads of all columns are issued first.
+template <int CPW>
+__device__ __forceinline__ void ps_dots(const float *const *col, const float *col0, const float *uv, int dim,
                                         float *x)
 {
     const int lane = threadIdx.x & 63;
     // every load is issued (a missing column reads col0, an element beyond dim
     // reads element dim-1) so that none waits on a branch; those values are
     // never used
-    float ev[PS_CPW][3];
+    float ev[CPW][3];
 #pragma unroll
-    for (int g = 0; g < PS_CPW; g++)
+    for (int g = 0; g < CPW; g++)
 #pragma unroll
         for (int j = 0; j < 3; j++) ev[g][j] = ld_c((col[g] ? col[g] : col0) + min(lane + 64 * j, dim - 1));
-    float uv[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) uv[j] = lane + 64 * j < dim ? u[lane + 64 * j] : 0.f;
-#pragma unroll
-    for (int g = 0; g < PS_CPW; g++) {
+    for (int g = 0; g < CPW; g++) {
         float s = 0.f;
 #pragma unroll
         for (int j = 0; j < 3; j++)
             if (lane + 64 * j < dim) s = fmaf(ev[g][j], uv[j], s);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        x[g] = s;
+        x[g] = wave_sum_dpp(s);
     }
 }
 
-__device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm, float *red, int *bail)
+// P2 for the hand-over list hl[0 .. nh): wavefront w takes entries w + PS_W*g,
+// g < CPW.  Column tl = hl[i] of E_k (in LDS) loses d_{k-1} x, x = e . d_{k-1}
+// with e column hrank[tl] of E_{k-1}.
+template <int CPW>
+__device__ __forceinline__ void ps_handover(float *Es, int dpad, int dim, const int *hl, const int *hrank, int nh,
+                                            const float *Eprev, const float *uv)
 {
-    const int tid = threadIdx.x, dim = a.dim;
-    float *Gs = sm;             // dim x dim
-    float *u = Gs + dim * dim;  // dim
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float *colp[CPW];
+    int tls[CPW];
+#pragma unroll
+    for (int g = 0; g < CPW; g++) {
+        const int i = wave + PS_W * g;
+        const int tl = i < nh ? hl[i] : -1;
+        tls[g] = tl;
+        colp[g] = tl >= 0 ? Eprev + (int64_t)hrank[tl] * dpad : nullptr;
+    }
+    float x[CPW];
+    ps_dots<CPW>(colp, Eprev, uv, dim, x);
+#pragma unroll
+    for (int g = 0; g < CPW; g++) {
+        if (tls[g] < 0) continue;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int e = lane + 64 * j;
+            if (e < dim) Es[tls[g] * dpad + e] = fmaf(-uv[j], x[g], Es[tls[g] * dpad + e]);
+        }
+    }
+}
+
+template <int RF, int QL>
+__device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm, int *bail)
+{
+    __shared__ float red[2 * (PS_T / 64)];
+    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
+    float *Gs = sm;               // dpad x dpad (rows and columns past dim are 0)
+    float *u = Gs + dpad * dpad;  // 4 * QL, zero past dim
     const int nf4 = a.ntiles * 64;
     const unsigned gbytes = (unsigned)(H * nf4 * 16);
     unsigned long long iters = 0;
@@ -548,34 +662,48 @@ __device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm,
             if (!ps_wait(flags, PSF_DONE, H, (unsigned)(k + 1), bail)) return;
             const int cw = (m + H - 1) / H;
             const int nb = (m + cw - 1) / cw;
-            // G = sum of the nb partial Grams (helper order), both triangles
-            for (int f = tid; f < nf4; f += PS_T) {
-                f4 s = f4{0.f, 0.f, 0.f, 0.f};
-                for (int b0 = 0; b0 < nb; b0 += 8) {
-                    // unconditional: a load past the buffer's end returns 0;
-                    // partials past nb are read but not added
-                    f4 pv[8];
+            // G = sum of the nb partial Grams in helper order.  Each thread
+            // takes float4 entries f = tid + PS_T * i (i < RF); a pass loads
+            // PB partials of all of them at once (a load past the buffer's
+            // end returns 0; partials past nb are read but not added).  Tiles
+            // cover dpad x dpad, and the padding rows of E are 0, so the
+            // padding of Gs comes out 0.
+            f4 sum[RF];
 #pragma unroll
-                    for (int b = 0; b < 8; b++) pv[b] = ld4_c(a.Gpart, gbytes, (unsigned)(((b0 + b) * nf4 + f) * 16));
+            for (int i = 0; i < RF; i++) sum[i] = f4{0.f, 0.f, 0.f, 0.f};
+            constexpr int PB = RF <= 3 ? 4 : 2;  // partials per pass (register budget)
+            for (int b0 = 0; b0 < nb; b0 += PB) {
+                f4 pv[RF][PB];
 #pragma unroll
-                    for (int b = 0; b < 8; b++)
-                        if (b0 + b < nb) s += pv[b];
+                for (int i = 0; i < RF; i++) {
+                    const int f = min(tid + PS_T * i, nf4 - 1);
+#pragma unroll
+                    for (int b = 0; b < PB; b++)
+                        pv[i][b] = ld4_c(a.Gpart, gbytes, (unsigned)(((b0 + b) * nf4 + f) * 16));
                 }
+#pragma unroll
+                for (int i = 0; i < RF; i++)
+#pragma unroll
+                    for (int b = 0; b < PB; b++)
+                        if (b0 + b < nb) sum[i] += pv[i][b];
+            }
+#pragma unroll
+            for (int i = 0; i < RF; i++) {
+                const int f = tid + PS_T * i;
+                if (f >= nf4) continue;
                 const int ti = f >> 6, l = f & 63;
                 int ba, bb;
                 tile_coords(ti, a.nbt, &ba, &bb);
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int ga = ba * 16 + 4 * (l >> 4) + i, gb = bb * 16 + (l & 15);
-                    if (ga < dim && gb < dim) {
-                        Gs[ga * dim + gb] = s[i];
-                        Gs[gb * dim + ga] = s[i];
-                    }
+                for (int j = 0; j < 4; j++) {
+                    const int ga = ba * 16 + 4 * (l >> 4) + j, gb = bb * 16 + (l & 15);
+                    Gs[ga * dpad + gb] = sum[i][j];
+                    Gs[gb * dpad + ga] = sum[i][j];
                 }
             }
-            for (int e = tid; e < dim; e += PS_T) u[e] = ld_c(a.D + (int64_t)k * dim + e);
+            for (int e = tid; e < 4 * QL; e += PS_T) u[e] = e < dim ? ld_c(a.D + (int64_t)k * dim + e) : 0.f;
             __syncthreads();
-            iters += (unsigned long long)power_iteration(Gs, u, dim, red, a.tol2, a.maxit);
+            iters += (unsigned long long)power_iteration_regs<QL>(Gs, dpad, u, dim, red, a.tol2, a.maxit);
             for (int e = tid; e < dim; e += PS_T) a.D[(int64_t)k * dim + e] = u[e];
         }
         ps_publish(flags + PSF_AFLAG, (unsigned)(k + 1));
@@ -590,16 +718,18 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
     const int dim = a.dim, dpad = a.dpad, kmax = a.kmax, K = a.K;
     const int no = kmax > 1 ? kmax - 1 : 1;
     float *Es = sm;                              // PS_CWMAX x dpad, columns of E_k
-    float *uprev = Es + PS_CWMAX * dpad;         // dpad: d_{k-1}
-    float *sx = uprev + dpad;                    // PS_CWMAX x PS_NOMAX coefficients
+    float *sx = Es + PS_CWMAX * dpad;            // PS_CWMAX x PS_NOMAX coefficients
     int *sat = (int *)(sx + PS_CWMAX * PS_NOMAX);  // PS_CWMAX x PS_NOMAX atoms (-1: none)
     int *sp = sat + PS_CWMAX * PS_NOMAX;         // PS_CWMAX patches
     int *hrank = sp + PS_CWMAX;                  // PS_CWMAX: rank among atom k-1's users, -1: none
+    int *hl = hrank + PS_CWMAX;                  // PS_CWMAX: columns with a hand-over (any order)
+    int *nhp = hl + PS_CWMAX;                    // their number
     for (int k = 0; k <= K; k++) {
         // ---- P1: E_k without atom k-1 ---------------------------------------
         int m = 0, nc = 0, tb = 0;
         int cprev = 0, mprev = 0;
         if (k >= 1) mprev = atom_m(a.uoff, k - 1, a.cap, &cprev);
+        if (tid == 0) *nhp = 0;
         if (k < K) {
             int c;
             m = atom_m(a.uoff, k, a.cap, &c);
@@ -634,6 +764,7 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                         sx[tl * PS_NOMAX + i] = at >= 0 ? xv[i] : 0.f;
                     }
                     hrank[tl] = hr;
+                    if (hr >= 0) hl[atomicAdd(nhp, 1)] = tl;
                 }
                 const int nc4 = (nc + 3) & ~3;
                 for (int q = tid; q < nc4 * dpad; q += PS_T) {
@@ -671,39 +802,35 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
             }
         }
         // ---- W: d_{k-1} -------------------------------------------------------
+        // every wavefront keeps d_{k-1} in registers: uv[j] = d_{k-1}[lane + 64j]
+        float uv[3] = {0.f, 0.f, 0.f};
         if (k >= 1) {
             if (!ps_wait(flags, PSF_AFLAG, 1, (unsigned)k, bail)) return;
-            for (int e = tid; e < dpad; e += PS_T) uprev[e] = e < dim ? ld_c(a.D + (int64_t)(k - 1) * dim + e) : 0.f;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float v = ld_c(a.D + (int64_t)(k - 1) * dim + min(lane + 64 * j, dim - 1));
+                uv[j] = lane + 64 * j < dim ? v : 0.f;
+            }
         }
-        __syncthreads();
         const float *Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
         // ---- P2: atom k-1's share of E_k --------------------------------------
         if (k >= 1 && nc > 0) {
-            const float *colp[PS_CPW];
-            int tls[PS_CPW];
-#pragma unroll
-            for (int g = 0; g < PS_CPW; g++) {
-                const int tl = wave + PS_W * g;
-                const int r = tl < nc ? hrank[tl] : -1;
-                tls[g] = r >= 0 ? tl : -1;
-                colp[g] = r >= 0 ? Eprev + (int64_t)r * dpad : nullptr;
-            }
-            float x[PS_CPW];
-            ps_dots(colp, Eprev, uprev, dim, x);
-#pragma unroll
-            for (int g = 0; g < PS_CPW; g++) {
-                if (tls[g] < 0) continue;
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int e = lane + 64 * j;
-                    if (e < dim) Es[tls[g] * dpad + e] = fmaf(-uprev[e], x[g], Es[tls[g] * dpad + e]);
-                }
-            }
+            const int nh = *nhp;
+            if (nh > 4 * PS_W) ps_handover<8>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
+            else if (nh > 2 * PS_W) ps_handover<4>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
+            else if (nh > PS_W) ps_handover<2>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
+            else if (nh > 0) ps_handover<1>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
         }
         __syncthreads();
         // ---- P3: partial Gram, E_k columns to global --------------------------
         if (k < K) {
             if (nc > 0) {
+                // E_k columns to global first: their stores drain during the MFMAs
+                float *Ecur = (k & 1) ? a.E1 : a.E0;
+                for (int q = tid; q < nc * dim; q += PS_T) {
+                    const int tl = q / dim, e = q - tl * dim;
+                    Ecur[(int64_t)(tb + tl) * dpad + e] = Es[tl * dpad + e];
+                }
                 f4 acc[TPW];
 #pragma unroll
                 for (int q = 0; q < TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
@@ -729,11 +856,6 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                     const int ti = wave + PS_W * q;
                     if (ti < a.ntiles) *(f4 *)(gp + (size_t)ti * 256 + lane * 4) = acc[q];
                 }
-                float *Ecur = (k & 1) ? a.E1 : a.E0;
-                for (int q = tid; q < nc * dim; q += PS_T) {
-                    const int tl = q / dim, e = q - tl * dim;
-                    Ecur[(int64_t)(tb + tl) * dpad + e] = Es[tl * dpad + e];
-                }
             }
             ps_publish(flags + PSF_DONE + h, (unsigned)(k + 1));
         }
@@ -750,7 +872,7 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                 colp[g] = tl < ncp ? Eprev + (int64_t)(tbp + tl) * dpad : nullptr;
             }
             float x[PS_CPW];
-            ps_dots(colp, Eprev, uprev, dim, x);
+            ps_dots<PS_CPW>(colp, Eprev, uv, dim, x);
             if (lane == 0) {
 #pragma unroll
                 for (int g = 0; g < PS_CPW; g++) {
@@ -772,7 +894,6 @@ template <int TPW, int NO, int BU>
 __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *flags, int H)
 {
     extern __shared__ float sm[];
-    __shared__ float red[PS_W];
     __shared__ int s_role, s_bail;
     if (threadIdx.x == 0) {
         const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
@@ -783,7 +904,9 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
     __syncthreads();
     const int role = s_role;
     if (role < 0) return;
-    if (role == 0) ps_master(a, flags, H, sm, red, &s_bail);
+    // per master thread: float4 Gram entries ntiles * 64 <= 1024 * RF, Gram
+    // entries 4 * QL >= dpad (TPW <= 3: dpad <= 144)
+    if (role == 0) ps_master<TPW <= 3 ? 3 : 5, TPW <= 3 ? 36 : 48>(a, flags, H, sm, &s_bail);
     else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, sm, &s_bail);
 }
 
@@ -845,9 +968,9 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
     const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (persistent) {
-        const size_t lds_helper = sizeof(float) * ((size_t)PS_CWMAX * dpad + dpad + (size_t)PS_CWMAX * PS_NOMAX) +
-                                  sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 2 * PS_CWMAX);
-        const size_t lds_master = sizeof(float) * ((size_t)dim * dim + dim);
+        const size_t lds_helper = sizeof(float) * ((size_t)PS_CWMAX * dpad + (size_t)PS_CWMAX * PS_NOMAX) +
+                                  sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1);
+        const size_t lds_master = sizeof(float) * ((size_t)dpad * dpad + 4 * 48);
         const size_t lds = std::max(lds_helper, lds_master);
         typedef void (*PsFn)(SweepArgs, unsigned *, int);
         PsFn ps = nullptr;
