@@ -307,7 +307,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->handx, sizeof(float) * (size_t)tcols * (kmax > 1 ? kmax - 1 : 1)},
         {&ctx->E2, sizeof(float) * (size_t)std::max(dim, 192) * ecols},
         {&ctx->gpart, sweep_gpart_bytes(dim)},
-        {&ctx->ticket, 256},
+        {&ctx->ticket, 1024},  // flags of the persistent sweep
         {&ctx->rsum, sizeof(float) * (size_t)N},
         {&ctx->rweight, sizeof(float) * (size_t)N},
         {&ctx->rout, sizeof(float) * (size_t)N},

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(256) void gather_patches_kernel(const float *__rest
                                                              int pw, int ph, int sw, int sh, int Pc,
                                                              int TP, int TW, float *__restrict__ Y)
 {
-    extern __shared__ float tile[];  // ph x TW
+    extern __shared__ __attribute__((aligned(16))) float tile[];  // ph x TW
     const int pr = blockIdx.y;
     const int pc0 = blockIdx.x * TP;
     const int np = min(TP, Pc - pc0);

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
 {
     constexpr int LD = 4 * Q + 4;
     constexpr int KP = 16 * KT;
-    extern __shared__ float Ds[];  // KP x LD, zero beyond (K, dim)
+    extern __shared__ __attribute__((aligned(16))) float Ds[];  // KP x LD, zero beyond (K, dim)
     for (int q = threadIdx.x; q < KP * LD; q += blockDim.x) {
         const int a = q / LD, e = q - a * LD;
         Ds[q] = (a < K && e < dim) ? D[(int64_t)a * dim + e] : 0.f;

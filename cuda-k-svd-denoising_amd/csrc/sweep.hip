@@ -130,7 +130,7 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
 template <int TPW, int NO, int BU>
 __global__ __launch_bounds__(T1) void k1_kernel(SweepArgs a, int k)
 {
-    extern __shared__ float sm[];
+    extern __shared__ __attribute__((aligned(16))) float sm[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int dim = a.dim, dpad = a.dpad, kmax = a.kmax;
     const int no = kmax > 1 ? kmax - 1 : 1;
@@ -302,33 +302,40 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
 // row.  Stops when two successive unit iterates differ by less than sqrt(tol2)
 // (after at least 2 steps) or after maxit steps; a zero product keeps u.
 // Returns the number of steps.
-// Sum over the T2 threads of a workgroup (fixed order), returned to every
-// thread; `red` holds 2 x (T2 / 64) floats and `phase` alternates between
-// calls so that one barrier suffices.
-__device__ __forceinline__ float block_sum_dpp(float v, float *red, int &phase)
+// Sums of a and of b over the T2 threads of a workgroup (fixed order),
+// returned to every thread; red2 holds 2 x (T2 / 64) float2 and `phase`
+// alternates between calls so that one barrier suffices.
+__device__ __forceinline__ float2 block_sum2(float a, float b, float2 *red2, int &phase)
 {
-    v = wave_sum_dpp(v);
-    float *r = red + phase * (T2 / 64);
+    a = wave_sum_dpp(a);
+    b = wave_sum_dpp(b);
+    float2 *r = red2 + phase * (T2 / 64);
     phase ^= 1;
-    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = make_float2(a, b);
     __syncthreads();
-    float s = 0.f;
+    float2 s = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int w = 0; w < T2 / 64; w++) s += r[w];
+    for (int w = 0; w < T2 / 64; w++) {
+        const float2 v = r[w];
+        s.x += v.x;
+        s.y += v.y;
+    }
     return s;
 }
 
-// Power iteration from u (LDS) on the symmetric Gs (LDS, pitch ldg), T2
-// threads, 4 per row, each holding its quarter of a row of G in registers
-// (QL >= ceil(dim / 4); u must hold 4*QL floats, zero past dim).  Stops when
-// two successive unit iterates differ by less than sqrt(tol2) (after at least
-// 2 steps) or after maxit steps; a zero product keeps u.  Returns the steps.
+// Power iteration u <- G u / |G u| from u (LDS) on the symmetric Gs (LDS,
+// pitch ldg), T2 threads, 4 per row, each holding its quarter of a row of G
+// in registers (quarters of ql = 4 * ceil(dim / 16) entries, ql <= QL; u must
+// hold 4 * QL floats, zero past dim).  Stops at the first iterate u_i with
+// i >= 2 and |u_i - u_{i-1}|^2 < tol2, or at i = maxit; a zero product keeps
+// u.  The test for u_i shares one block reduction with |G u_i|^2, so each step
+// costs one reduction.  Returns i.
 template <int QL>
-__device__ int power_iteration_regs(const float *Gs, int ldg, float *u, int dim, float *red, float tol2, int maxit)
+__device__ int power_iteration_ps(const float *Gs, int ldg, float *u, int dim, float2 *red2, float tol2, int maxit)
 {
     const int tid = threadIdx.x;
     const int row = tid >> 2, part = tid & 3;
-    const int ql = (dim + 3) / 4;
+    const int ql = (dim + 15) / 16 * 4;
     const int q0 = part * ql;
     float g[QL];
 #pragma unroll
@@ -337,23 +344,32 @@ __device__ int power_iteration_regs(const float *Gs, int ldg, float *u, int dim,
         g[i] = (row < dim && i < ql && q < dim) ? Gs[q * ldg + row] : 0.f;
     }
     const bool owner = row < dim && part == 0;
+    float ucur = owner ? u[row] : 0.f, uprev = 0.f;
     int phase = 0;
     int it = 0;
     for (;;) {
         // terms past the thread's range have g = 0 and add exactly nothing
         float z = 0.f;
 #pragma unroll
-        for (int i = 0; i < QL; i++) z = fmaf(g[i], u[q0 + i], z);
+        for (int c = 0; c < QL / 4; c++) {
+            const f4 uu = *(const f4 *)(u + q0 + 4 * c);
+            z = fmaf(g[4 * c], uu.x, z);
+            z = fmaf(g[4 * c + 1], uu.y, z);
+            z = fmaf(g[4 * c + 2], uu.z, z);
+            z = fmaf(g[4 * c + 3], uu.w, z);
+        }
         z = quad_sum(z);
-        const float nz = sqrtf(block_sum_dpp(owner ? z * z : 0.f, red, phase));
+        const float du = ucur - uprev;
+        const float2 sz = block_sum2(owner ? z * z : 0.f, owner ? du * du : 0.f, red2, phase);
+        if ((it >= 2 && sz.y < tol2) || it >= maxit) break;
+        const float nz = sqrtf(sz.x);
         if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
         const float un = owner ? z / nz : 0.f;
-        const float du = owner ? un - u[row] : 0.f;
-        const float d2 = block_sum_dpp(du * du, red, phase);
+        uprev = ucur;
+        ucur = un;
         if (owner) u[row] = un;
         __syncthreads();
         it++;
-        if ((it >= 2 && d2 < tol2) || it >= maxit) break;
     }
     return it;
 }
@@ -387,7 +403,7 @@ __device__ int power_iteration(const float *Gs, float *u, int dim, float *red, f
 
 __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 {
-    extern __shared__ float sm[];
+    extern __shared__ __attribute__((aligned(16))) float sm[];
     __shared__ float red[T2 / 64];
     const int tid = threadIdx.x;
     const int dim = a.dim;
@@ -498,33 +514,47 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 // is read with ordinary loads.
 //
 // Phase k (atom k); helper h owns columns [h*cw, (h+1)*cw) of atom k's m
-// capped users, cw = ceil(m / H):
-//   P1 (before atom k-1 is final) builds its columns of E_k in LDS from every
+// capped users, cw = ceil(m / H), and keeps them in LDS buffer k & 1:
+//   P1 (before atom k-1 is final) builds its columns of E_k from every
 //      support entry except atom k-1's, once every helper has written atom
 //      k-2's final coefficients (cdone = k);
 //   W  waits until the master has published d_{k-1} (aflag = k);
 //   P2 for each column whose patch is among atom k-1's first m_{k-1} users
-//      (rank r), x = e_{k-1,r} . d_{k-1} -- the same arithmetic on the same
-//      values as P4's stored coefficient, so both agree bit for bit -- and
+//      (rank r), x = e_{k-1,r} . d_{k-1} (column r from global memory) and
 //      e -= d_{k-1} x;
-//   P3 partial Gram of its columns on MFMA -> Gpart[h], its E_k columns ->
-//      global; done[h] = k + 1;
-//   P4 atom k-1's final coefficients for its share of atom k-1's users (0 for
-//      users beyond the cap); cdone[h] = k + 1.
-// Master, phase k: waits for done[*] = k + 1, sums the partial Grams in helper
-// order into LDS, runs the power iteration from d_k, writes d_k and sets
-// aflag = k + 1.  After phase K - 1 the helpers run P4 for atom K - 1.
+//   P3 its E_k columns -> global, partial Gram of its columns on MFMA ->
+//      Gpart[h]; done[h] = k + 1;
+//   RS once every done[*] = k + 1: sums the partials of the Gram tiles
+//      h, h + H, ... in helper order -> Gred; rdone[h] = k + 1;
+//   P4 atom k-1's final coefficients for its own columns of atom k-1 (its
+//      other LDS buffer) -- the same arithmetic on the same values as P2, so
+//      both agree bit for bit -- and 0 for its share of the users beyond the
+//      cap; cdone[h] = k + 1.
+// Master, phase k: once every rdone[*] = k + 1, reads Gred into LDS, runs the
+// power iteration from d_k, writes d_k and sets aflag = k + 1.  After phase
+// K - 1 the helpers run P4 for atom K - 1.
 // Every wait is bounded: one that runs out sets an abort word (and a sticky
 // copy the host reads) which every other wait also polls, so all workgroups
 // leave; the host then reports the sweep as failed.
 constexpr int PS_T = T2;                // threads per workgroup (the power iteration's block size)
 constexpr int PS_W = PS_T / 64;         // wavefronts per workgroup
-constexpr int PS_HMAX = 16;             // helpers at most
-constexpr int PS_CWMAX = 128;           // columns per helper slice
+constexpr int PS_HMAX = 31;             // helpers at most (one master + 31 helpers fill XCD 0's 32 CUs)
+constexpr int PS_CWMAX = 64;            // columns per helper slice
 constexpr int PS_CPW = PS_CWMAX / PS_W; // columns per wavefront in the dot passes
 constexpr int PS_NOMAX = 7;             // other support entries per column (ompIter <= 8)
 constexpr long PS_SPIN_MAX = 1L << 21;  // polls before a wait gives up
-enum { PSF_REG = 0, PSF_DONE = 16, PSF_CDONE = 32, PSF_AFLAG = 48, PSF_ABORT = 49, PSF_STICKY = 50, PSF_WORDS = 64 };
+// flag words: per-XCD registration; per-helper done, rdone, cdone; the
+// master's aflag; the abort word and its sticky copy
+enum {
+    PSF_REG = 0,
+    PSF_DONE = 32,
+    PSF_RDONE = 64,
+    PSF_CDONE = 96,
+    PSF_AFLAG = 128,
+    PSF_ABORT = 129,
+    PSF_STICKY = 130,
+    PSF_WORDS = 160
+};
 
 __device__ __forceinline__ unsigned ld_flag(const unsigned *p)
 {
@@ -542,7 +572,7 @@ __device__ __forceinline__ float ld_c(const float *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// sc1 16-byte load, `off` bytes into a buffer of `bytes` bytes
+// sc1 16-byte load, `off` bytes into a buffer of `bytes` bytes (0 past its end)
 __device__ __forceinline__ f4 ld4_c(const float *base, unsigned bytes, unsigned off)
 {
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), 0, bytes, 0x00020000);
@@ -588,122 +618,91 @@ __device__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *
     return ok;
 }
 
-// x[g] = e_g . u for the CPW columns col[g] of one wavefront (nullptr: no
-// column, x[g] meaningless; col0 is any valid column); lane l holds u[l + 64j]
-// in uv[j] (0 past dim).  Lane l sums elements l, l+64, l+128 in that order
-// with fmaf, then wave_sum_dpp; the loads of all columns are issued first.
-template <int CPW>
-__device__ __forceinline__ void ps_dots(const float *const *col, const float *col0, const float *uv, int dim,
-                                        float *x)
+// e . u for one column held by a wavefront as ev[j] = e[lane + 64j], with u
+// held the same way (0 past dim): lane partials in j order with fmaf, then
+// wave_sum_dpp.  The hand-over (P2) and the stored coefficient (P4) both use
+// it on the same values, so they agree bit for bit.
+__device__ __forceinline__ float ps_dot(const float *ev, const float *uv, int dim)
 {
     const int lane = threadIdx.x & 63;
-    // every load is issued (a missing column reads col0, an element beyond dim
-    // reads element dim-1) so that none waits on a branch; those values are
-    // never used
-    float ev[CPW][3];
+    float s = 0.f;
 #pragma unroll
-    for (int g = 0; g < CPW; g++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) ev[g][j] = ld_c((col[g] ? col[g] : col0) + min(lane + 64 * j, dim - 1));
-#pragma unroll
-    for (int g = 0; g < CPW; g++) {
-        float s = 0.f;
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            if (lane + 64 * j < dim) s = fmaf(ev[g][j], uv[j], s);
-        x[g] = wave_sum_dpp(s);
-    }
+    for (int j = 0; j < 3; j++)
+        if (lane + 64 * j < dim) s = fmaf(ev[j], uv[j], s);
+    return wave_sum_dpp(s);
 }
 
 // P2 for the hand-over list hl[0 .. nh): wavefront w takes entries w + PS_W*g,
-// g < CPW.  Column tl = hl[i] of E_k (in LDS) loses d_{k-1} x, x = e . d_{k-1}
-// with e column hrank[tl] of E_{k-1}.
+// g < CPW.  Column tl = hl[i] of E_k (LDS) loses d_{k-1} x, x = e . d_{k-1}
+// with e column hrank[tl] of E_{k-1} (global; all loads issued first).
 template <int CPW>
 __device__ __forceinline__ void ps_handover(float *Es, int dpad, int dim, const int *hl, const int *hrank, int nh,
                                             const float *Eprev, const float *uv)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const float *colp[CPW];
+    float ev[CPW][3];
     int tls[CPW];
 #pragma unroll
     for (int g = 0; g < CPW; g++) {
         const int i = wave + PS_W * g;
         const int tl = i < nh ? hl[i] : -1;
         tls[g] = tl;
-        colp[g] = tl >= 0 ? Eprev + (int64_t)hrank[tl] * dpad : nullptr;
+        const float *col = Eprev + (int64_t)(tl >= 0 ? hrank[tl] : 0) * dpad;
+#pragma unroll
+        for (int j = 0; j < 3; j++) ev[g][j] = ld_c(col + min(lane + 64 * j, dim - 1));
     }
-    float x[CPW];
-    ps_dots<CPW>(colp, Eprev, uv, dim, x);
 #pragma unroll
     for (int g = 0; g < CPW; g++) {
-        if (tls[g] < 0) continue;
+        const int tl = tls[g];
+        if (tl < 0) continue;
+        const float x = ps_dot(ev[g], uv, dim);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int e = lane + 64 * j;
-            if (e < dim) Es[tls[g] * dpad + e] = fmaf(-uv[j], x[g], Es[tls[g] * dpad + e]);
+            if (e < dim) Es[tl * dpad + e] = fmaf(-uv[j], x, Es[tl * dpad + e]);
         }
     }
 }
 
-template <int RF, int QL>
-__device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm, int *bail)
+template <int QL>
+__device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, const float *Gred, float *sm, int *bail)
 {
-    __shared__ float red[2 * (PS_T / 64)];
+    __shared__ float2 red2[2 * (PS_T / 64)];
     const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
     float *Gs = sm;               // dpad x dpad (rows and columns past dim are 0)
     float *u = Gs + dpad * dpad;  // 4 * QL, zero past dim
     const int nf4 = a.ntiles * 64;
-    const unsigned gbytes = (unsigned)(H * nf4 * 16);
+    const unsigned rbytes = (unsigned)(nf4 * 16);
     unsigned long long iters = 0;
     for (int k = 0; k < a.K; k++) {
         int c;
         const int m = atom_m(a.uoff, k, a.cap, &c);
         if (m > 0) {
-            if (!ps_wait(flags, PSF_DONE, H, (unsigned)(k + 1), bail)) return;
-            const int cw = (m + H - 1) / H;
-            const int nb = (m + cw - 1) / cw;
-            // G = sum of the nb partial Grams in helper order.  Each thread
-            // takes float4 entries f = tid + PS_T * i (i < RF); a pass loads
-            // PB partials of all of them at once (a load past the buffer's
-            // end returns 0; partials past nb are read but not added).  Tiles
-            // cover dpad x dpad, and the padding rows of E are 0, so the
-            // padding of Gs comes out 0.
-            f4 sum[RF];
+            if (!ps_wait(flags, PSF_RDONE, H, (unsigned)(k + 1), bail)) return;
+            // the reduced upper tiles, both triangles; tiles cover dpad x dpad
+            // and the padding rows of E are 0, so the padding of Gs is 0
+            for (int f0 = tid; f0 < nf4; f0 += 4 * PS_T) {
+                f4 v[4];
 #pragma unroll
-            for (int i = 0; i < RF; i++) sum[i] = f4{0.f, 0.f, 0.f, 0.f};
-            constexpr int PB = RF <= 3 ? 4 : 2;  // partials per pass (register budget)
-            for (int b0 = 0; b0 < nb; b0 += PB) {
-                f4 pv[RF][PB];
+                for (int i = 0; i < 4; i++) v[i] = ld4_c(Gred, rbytes, (unsigned)((f0 + PS_T * i) * 16));
 #pragma unroll
-                for (int i = 0; i < RF; i++) {
-                    const int f = min(tid + PS_T * i, nf4 - 1);
+                for (int i = 0; i < 4; i++) {
+                    const int f = f0 + PS_T * i;
+                    if (f >= nf4) continue;
+                    const int ti = f >> 6, l = f & 63;
+                    int ba, bb;
+                    tile_coords(ti, a.nbt, &ba, &bb);
 #pragma unroll
-                    for (int b = 0; b < PB; b++)
-                        pv[i][b] = ld4_c(a.Gpart, gbytes, (unsigned)(((b0 + b) * nf4 + f) * 16));
-                }
-#pragma unroll
-                for (int i = 0; i < RF; i++)
-#pragma unroll
-                    for (int b = 0; b < PB; b++)
-                        if (b0 + b < nb) sum[i] += pv[i][b];
-            }
-#pragma unroll
-            for (int i = 0; i < RF; i++) {
-                const int f = tid + PS_T * i;
-                if (f >= nf4) continue;
-                const int ti = f >> 6, l = f & 63;
-                int ba, bb;
-                tile_coords(ti, a.nbt, &ba, &bb);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int ga = ba * 16 + 4 * (l >> 4) + j, gb = bb * 16 + (l & 15);
-                    Gs[ga * dpad + gb] = sum[i][j];
-                    Gs[gb * dpad + ga] = sum[i][j];
+                    for (int j = 0; j < 4; j++) {
+                        const int ga = ba * 16 + 4 * (l >> 4) + j, gb = bb * 16 + (l & 15);
+                        Gs[ga * dpad + gb] = v[i][j];
+                        Gs[gb * dpad + ga] = v[i][j];
+                    }
                 }
             }
             for (int e = tid; e < 4 * QL; e += PS_T) u[e] = e < dim ? ld_c(a.D + (int64_t)k * dim + e) : 0.f;
             __syncthreads();
-            iters += (unsigned long long)power_iteration_regs<QL>(Gs, dpad, u, dim, red, a.tol2, a.maxit);
+            iters += (unsigned long long)power_iteration_ps<QL>(Gs, dpad, u, dim, red2, a.tol2, a.maxit);
             for (int e = tid; e < dim; e += PS_T) a.D[(int64_t)k * dim + e] = u[e];
         }
         ps_publish(flags + PSF_AFLAG, (unsigned)(k + 1));
@@ -712,19 +711,23 @@ __device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, float *sm,
 }
 
 template <int TPW, int NO, int BU>
-__device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, float *sm, int *bail)
+__device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, float *Gred, float *sm, int *bail)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int dim = a.dim, dpad = a.dpad, kmax = a.kmax, K = a.K;
     const int no = kmax > 1 ? kmax - 1 : 1;
-    float *Es = sm;                              // PS_CWMAX x dpad, columns of E_k
-    float *sx = Es + PS_CWMAX * dpad;            // PS_CWMAX x PS_NOMAX coefficients
-    int *sat = (int *)(sx + PS_CWMAX * PS_NOMAX);  // PS_CWMAX x PS_NOMAX atoms (-1: none)
-    int *sp = sat + PS_CWMAX * PS_NOMAX;         // PS_CWMAX patches
-    int *hrank = sp + PS_CWMAX;                  // PS_CWMAX: rank among atom k-1's users, -1: none
-    int *hl = hrank + PS_CWMAX;                  // PS_CWMAX: columns with a hand-over (any order)
-    int *nhp = hl + PS_CWMAX;                    // their number
+    const int nf4 = a.ntiles * 64;
+    const unsigned gbytes = (unsigned)(H * nf4 * 16);
+    float *Esb = sm;                                   // 2 x PS_CWMAX x dpad: columns of E_k in buffer k & 1
+    float *sx = Esb + 2 * PS_CWMAX * dpad;             // PS_CWMAX x PS_NOMAX coefficients
+    int *sat = (int *)(sx + PS_CWMAX * PS_NOMAX);      // PS_CWMAX x PS_NOMAX atoms (-1: none)
+    int *sp = sat + PS_CWMAX * PS_NOMAX;               // PS_CWMAX patches
+    int *hrank = sp + PS_CWMAX;                        // PS_CWMAX: rank among atom k-1's users, -1: none
+    int *hl = hrank + PS_CWMAX;                        // PS_CWMAX: columns with a hand-over (any order)
+    int *nhp = hl + PS_CWMAX;                          // their number
     for (int k = 0; k <= K; k++) {
+        float *Es = Esb + (k & 1) * PS_CWMAX * dpad;
+        const float *Eo = Esb + ((k + 1) & 1) * PS_CWMAX * dpad;  // this helper's columns of E_{k-1}
         // ---- P1: E_k without atom k-1 ---------------------------------------
         int m = 0, nc = 0, tb = 0;
         int cprev = 0, mprev = 0;
@@ -812,20 +815,19 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                 uv[j] = lane + 64 * j < dim ? v : 0.f;
             }
         }
-        const float *Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
         // ---- P2: atom k-1's share of E_k --------------------------------------
         if (k >= 1 && nc > 0) {
+            const float *Eprev = ((k - 1) & 1) ? a.E1 : a.E0;
             const int nh = *nhp;
-            if (nh > 4 * PS_W) ps_handover<8>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
-            else if (nh > 2 * PS_W) ps_handover<4>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
+            if (nh > 2 * PS_W) ps_handover<4>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
             else if (nh > PS_W) ps_handover<2>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
             else if (nh > 0) ps_handover<1>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
         }
         __syncthreads();
-        // ---- P3: partial Gram, E_k columns to global --------------------------
+        // ---- P3: E_k columns to global, partial Gram ---------------------------
         if (k < K) {
             if (nc > 0) {
-                // E_k columns to global first: their stores drain during the MFMAs
+                // stores first: they drain during the MFMAs
                 float *Ecur = (k & 1) ? a.E1 : a.E0;
                 for (int q = tid; q < nc * dim; q += PS_T) {
                     const int tl = q / dim, e = q - tl * dim;
@@ -850,7 +852,7 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                         }
                     }
                 }
-                float *gp = a.Gpart + (size_t)h * a.ntiles * 256;
+                float *gp = a.Gpart + (size_t)h * nf4 * 4;
 #pragma unroll
                 for (int q = 0; q < TPW; q++) {
                     const int ti = wave + PS_W * q;
@@ -858,27 +860,42 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                 }
             }
             ps_publish(flags + PSF_DONE + h, (unsigned)(k + 1));
+            // ---- RS: this helper's tiles of the Gram, summed over the partials
+            if (m > 0) {
+                if (!ps_wait(flags, PSF_DONE, H, (unsigned)(k + 1), bail)) return;
+                const int cw = (m + H - 1) / H;
+                const int nb = (m + cw - 1) / cw;
+                for (int ti = h + H * wave; ti < a.ntiles; ti += H * PS_W) {
+                    const int f = ti * 64 + lane;
+                    f4 sum = f4{0.f, 0.f, 0.f, 0.f};
+                    for (int b0 = 0; b0 < nb; b0 += 8) {
+                        f4 pv[8];
+#pragma unroll
+                        for (int b = 0; b < 8; b++) pv[b] = ld4_c(a.Gpart, gbytes, (unsigned)(((b0 + b) * nf4 + f) * 16));
+#pragma unroll
+                        for (int b = 0; b < 8; b++)
+                            if (b0 + b < nb) sum += pv[b];
+                    }
+                    *(f4 *)(Gred + (size_t)f * 4) = sum;
+                }
+            }
+            ps_publish(flags + PSF_RDONE + h, (unsigned)(k + 1));
         }
-        // ---- P4: atom k-1's final coefficients --------------------------------
+        // ---- P4: atom k-1's final coefficients, from this helper's LDS copy ---
         if (k >= 1 && cprev > 0) {
             const int cwp = (mprev + H - 1) / H;
             const int tbp = h * cwp;
             const int ncp = max(0, min(mprev, tbp + cwp) - tbp);
             const int64_t cbp = a.moff[k - 1] + tbp;
-            const float *colp[PS_CPW];
 #pragma unroll
             for (int g = 0; g < PS_CPW; g++) {
                 const int tl = wave + PS_W * g;
-                colp[g] = tl < ncp ? Eprev + (int64_t)(tbp + tl) * dpad : nullptr;
-            }
-            float x[PS_CPW];
-            ps_dots<PS_CPW>(colp, Eprev, uv, dim, x);
-            if (lane == 0) {
+                if (tl >= ncp) continue;
+                float ev[3];
 #pragma unroll
-                for (int g = 0; g < PS_CPW; g++) {
-                    const int tl = wave + PS_W * g;
-                    if (tl < ncp) a.coef[a.tab_self[cbp + tl]] = x[g];
-                }
+                for (int jj = 0; jj < 3; jj++) ev[jj] = Eo[tl * dpad + min(lane + 64 * jj, dim - 1)];
+                const float x = ps_dot(ev, uv, dim);
+                if (lane == 0) a.coef[a.tab_self[cbp + tl]] = x;
             }
             const int basep = a.uoff[k - 1];
             for (int t = mprev + h * PS_T + tid; t < cprev; t += H * PS_T) {
@@ -891,9 +908,9 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
 }
 
 template <int TPW, int NO, int BU>
-__global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *flags, int H)
+__global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *flags, int H, float *Gred)
 {
-    extern __shared__ float sm[];
+    extern __shared__ __attribute__((aligned(16))) float sm[];
     __shared__ int s_role, s_bail;
     if (threadIdx.x == 0) {
         const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
@@ -904,10 +921,9 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
     __syncthreads();
     const int role = s_role;
     if (role < 0) return;
-    // per master thread: float4 Gram entries ntiles * 64 <= 1024 * RF, Gram
-    // entries 4 * QL >= dpad (TPW <= 3: dpad <= 144)
-    if (role == 0) ps_master<TPW <= 3 ? 3 : 5, TPW <= 3 ? 36 : 48>(a, flags, H, sm, &s_bail);
-    else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, sm, &s_bail);
+    // Gram entries per master thread: 4 * QL >= dpad (TPW <= 3: dpad <= 144)
+    if (role == 0) ps_master<TPW <= 3 ? 36 : 48>(a, flags, H, Gred, sm, &s_bail);
+    else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, Gred, sm, &s_bail);
 }
 
 // Clears the sweep's flags (all but the sticky abort word) before a launch.
@@ -963,16 +979,16 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
 
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
-    int H = 8;
+    int H = 16;
     if (const char *hv = getenv("KSVD_SWEEP_HELPERS")) H = std::max(1, std::min(PS_HMAX, atoi(hv)));
     if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
     const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (persistent) {
-        const size_t lds_helper = sizeof(float) * ((size_t)PS_CWMAX * dpad + (size_t)PS_CWMAX * PS_NOMAX) +
-                                  sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1);
+        const size_t lds_helper = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (size_t)PS_CWMAX * PS_NOMAX) +
+                                  sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 3 * PS_CWMAX + 1);
         const size_t lds_master = sizeof(float) * ((size_t)dpad * dpad + 4 * 48);
         const size_t lds = std::max(lds_helper, lds_master);
-        typedef void (*PsFn)(SweepArgs, unsigned *, int);
+        typedef void (*PsFn)(SweepArgs, unsigned *, int, float *);
         PsFn ps = nullptr;
         if (!wide) ps = tpw <= 1 ? ps_sweep_kernel<1, 4, 4> : tpw <= 2 ? ps_sweep_kernel<2, 4, 4>
                       : tpw <= 3 ? ps_sweep_kernel<3, 4, 4> : ps_sweep_kernel<5, 4, 4>;
@@ -980,9 +996,11 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                  : tpw <= 3 ? ps_sweep_kernel<3, 7, 2> : ps_sweep_kernel<5, 7, 2>;
         KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)ps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned *flags = (unsigned *)ctx->ticket.ptr;
-        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, flags);
+        // the reduced Gram follows the partials in the same buffer
+        float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
+        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags);
         // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on XCD 0
-        hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H);
+        hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
     }
@@ -1028,5 +1046,6 @@ int check_sweep(ksvd_ctx *ctx)
 size_t sweep_gpart_bytes(int dim)
 {
     const int nbt = (dim + 15) / 16;
-    return sizeof(float) * (size_t)std::max(B1, PS_HMAX) * (nbt * (nbt + 1) / 2) * 256;
+    // B1 (or PS_HMAX) partial Grams, then the reduced one
+    return sizeof(float) * (size_t)(std::max(B1, PS_HMAX) + 1) * (nbt * (nbt + 1) / 2) * 256;
 }
