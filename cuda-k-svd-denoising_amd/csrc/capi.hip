@@ -122,7 +122,8 @@ int check_params(ksvd_ctx *ctx, const ksvd_params *p)
         return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 190 not supported by the rank-1 update");
     if (p->precision != KSVD_PREC_FP32 && p->precision != KSVD_PREC_BF16)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown precision");
-    if (p->precision == KSVD_PREC_BF16) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "BF16 form not built");
+    if (p->precision == KSVD_PREC_BF16 && (dim > 144 || p->atoms > 256 || p->omp_iter > 8))
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the BF16 form needs patch dim <= 144, atoms <= 256, ompIter <= 8");
     if (p->geometry != KSVD_GEOM_TRUE && p->geometry != KSVD_GEOM_COMPAT)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown geometry");
     return KSVD_OK;
@@ -234,6 +235,11 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
     if (rc) return rc;
     rc = ksvd_reserve(ctx, ctx->rlam, 16);
     if (rc) return rc;
+    if (p->precision == KSVD_PREC_BF16) {
+        if ((rc = ksvd_reserve(ctx, ctx->Dbf, bf16_dict_bytes(ctx->dim)))) return rc;
+        if ((rc = ksvd_reserve(ctx, ctx->Dq, bf16_dictq_bytes(ctx->dim)))) return rc;
+        if ((rc = ksvd_reserve(ctx, ctx->bfscal, 256))) return rc;
+    }
 
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return KSVD_OK;
@@ -251,7 +257,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
-                      &ctx->rout, &ctx->rpart,  &ctx->rlam};
+                      &ctx->rout, &ctx->rpart,  &ctx->rlam, &ctx->Dbf, &ctx->Dq, &ctx->bfscal};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -576,6 +582,9 @@ int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
     ctx->tm.omp_dup_stops = (int64_t)c[CNT_DUP];
     ctx->tm.omp_degenerate = (int64_t)c[CNT_DEGEN];
     ctx->tm.eig_iters = (int64_t)c[CNT_EIG_IT];
+    ctx->tm.omp_rescored = (int64_t)c[CNT_RESCORE];
+    ctx->tm.omp_f64_refactors = (int64_t)c[CNT_F64];
+    ctx->tm.omp_candidates_cut = (int64_t)c[CNT_CAND_CUT];
     *t = ctx->tm;
     return KSVD_OK;
 }

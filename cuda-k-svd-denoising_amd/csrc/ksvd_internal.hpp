@@ -15,7 +15,11 @@ constexpr int KSVD_MAX_SUPPORT = 16;
 constexpr int KSVD_MAX_DIM = 1024;
 
 // Device counters, one slot each.
-enum { CNT_DUP = 0, CNT_DEGEN = 1, CNT_EIG_IT = 2, CNT_SLOTS = 8 };
+// Slot 3 counts sweep hand-over lists that overflowed (sweep.hip); slots 4-6
+// belong to the BF16 pursuit: patch-steps re-scored in FP32, FP64
+// re-factorisations, patch-steps whose candidate list was cut at 8.
+enum { CNT_DUP = 0, CNT_DEGEN = 1, CNT_EIG_IT = 2, CNT_HAND_OVF = 3, CNT_RESCORE = 4, CNT_F64 = 5,
+       CNT_CAND_CUT = 6, CNT_SLOTS = 8 };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -47,6 +51,7 @@ struct ksvd_ctx {
     DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
     DevBuf srank, handx, E2, gpart, ticket; // sweep kernels (sweep.hip); ticket: flags of the persistent sweep
     DevBuf rsum, rweight, rout, rpart, rlam;
+    DevBuf Dbf, Dq, bfscal;  // BF16 form: LDS image of D, padded FP32 atoms, largest atom norm
 
     ksvd_timings tm{};
     std::string err;
@@ -73,6 +78,9 @@ int launch_gather(ksvd_ctx *ctx);
 int launch_init_dict(ksvd_ctx *ctx);
 int launch_dict_gram(ksvd_ctx *ctx);
 int launch_omp(ksvd_ctx *ctx);
+int launch_omp_bf16(ksvd_ctx *ctx);
+size_t bf16_dict_bytes(int dim);
+size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);
 void drop_update_graph(ksvd_ctx *ctx);
 size_t sweep_gpart_bytes(int dim);

@@ -423,6 +423,7 @@ int launch_omp(ksvd_ctx *ctx)
         int rc = launch_dict_gram(ctx);
         if (rc) return rc;
     }
+    if (ctx->prm.precision == KSVD_PREC_BF16) return launch_omp_bf16(ctx);
     if (mfma_ok(ctx)) return ctx->dim <= 64 ? launch_mfma<16>(ctx) : launch_mfma<36>(ctx);
     const int rj = (ctx->dim + 63) / 64;
     if (rj <= 1) return launch_generic<1>(ctx);

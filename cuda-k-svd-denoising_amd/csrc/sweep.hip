@@ -496,7 +496,7 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
         const float x = wave_dot(Ek + (int64_t)r * a.dpad, u, dim);
         if (lane == 0) a.handx[cb1 * no + q] = x;
     }
-    if (hcount > HAND_MAX && tid == 0) atomicAdd(&a.counters[CNT_DEGEN + 2], 1ull);
+    if (hcount > HAND_MAX && tid == 0) atomicAdd(&a.counters[CNT_HAND_OVF], 1ull);
 }
 
 // ===========================================================================

@@ -55,6 +55,9 @@ class Timings(ctypes.Structure):
         ("omp_dup_stops", ctypes.c_int64),
         ("omp_degenerate", ctypes.c_int64),
         ("eig_iters", ctypes.c_int64),
+        ("omp_rescored", ctypes.c_int64),
+        ("omp_f64_refactors", ctypes.c_int64),
+        ("omp_candidates_cut", ctypes.c_int64),
     ]
 
 

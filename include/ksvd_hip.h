@@ -82,6 +82,9 @@ typedef struct ksvd_timings {
     int64_t omp_dup_stops;  /* pursuits ended by a repeated atom pick */
     int64_t omp_degenerate; /* pursuits ended by a non-positive pivot */
     int64_t eig_iters;      /* power iterations spent in rank-1 updates */
+    int64_t omp_rescored;   /* BF16 form: patch-steps whose pick was re-scored in FP32 */
+    int64_t omp_f64_refactors; /* BF16 form: support Grams re-factorised in FP64 */
+    int64_t omp_candidates_cut; /* BF16 form: patch-steps with more than 8 candidates */
 } ksvd_timings;
 
 typedef struct ksvd_ctx ksvd_ctx;

@@ -1,0 +1,126 @@
+"""BF16 precision form (KSVD_PREC_BF16) of the HIP path, through the C ABI.
+
+The correlation D^T r runs in BF16 on MFMA; every atom whose BF16 score is
+within the proven BF16 error margin of the best is re-scored in FP32, so the
+pick is the FP32 argmax.  Least squares stay in FP32 (FP64 re-factorisation on
+pivots below 1e-6 relative).  Bars:
+  * OMP on the same dictionary: supports identical to the FP32 oracle's on
+    >= 99.9 % of patches, coefficients within 1e-4 relative there;
+  * FP64 re-factorisation: fires on a near-collinear pair of atoms and gives
+    the least-squares residual of an FP64 solve;
+  * end to end: denoised PSNR within 0.05 dB of the plain FP32 form.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import CFG1, CFG2, coef_rel_err, make, run_oracle, support_agreement
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cfg", [CFG1, CFG2], ids=["128sq_8x8", "512sq_12x12"])
+def test_bf16_omp_supports(kl, orc, cfg):
+    clean, noisy, ks = make(kl, cfg, precision=kl.PREC_BF16)
+    ks.extract()
+    ks.init_dictionary()
+    ks.reset_timings()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    Y = ks.patches()
+    D = ks.dictionary()
+    io, co, no, _ = orc.omp(Y, D, cfg["k"], real="f32")
+    i64, c64, n64, _ = orc.omp(Y, D.astype(np.float64), cfg["k"], real="f64")
+    same = support_agreement(idx, cnt, io, no)
+    err = coef_rel_err(idx, coef, io, co, cnt, same)
+    same64 = support_agreement(idx, cnt, i64, n64)
+    t = ks.timings()
+    steps = int(cnt.sum())
+    print(f"\nBF16 OMP {cfg['W']}^2: supports equal {same.mean() * 100:.4f}% (FP64 oracle "
+          f"{same64.mean() * 100:.4f}%); coef rel err max {err.max():.2e}; re-scored {t['omp_rescored']} of "
+          f"{steps} steps, candidate lists cut {t['omp_candidates_cut']}, FP64 refactors "
+          f"{t['omp_f64_refactors']}")
+    assert same.mean() >= 0.999
+    assert err.max() <= 1e-4
+    ks.close()
+
+
+def test_bf16_f64_refactor(kl):
+    """Atoms 0 and 1 are 5e-4 rad apart in coordinates 0-7; the other atoms
+    are unit vectors of coordinates 8-37 (exactly orthogonal to them, exact
+    in BF16).  Patches 300 d0 + 2 u + two unit-vector terms make the pursuit
+    take atom 1, the two unit vectors, then atom 0, whose pivot is ~2.5e-7
+    of its Gram diagonal: the support Gram is re-factorised in FP64.  The
+    resulting codes must reach the FP64 least-squares residual."""
+    rng = np.random.default_rng(7)
+    dim, K, P = 64, 32, 289
+    d0 = np.zeros(dim)
+    d0[:8] = rng.standard_normal(8)
+    d0 /= np.linalg.norm(d0)
+    u = np.zeros(dim)
+    u[:8] = rng.standard_normal(8)
+    u -= (u @ d0) * d0
+    u /= np.linalg.norm(u)
+    D = np.zeros((K, dim))
+    D[0] = d0
+    D[1] = np.cos(5e-4) * d0 + np.sin(5e-4) * u
+    for k in range(2, K):
+        D[k, 6 + k] = 1.0
+    D = D.astype(np.float32)
+    Y = np.zeros((P, dim))
+    for p in range(P):
+        a, b = rng.choice(np.arange(2, K), 2, replace=False)
+        Y[p] = rng.uniform(200, 400) * D[0] + rng.uniform(1, 3) * u + 50.0 * D[a] + 40.0 * D[b]
+    Y = Y.astype(np.float32)
+    # build a frame of non-overlapping 8x8 tiles (slide 8) holding the patches
+    n_side = int(np.ceil(np.sqrt(P)))
+    m = n = 8 * n_side
+    frame = np.zeros((n, m), np.float32)
+    for p in range(P):
+        a, b = divmod(p, n_side)
+        frame[8 * a:8 * a + 8, 8 * b:8 * b + 8] = Y[p].reshape(8, 8)
+    ks = kl.KSvd(patch=(8, 8), slide=(8, 8), atoms=K, omp_iter=4, precision=kl.PREC_BF16)
+    ks.set_frame(frame, m, n)
+    ks.extract()
+    Yg = ks.patches()[:P]
+    np.testing.assert_array_equal(Yg, Y)
+    ks.set_dictionary(D)
+    ks.reset_timings()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    t = ks.timings()
+    print(f"\nFP64 refactors {t['omp_f64_refactors']}, degenerate stops {t['omp_degenerate']}")
+    assert t["omp_f64_refactors"] > 0
+    both = [p for p in range(P) if {0, 1} <= set(idx[p, :cnt[p]])]
+    assert len(both) > 0.5 * P
+    worst = 0.0
+    for p in both:
+        S = idx[p, :cnt[p]]
+        A = D[S].astype(np.float64).T
+        y = Y[p].astype(np.float64)
+        r_gpu = np.linalg.norm(y - A @ coef[p, :cnt[p]].astype(np.float64))
+        x64 = np.linalg.lstsq(A, y, rcond=None)[0]
+        r64 = np.linalg.norm(y - A @ x64)
+        worst = max(worst, (r_gpu - r64) / np.linalg.norm(y))
+    print(f"worst excess residual over FP64 least squares: {worst:.2e} of |y|")
+    assert worst <= 1e-5
+    ks.close()
+
+
+@pytest.mark.parametrize("cfg", [CFG1, CFG2], ids=["128sq_8x8", "512sq_12x12"])
+def test_bf16_end_to_end_psnr(kl, orc, cfg):
+    out = {}
+    for prec in (kl.PREC_FP32, kl.PREC_BF16):
+        clean, noisy, ks = make(kl, cfg, precision=prec)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        s2 = kl.noise_variance(noisy)
+        out[prec] = ks.rebuild_picture(s2)
+        ks.close()
+    r32, rb = (kl.psnr(clean, out[p]) for p in (kl.PREC_FP32, kl.PREC_BF16))
+    msg = f"\nPSNR raw {cfg['W']}^2: plain FP32 {r32:.4f}, BF16 form {rb:.4f} (delta {rb - r32:+.4f} dB)"
+    if cfg is CFG1:
+        o64 = run_oracle(kl, orc, cfg, "f64")
+        msg += f"; FP64 oracle {kl.psnr(clean, o64):.4f}"
+    print(msg)
+    assert abs(rb - r32) <= 0.05
