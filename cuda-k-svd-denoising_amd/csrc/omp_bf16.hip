@@ -60,6 +60,14 @@ __device__ __forceinline__ float patch_sum(float v)
 }
 
 __device__ __forceinline__ unsigned umax(unsigned a, unsigned b) { return a > b ? a : b; }
+__device__ __forceinline__ unsigned umin(unsigned a, unsigned b) { return a < b ? a : b; }
+// median of three (v_med3_u32)
+__device__ __forceinline__ unsigned umed3(unsigned a, unsigned b, unsigned c)
+{
+    unsigned d;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
 
 // Score key of atom a: |c| with the 8 low significand bits replaced by
 // 255 - a, so an unsigned max picks the largest |c| and, among equal
@@ -228,6 +236,9 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, g = lane >> 4;
     const int e0 = g * Q;
+    // this lane's A-operand rows: atom tile t adds t * 16 * LDB
+    const __bf16 *bh = Ds + col * LDB + g * QM, *bl = Dlo + col * LDB + g * QM;
+    const __bf16 *bth = Ds + col * LDB + 4 * QM + 4 * g, *btl = Dlo + col * LDB + 4 * QM + 4 * g;
     // score error bound per unit |r|: 3 * 2^-16 for the hi/lo splits, 2^-23
     // per FP32 addition of the 3 * 4Q products, 0.1 % slack
     const float delta_u = __uint_as_float(dmax_bits[0]) * (3.f * 1.5259e-5f + 3.f * 4.f * Q * 1.1921e-7f) * 1.001f;
@@ -258,9 +269,8 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
         for (int t = 0; t < kmax; t++) {
             if (__ballot(active) == 0) break;
             // ---- correlation: three BF16 MFMAs per slice ---------------------
+            // (the first slice starts from a zero accumulator constant)
             f4 acc[KT];
-#pragma unroll
-            for (int tt = 0; tt < KT; tt++) acc[tt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kk = 0; kk < NKK; kk++) {
                 bf8 rh, rl;
@@ -275,13 +285,13 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
                     bf8 ah[4], al[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        const int off = ((t0 + u) * 16 + col) * LDB + g * QM + 8 * kk;
-                        ah[u] = *(const bf8 *)(Ds + off);
-                        al[u] = *(const bf8 *)(Dlo + off);
+                        ah[u] = *(const bf8 *)(bh + (t0 + u) * 16 * LDB + 8 * kk);
+                        al[u] = *(const bf8 *)(bl + (t0 + u) * 16 * LDB + 8 * kk);
                     }
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        acc[t0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], rh, acc[t0 + u], 0, 0, 0);
+                        const f4 c0 = kk == 0 ? f4{0.f, 0.f, 0.f, 0.f} : acc[t0 + u];
+                        acc[t0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], rh, c0, 0, 0, 0);
                         acc[t0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[u], rl, acc[t0 + u], 0, 0, 0);
                         acc[t0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[u], rh, acc[t0 + u], 0, 0, 0);
                     }
@@ -300,9 +310,8 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
                     bf4 ah[4], al[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        const int off = ((t0 + u) * 16 + col) * LDB + 4 * QM + 4 * g;
-                        ah[u] = *(const bf4 *)(Ds + off);
-                        al[u] = *(const bf4 *)(Dlo + off);
+                        ah[u] = *(const bf4 *)(bth + (t0 + u) * 16 * LDB);
+                        al[u] = *(const bf4 *)(btl + (t0 + u) * 16 * LDB);
                     }
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
@@ -314,15 +323,26 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
             }
             // ---- candidates: scores within 2 delta of the best ----------------
             // lane (col, g) holds atoms tt*16 + 4g + i; keys replace the scores
-            unsigned m1 = 0u;
+            // per-lane best two keys: m2 = med3(m1, m2, k) keeps the second
+            // largest (m2 <= m1 throughout), in four independent chains
+            unsigned c1[4], c2[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) c1[c] = c2[c] = 0u;
 #pragma unroll
             for (int tt = 0; tt < KT; tt++)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const unsigned k = score_key(acc[tt][i], tt * 16 + 4 * g + i);
                     acc[tt][i] = __uint_as_float(k);
-                    m1 = umax(m1, k);
+                    c2[i] = umed3(c1[i], c2[i], k);
+                    c1[i] = umax(c1[i], k);
                 }
+            unsigned m1 = c1[0], m2 = c2[0];
+#pragma unroll
+            for (int c = 1; c < 4; c++) {
+                m2 = umax(umin(m1, c1[c]), umax(m2, c2[c]));
+                m1 = umax(m1, c1[c]);
+            }
             unsigned M = umax(m1, (unsigned)__shfl_xor((int)m1, 16));
             M = umax(M, (unsigned)__shfl_xor((int)M, 32));
             const float Ms = __uint_as_float(M & 0x7FFFFF00u);
@@ -330,72 +350,35 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
             const float tau = Ms - 2.f * delta_u * rn - Ms * 6.11e-5f;
             const unsigned tk = tau > 0.f ? (__float_as_uint(tau) & 0x7FFFFF00u) : 0u;
             const bool finite = M < 0x7F800000u && rn == rn;
-            // lanes with two or more candidates (scalar bookkeeping)
-            unsigned long long seen = 0ull, multi = 0ull;
-#pragma unroll
-            for (int tt = 0; tt < KT; tt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const unsigned long long b = __ballot(active && __float_as_uint(acc[tt][i]) >= tk);
-                    multi |= seen & b;
-                    seen |= b;
-                }
+            // candidates: the two best keys of each of the patch's four lanes
+            // that reach tk (at most 8), in lane-group order
             unsigned ck[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) ck[q] = 0u;
             int nc = 0;
-            if (multi == 0ull) {
-                // at most one candidate per lane: the lanes' best keys, in
-                // lane-group order, those >= tk moved to the front
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const unsigned kq = (unsigned)__shfl((int)m1, col + 16 * q);
-                    if (kq >= tk) {
+            for (int q = 0; q < 8; q++) {
+                const unsigned kq = (unsigned)__shfl((int)(q < 4 ? m1 : m2), col + 16 * (q & 3));
+                ck[q] = 0u;
+                if (kq >= tk) {
 #pragma unroll
-                        for (int z = 0; z < 4; z++)
-                            if (z == nc) ck[z] = kq;
-                        nc++;
+                    for (int z = 0; z < 8; z++)
+                        if (z == nc) ck[z] = kq;
+                    nc++;
+                }
+            }
+            if (__ballot(active && m2 >= tk) != 0ull) {
+                // a lane with two candidates may hold a third: count those
+                // patches (the third is not re-scored)
+                unsigned m3 = 0u;
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const unsigned k = __float_as_uint(acc[tt][i]);
+                        m3 = umax(m3, k < m2 ? k : 0u);
                     }
-                }
-            } else {
-                // the best 8 keys of the patch within the margin, by extraction
-                unsigned prev = 0xFFFFFFFFu;
-#pragma unroll 1
-                for (int q = 0; q < 8; q++) {
-                    unsigned lb = 0u;
-#pragma unroll
-                    for (int tt = 0; tt < KT; tt++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const unsigned k = __float_as_uint(acc[tt][i]);
-                            lb = umax(lb, k < prev ? k : 0u);
-                        }
-                    unsigned pb = umax(lb, (unsigned)__shfl_xor((int)lb, 16));
-                    pb = umax(pb, (unsigned)__shfl_xor((int)pb, 32));
-                    const bool take = pb >= tk && pb > 0u && nc == q;
-                    if (take) {
-#pragma unroll
-                        for (int z = 0; z < 8; z++)
-                            if (z == q) ck[z] = pb;
-                        nc = q + 1;
-                        prev = pb;
-                    }
-                    if (__ballot(active && take) == 0ull) break;
-                }
-                if (__ballot(active && nc == 8) != 0ull) {
-                    // count patches that left a ninth candidate within the margin out
-                    unsigned lb = 0u;
-#pragma unroll
-                    for (int tt = 0; tt < KT; tt++)
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const unsigned k = __float_as_uint(acc[tt][i]);
-                            lb = umax(lb, k < prev ? k : 0u);
-                        }
-                    lb = umax(lb, (unsigned)__shfl_xor((int)lb, 16));
-                    lb = umax(lb, (unsigned)__shfl_xor((int)lb, 32));
-                    if (active && g == 0 && lb >= tk && lb > 0u) ncut++;
-                }
+                m3 = umax(m3, (unsigned)__shfl_xor((int)m3, 16));
+                m3 = umax(m3, (unsigned)__shfl_xor((int)m3, 32));
+                if (active && g == 0 && m3 >= tk && m3 > 0u) ncut++;
             }
             if (!finite) nc = 0;
             if (active && g == 0 && nc >= 2) nresc++;
