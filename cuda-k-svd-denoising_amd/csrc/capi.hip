@@ -12,7 +12,7 @@
 
 namespace {
 
-enum SpanKind { SPAN_GATHER, SPAN_INIT, SPAN_OMP, SPAN_OMP_KERNEL, SPAN_UPDATE, SPAN_REBUILD, SPAN_KINDS };
+enum SpanKind { SPAN_GATHER, SPAN_INIT, SPAN_OMP, SPAN_OMP_KERNEL, SPAN_UPDATE, SPAN_REBUILD, SPAN_TRAIN, SPAN_KINDS };
 
 struct Span {
     hipEvent_t a, b;
@@ -90,6 +90,7 @@ int resolve(ksvd_ctx *ctx)
         case SPAN_OMP_KERNEL: ctx->tm.omp_kernel_ms += ms; break;
         case SPAN_UPDATE: ctx->tm.update_ms += ms; ctx->tm.update_calls++; break;
         case SPAN_REBUILD: ctx->tm.rebuild_ms += ms; break;
+        case SPAN_TRAIN: ctx->tm.train_ms += ms; break;
         default: break;
         }
         tl.pool.push_back(s.a);
@@ -126,6 +127,9 @@ int check_params(ksvd_ctx *ctx, const ksvd_params *p)
         return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the BF16 form needs patch dim <= 144, atoms <= 256, ompIter <= 8");
     if (p->geometry != KSVD_GEOM_TRUE && p->geometry != KSVD_GEOM_COMPAT)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown geometry");
+    if (p->proposer != 0 && p->proposer != 1) return ksvd_fail(ctx, KSVD_ERR_ARG, "proposer must be 0 or 1");
+    if (p->proposer && p->precision != KSVD_PREC_BF16)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the learned proposer belongs to the BF16 form");
     return KSVD_OK;
 }
 
@@ -182,6 +186,7 @@ void ksvd_default_params(ksvd_params *p)
     p->precision = KSVD_PREC_FP32;
     p->geometry = KSVD_GEOM_TRUE;
     p->device = 0;
+    p->proposer = 0;
 }
 
 const char *ksvd_last_error(const ksvd_ctx *ctx)
@@ -257,7 +262,11 @@ int ksvd_destroy(ksvd_ctx *ctx)
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
-                      &ctx->rout, &ctx->rpart,  &ctx->rlam, &ctx->Dbf, &ctx->Dq, &ctx->bfscal};
+                      &ctx->rout, &ctx->rpart,  &ctx->rlam, &ctx->Dbf, &ctx->Dq, &ctx->bfscal, &ctx->rbound,
+                      &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
+                      &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
+                      &ctx->mlp_lpart, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
+                      &ctx->mlp_fb, &ctx->mlp_cnt};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -318,6 +327,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->rweight, sizeof(float) * (size_t)N},
         {&ctx->rout, sizeof(float) * (size_t)N},
         {&ctx->rpart, sizeof(double) * (size_t)((N + 255) / 256)},
+        {&ctx->rbound, sizeof(float) * (size_t)P},
     };
     drop_update_graph(ctx);  // buffers may move
     for (auto &e : plan) {
@@ -330,6 +340,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
     ctx->has_image = true;
     ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
+    ctx->mlp_x_valid = false;
     return KSVD_OK;
 }
 
@@ -355,6 +366,7 @@ int ksvd_extract_patches(ksvd_ctx *ctx)
     if ((rc = span_end(ctx, &s))) return rc;
     ctx->has_patches = true;
     ctx->has_codes = ctx->has_rebuild = false;
+    ctx->mlp_x_valid = false;
     return KSVD_OK;
 }
 
@@ -372,6 +384,9 @@ int ksvd_init_dictionary(ksvd_ctx *ctx)
     if ((rc = span_end(ctx, &s))) return rc;
     ctx->has_dict = true;
     ctx->gram_valid = false;
+    ctx->mlp_trained = false;  // a new dictionary lineage: its atoms are not the ones the proposer learned
+    ctx->mlp_gated = false;
+    ctx->iters_done = 0;
     return KSVD_OK;
 }
 
@@ -416,17 +431,67 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
     return check_sweep(ctx);
 }
 
+// Learned form schedule: the first iteration on a dictionary runs OMP on
+// every patch and trains the proposer on those pairs (PROP_STEPS0 steps);
+// later iterations propose, run OMP on the rejected patches, and train on
+// them (up to PROP_STEPS1 steps).  A proposal pass costs about as much as an
+// OMP pass, so when a pass accepts less than PROP_MIN_ACCEPT of the patches
+// the proposer stands down for the rest of this dictionary's iterations and
+// plain OMP runs instead.
+constexpr int PROP_EPOCHS0 = 2, PROP_MIN_STEPS0 = 24, PROP_MAX_STEPS0 = 96, PROP_STEPS1 = 4;
+constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
+constexpr uint64_t PROP_SEED = 0x5EEDull;
+
+static int proposer_step(ksvd_ctx *ctx)
+{
+    int rc;
+    if (ctx->mlp_gated) return ksvd_omp(ctx);
+    if (!ctx->mlp_trained) {
+        if ((rc = ksvd_omp(ctx))) return rc;
+        if (ctx->P < MLP_BATCH) return KSVD_OK;  // too few patches to train on: plain OMP
+        if (!ctx->mlp_ready && (rc = mlp_init(ctx, PROP_SEED))) return rc;
+        int steps = (int)std::min<int64_t>(PROP_MAX_STEPS0, std::max<int64_t>(PROP_MIN_STEPS0,
+                                                                              PROP_EPOCHS0 * ctx->P / MLP_BATCH));
+        Span s;
+        if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
+        if ((rc = mlp_train_all(ctx, steps, PROP_LR, PROP_SEED + ctx->iters_done))) return rc;
+        if ((rc = span_end(ctx, &s))) return rc;
+        ctx->mlp_trained = true;
+        return KSVD_OK;
+    }
+    if ((rc = ksvd_proposer_omp(ctx))) return rc;
+    unsigned nfb = 0;
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&nfb, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((double)(ctx->P - (int64_t)nfb) < PROP_MIN_ACCEPT * (double)ctx->P) {
+        ctx->mlp_gated = true;
+        return KSVD_OK;
+    }
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
+    if ((rc = mlp_train_fallback(ctx, PROP_STEPS1, PROP_LR))) return rc;
+    return span_end(ctx, &s);
+}
+
 int ksvd_iterate(ksvd_ctx *ctx, int iters)
 {
     if (!ctx) return KSVD_ERR_ARG;
     if (iters < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "iters must be >= 0");
     for (int i = 0; i < iters; i++) {
-        int rc = ksvd_omp(ctx);
+        int rc = ctx->prm.proposer ? proposer_step(ctx) : ksvd_omp(ctx);
         if (rc) return rc;
         rc = update_dictionary_async(ctx);
         if (rc) return rc;
+        ctx->iters_done++;
     }
     return iters > 0 ? check_sweep(ctx) : KSVD_OK;
+}
+
+int ksvd_run_final_code(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (ctx->prm.proposer && ctx->mlp_trained && !ctx->mlp_gated) return ksvd_proposer_omp(ctx);
+    return ksvd_omp(ctx);
 }
 
 int ksvd_run(ksvd_ctx *ctx)
@@ -434,7 +499,7 @@ int ksvd_run(ksvd_ctx *ctx)
     if (!ctx) return KSVD_ERR_ARG;
     int rc = ksvd_iterate(ctx, ctx->prm.ksvd_iter);
     if (rc) return rc;
-    return ksvd_omp(ctx);
+    return ksvd_run_final_code(ctx);
 }
 
 int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
@@ -513,6 +578,9 @@ int ksvd_set_dictionary(ksvd_ctx *ctx, const float *D)
     ctx->has_dict = true;
     ctx->gram_valid = false;
     ctx->has_rebuild = false;
+    ctx->mlp_trained = false;
+    ctx->mlp_gated = false;
+    ctx->iters_done = 0;
     return KSVD_OK;
 }
 
@@ -571,6 +639,106 @@ int ksvd_get_codes_dense(ksvd_ctx *ctx, float *X)
     return KSVD_OK;
 }
 
+// ---- learned support proposer ------------------------------------------------
+
+__global__ void proposer_count_kernel(const unsigned *n, int64_t P, unsigned long long *counters)
+{
+    counters[CNT_PROPOSED] += (unsigned long long)P;
+    counters[CNT_FALLBACK] += (unsigned long long)*n;
+}
+
+static int proposer_ok(ksvd_ctx *ctx)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
+    if (ctx->prm.precision != KSVD_PREC_BF16)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the learned proposer belongs to the BF16 form");
+    return set_device(ctx);
+}
+
+int ksvd_proposer_dims(ksvd_ctx *ctx, int *kin, int *ko, int64_t *nparams)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    int a, b;
+    mlp_dims(ctx, &a, &b);
+    if (kin) *kin = a;
+    if (ko) *ko = b;
+    if (nparams) *nparams = mlp_param_count(ctx);
+    return KSVD_OK;
+}
+
+int ksvd_proposer_init(ksvd_ctx *ctx, uint64_t seed)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    ctx->mlp_trained = false;
+    return mlp_init(ctx, seed);
+}
+
+int ksvd_proposer_get_params(ksvd_ctx *ctx, float *theta)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    if (!theta) return ksvd_fail(ctx, KSVD_ERR_ARG, "null parameter buffer");
+    return mlp_get_params(ctx, theta);
+}
+
+int ksvd_proposer_set_params(ksvd_ctx *ctx, const float *theta)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    if (!theta) return ksvd_fail(ctx, KSVD_ERR_ARG, "null parameter buffer");
+    return mlp_set_params(ctx, theta);
+}
+
+int ksvd_proposer_train(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    if (!ctx->has_codes || !ctx->has_patches) return ksvd_fail(ctx, KSVD_ERR_STATE, "codes required as targets");
+    if (steps < 0 || !(lr > 0.f)) return ksvd_fail(ctx, KSVD_ERR_ARG, "steps >= 0 and lr > 0 required");
+    if (ctx->P < MLP_BATCH) return ksvd_fail(ctx, KSVD_ERR_ARG, "fewer patches than one training batch (16384)");
+    if (!ctx->mlp_ready && (rc = mlp_init(ctx, PROP_SEED))) return rc;
+    Span s;
+    if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
+    if ((rc = mlp_train_all(ctx, steps, lr, seed))) return rc;
+    if ((rc = span_end(ctx, &s))) return rc;
+    ctx->mlp_trained = true;
+    return KSVD_OK;
+}
+
+int ksvd_proposer_omp(ksvd_ctx *ctx)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    if (!ctx->has_patches || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches and dictionary required");
+    if (!ctx->mlp_ready) return ksvd_fail(ctx, KSVD_ERR_STATE, "proposer not initialised");
+    if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "a first OMP (the residual bounds) is required");
+    Span outer;
+    if ((rc = span_begin(ctx, &outer, SPAN_OMP))) return rc;
+    if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
+    if ((rc = mlp_propose(ctx, PROP_ACCEPT))) return rc;
+    if ((rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr))) return rc;
+    hipLaunchKernelGGL(proposer_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (const unsigned *)ctx->mlp_cnt.ptr,
+                       ctx->P, (unsigned long long *)ctx->counters.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    if ((rc = span_end(ctx, &outer))) return rc;
+    ctx->has_codes = true;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
+int ksvd_proposer_loss_grad(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad)
+{
+    int rc = proposer_ok(ctx);
+    if (rc) return rc;
+    if (!X || !T || !loss || !grad) return ksvd_fail(ctx, KSVD_ERR_ARG, "null buffer");
+    if (!ctx->mlp_ready) return ksvd_fail(ctx, KSVD_ERR_STATE, "proposer not initialised");
+    if ((rc = resolve(ctx))) return rc;
+    return mlp_loss_grad_host(ctx, X, T, nb, loss, grad);
+}
+
 int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
 {
     if (!ctx || !t) return KSVD_ERR_ARG;
@@ -585,6 +753,14 @@ int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
     ctx->tm.omp_rescored = (int64_t)c[CNT_RESCORE];
     ctx->tm.omp_f64_refactors = (int64_t)c[CNT_F64];
     ctx->tm.omp_candidates_cut = (int64_t)c[CNT_CAND_CUT];
+    ctx->tm.proposer_gated = ctx->mlp_gated ? 1 : 0;
+    ctx->tm.proposed = (int64_t)c[CNT_PROPOSED];
+    ctx->tm.fallback = (int64_t)c[CNT_FALLBACK];
+    if (ctx->mlp_ready) {
+        float l = 0.f;
+        KSVD_HIP_CHECK(ctx, hipMemcpy(&l, (const float *)ctx->mlp_cnt.ptr + 1, sizeof(float), hipMemcpyDeviceToHost));
+        ctx->tm.train_loss = l;
+    }
     *t = ctx->tm;
     return KSVD_OK;
 }

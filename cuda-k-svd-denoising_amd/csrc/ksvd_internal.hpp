@@ -9,6 +9,16 @@
 
 #include "ksvd_hip.h"
 
+// Learned support proposer: hidden width, padded activation pitch (the
+// constant-1 column at MLP_H), training batch, split-K partials of the weight
+// gradients, proposal chunk (rows whose activations stay in the last-level
+// cache).
+constexpr int MLP_H = 512;
+constexpr int MLP_HP = 544;
+constexpr int MLP_BATCH = 16384;
+constexpr int MLP_SPLIT = 16;
+constexpr int MLP_CHUNK = 65536;
+
 // Largest supported ompIter of the device path.
 constexpr int KSVD_MAX_SUPPORT = 16;
 // Largest patch dimension (pw * ph) of the device path.
@@ -18,8 +28,9 @@ constexpr int KSVD_MAX_DIM = 1024;
 // Slot 3 counts sweep hand-over lists that overflowed (sweep.hip); slots 4-6
 // belong to the BF16 pursuit: patch-steps re-scored in FP32, FP64
 // re-factorisations, patch-steps whose candidate list was cut at 8.
+// Slots 7-8: patches offered to the learned proposer, and those it left to OMP.
 enum { CNT_DUP = 0, CNT_DEGEN = 1, CNT_EIG_IT = 2, CNT_HAND_OVF = 3, CNT_RESCORE = 4, CNT_F64 = 5,
-       CNT_CAND_CUT = 6, CNT_SLOTS = 8 };
+       CNT_CAND_CUT = 6, CNT_PROPOSED = 7, CNT_FALLBACK = 8, CNT_SLOTS = 16 };
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -52,6 +63,18 @@ struct ksvd_ctx {
     DevBuf srank, handx, E2, gpart, ticket; // sweep kernels (sweep.hip); ticket: flags of the persistent sweep
     DevBuf rsum, rweight, rout, rpart, rlam;
     DevBuf Dbf, Dq, bfscal;  // BF16 form: LDS image of D, padded FP32 atoms, largest atom norm
+    DevBuf rbound;           // per patch: residual norm of its last OMP (the proposer's bound)
+
+    // learned support proposer (mlp.hip)
+    DevBuf mlp_theta, mlp_m, mlp_v, mlp_wb, mlp_grad;        // weights (FP32 master, Adam moments, BF16), split-K grads
+    DevBuf mlp_xb, mlp_h1b, mlp_h2b, mlp_zb, mlp_dz, mlp_dh2, mlp_dh1, mlp_lpart;  // one training batch
+    DevBuf mlp_x, mlp_perm;                                  // normalised patches (all), training order
+    DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
+    bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
+    bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
+    int64_t mlp_chunk_rows = 0;
+    int64_t mlp_step = 0;
+    int64_t iters_done = 0;  // K-SVD iterations run on this image (learned form schedule)
 
     ksvd_timings tm{};
     std::string err;
@@ -79,6 +102,18 @@ int launch_init_dict(ksvd_ctx *ctx);
 int launch_dict_gram(ksvd_ctx *ctx);
 int launch_omp(ksvd_ctx *ctx);
 int launch_omp_bf16(ksvd_ctx *ctx);
+// BF16 OMP over the patches listed in `plist` (length *plist_n on the device).
+int launch_omp_bf16_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n);
+// learned support proposer (mlp.hip)
+int mlp_dims(const ksvd_ctx *ctx, int *kin, int *ko);
+int64_t mlp_param_count(const ksvd_ctx *ctx);
+int mlp_init(ksvd_ctx *ctx, uint64_t seed);
+int mlp_set_params(ksvd_ctx *ctx, const float *theta);
+int mlp_get_params(ksvd_ctx *ctx, float *theta);
+int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad);
+int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed);
+int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr);
+int mlp_propose(ksvd_ctx *ctx, float accept);
 size_t bf16_dict_bytes(int dim);
 size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);

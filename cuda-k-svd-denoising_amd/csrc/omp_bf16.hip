@@ -217,7 +217,8 @@ template <int Q, int KM>
 __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
     const float *__restrict__ Y, int dim, int64_t P, const __bf16 *__restrict__ Dbf, const float *__restrict__ Dq,
     const unsigned *__restrict__ dmax_bits, int K, const float *__restrict__ G, int kmax, int ntiles,
-    int *__restrict__ idx_out, float *__restrict__ coef_out, int *__restrict__ cnt_out,
+    const int *__restrict__ plist, const unsigned *__restrict__ plist_n, int *__restrict__ idx_out,
+    float *__restrict__ coef_out, int *__restrict__ cnt_out, float *__restrict__ rnorm_out,
     unsigned long long *__restrict__ counters)
 {
     using R = BfRow<Q>;
@@ -244,8 +245,12 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
     const float delta_u = __uint_as_float(dmax_bits[0]) * (3.f * 1.5259e-5f + 3.f * 4.f * Q * 1.1921e-7f) * 1.001f;
     unsigned long long ndup = 0, ndeg = 0, nresc = 0, nf64 = 0, ncut = 0;
 
-    for (int tile = blockIdx.x * BW + wave; tile < ntiles; tile += gridDim.x * BW) {
-        const int64_t p = (int64_t)tile * 16 + col;
+    // list mode: the tiles cover plist[0 .. *plist_n)
+    const int64_t nl = plist ? (int64_t)*plist_n : 0;
+    const int ntl = plist ? (int)((nl + 15) / 16) : ntiles;
+    for (int tile = blockIdx.x * BW + wave; tile < ntl; tile += gridDim.x * BW) {
+        const int64_t slot = (int64_t)tile * 16 + col;
+        const int64_t p = plist ? (slot < nl ? (int64_t)plist[slot] : P) : slot;
         const bool valid = p < P;
         float y[Q], r[Q];
         load_quarter<Q>(y, Y, p, dim, e0, valid);
@@ -536,6 +541,7 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
             if (active && rn < 1e-3f) active = false;
         }
         if (valid && g == 0) {
+            if (rnorm_out) rnorm_out[p] = rn;
             cnt_out[p] = s;
 #pragma unroll
             for (int i = 0; i < KM; i++)
@@ -553,7 +559,7 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
 }
 
 template <int Q>
-int launch_bf16_q(ksvd_ctx *ctx)
+int launch_bf16_q(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n)
 {
     using R = BfRow<Q>;
     unsigned *dmax = (unsigned *)ctx->bfscal.ptr;
@@ -569,9 +575,9 @@ int launch_bf16_q(ksvd_ctx *ctx)
                                                 (int)lds));
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * BW), lds, ctx->stream, (const float *)ctx->Y.ptr, ctx->dim,
                            ctx->P, (const __bf16 *)ctx->Dbf.ptr, (const float *)ctx->Dq.ptr,
-                           (const unsigned *)dmax, ctx->K, (const float *)ctx->G.ptr, ctx->kmax, ntiles,
-                           (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
-                           (unsigned long long *)ctx->counters.ptr);
+                           (const unsigned *)dmax, ctx->K, (const float *)ctx->G.ptr, ctx->kmax, ntiles, plist,
+                           plist_n, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                           (float *)ctx->rbound.ptr, (unsigned long long *)ctx->counters.ptr);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
     };
@@ -598,5 +604,12 @@ int launch_omp_bf16(ksvd_ctx *ctx)
 {
     if (ctx->dim > 144 || ctx->K > KPAD || ctx->kmax > 8)
         return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the BF16 form needs patch dim <= 144, atoms <= 256, ompIter <= 8");
-    return ctx->dim <= 64 ? launch_bf16_q<16>(ctx) : launch_bf16_q<36>(ctx);
+    return ctx->dim <= 64 ? launch_bf16_q<16>(ctx, nullptr, nullptr) : launch_bf16_q<36>(ctx, nullptr, nullptr);
+}
+
+int launch_omp_bf16_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n)
+{
+    if (ctx->dim > 144 || ctx->K > KPAD || ctx->kmax > 8)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the BF16 form needs patch dim <= 144, atoms <= 256, ompIter <= 8");
+    return ctx->dim <= 64 ? launch_bf16_q<16>(ctx, plist, plist_n) : launch_bf16_q<36>(ctx, plist, plist_n);
 }

@@ -45,6 +45,7 @@ bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVa
     p.geometry = geometry;
     p.device = device;
     p.precision = (type == HIP_K_SVD_BF16 || type == HIP_K_SVD_BF16_LEARNED) ? KSVD_PREC_BF16 : KSVD_PREC_FP32;
+    p.proposer = type == HIP_K_SVD_BF16_LEARNED ? 1 : 0;
     ksvd_ctx *ctx = nullptr;
     int rc = ksvd_create(&p, &ctx);
     auto fail = [&](int code) {
@@ -67,7 +68,8 @@ bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVa
     nPatches = P;
     const double iter_ms = t.omp_ms + t.update_ms;
     patchIters = iter_ms > 0 ? (double)P * iter / (iter_ms * 1e-3) : 0;
-    if ((rc = ksvd_omp(ctx))) return fail(rc);
+    // the final sparse code: OMP, or the trained proposer with OMP fallback
+    if ((rc = ksvd_run_final_code(ctx))) return fail(rc);
     out.assign((size_t)m * n, 0.f);
     if ((rc = ksvd_rebuild(ctx, noiseVar, out.data()))) return fail(rc);
     ksvd_destroy(ctx);

@@ -39,6 +39,7 @@ class Params(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("geometry", ctypes.c_int),
         ("device", ctypes.c_int),
+        ("proposer", ctypes.c_int),
     ]
 
 
@@ -58,6 +59,11 @@ class Timings(ctypes.Structure):
         ("omp_rescored", ctypes.c_int64),
         ("omp_f64_refactors", ctypes.c_int64),
         ("omp_candidates_cut", ctypes.c_int64),
+        ("train_ms", ctypes.c_double),
+        ("proposed", ctypes.c_int64),
+        ("fallback", ctypes.c_int64),
+        ("train_loss", ctypes.c_double),
+        ("proposer_gated", ctypes.c_int64),
     ]
 
 
@@ -78,6 +84,7 @@ SIGNATURES = {
     "ksvd_update_dictionary": (ctypes.c_int, [_P]),
     "ksvd_iterate": (ctypes.c_int, [_P, ctypes.c_int]),
     "ksvd_run": (ctypes.c_int, [_P]),
+    "ksvd_run_final_code": (ctypes.c_int, [_P]),
     "ksvd_rebuild": (ctypes.c_int, [_P, ctypes.c_float, _FP]),
     "ksvd_get_rebuild_parts": (ctypes.c_int, [_P, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double)]),
     "ksvd_rebuild_device_ptr": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
@@ -92,6 +99,14 @@ SIGNATURES = {
     "ksvd_get_timings": (ctypes.c_int, [_P, ctypes.POINTER(Timings)]),
     "ksvd_reset_timings": (ctypes.c_int, [_P]),
     "ksvd_synchronize": (ctypes.c_int, [_P]),
+    "ksvd_proposer_dims": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int64)]),
+    "ksvd_proposer_init": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "ksvd_proposer_get_params": (ctypes.c_int, [_P, _FP]),
+    "ksvd_proposer_set_params": (ctypes.c_int, [_P, _FP]),
+    "ksvd_proposer_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_uint64]),
+    "ksvd_proposer_omp": (ctypes.c_int, [_P]),
+    "ksvd_proposer_loss_grad": (ctypes.c_int, [_P, _FP, _FP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _FP]),
     "ksvd_synth_image": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, _FP, _FP]),
     "ksvd_noise_variance": (ctypes.c_double, [_FP, ctypes.c_int, ctypes.c_int]),
     "ksvd_normalize": (None, [_FP, ctypes.c_int64, ctypes.c_float, ctypes.c_float]),
@@ -143,13 +158,14 @@ class KSvd:
     """One context of the HIP path: image -> patches -> K-SVD -> rebuild."""
 
     def __init__(self, patch=(8, 8), slide=(2, 2), atoms=256, omp_iter=5, ksvd_iter=10, user_cap=1024,
-                 geometry=GEOM_TRUE, precision=PREC_FP32, device=0):
+                 geometry=GEOM_TRUE, precision=PREC_FP32, device=0, proposer=0):
         L = lib()
         p = default_params()
         p.patch_w, p.patch_h = patch
         p.slide_w, p.slide_h = slide
         p.atoms, p.omp_iter, p.ksvd_iter = atoms, omp_iter, ksvd_iter
         p.user_cap, p.geometry, p.precision, p.device = user_cap, geometry, precision, device
+        p.proposer = proposer
         self.params = p
         h = _P()
         rc = L.ksvd_create(ctypes.byref(p), ctypes.byref(h))
@@ -290,6 +306,47 @@ class KSvd:
         t = Timings()
         self._check(lib().ksvd_get_timings(self._h, ctypes.byref(t)))
         return {f: getattr(t, f) for f, _ in Timings._fields_}
+
+    # -- learned support proposer (BF16 form) ------------------------------
+    def proposer_dims(self):
+        kin, ko, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        self._check(lib().ksvd_proposer_dims(self._h, ctypes.byref(kin), ctypes.byref(ko), ctypes.byref(n)))
+        return kin.value, ko.value, n.value
+
+    def proposer_init(self, seed):
+        self._check(lib().ksvd_proposer_init(self._h, ctypes.c_uint64(seed)))
+
+    def proposer_params(self):
+        """(W1 [512 x KIN], W2 [512 x 544], W3 [KO x 544]) FP32 master weights."""
+        kin, ko, n = self.proposer_dims()
+        th = np.empty(n, np.float32)
+        self._check(lib().ksvd_proposer_get_params(self._h, _fp(th)))
+        a, b = 512 * kin, 512 * 544
+        return th[:a].reshape(512, kin), th[a:a + b].reshape(512, 544), th[a + b:].reshape(ko, 544)
+
+    def set_proposer_params(self, W1, W2, W3):
+        th = np.ascontiguousarray(np.concatenate([np.ravel(W1), np.ravel(W2), np.ravel(W3)]), dtype=np.float32)
+        assert th.size == self.proposer_dims()[2]
+        self._check(lib().ksvd_proposer_set_params(self._h, _fp(th)))
+
+    def proposer_train(self, steps, lr=2e-3, seed=1):
+        self._check(lib().ksvd_proposer_train(self._h, steps, ctypes.c_float(lr), ctypes.c_uint64(seed)))
+
+    def proposer_omp(self):
+        self._check(lib().ksvd_proposer_omp(self._h))
+
+    def proposer_loss_grad(self, X, T):
+        """Mean loss and gradients (W1, W2, W3 shaped) of one batch on the GPU."""
+        kin, ko, n = self.proposer_dims()
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        nb = X.shape[0]
+        assert X.shape == (nb, kin) and T.shape == (nb, ko)
+        g = np.empty(n, np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().ksvd_proposer_loss_grad(self._h, _fp(X), _fp(T), nb, ctypes.byref(loss), _fp(g)))
+        a, b = 512 * kin, 512 * 544
+        return loss.value, (g[:a].reshape(512, kin), g[a:a + b].reshape(512, 544), g[a + b:].reshape(ko, 544))
 
     def reset_timings(self):
         self._check(lib().ksvd_reset_timings(self._h))

@@ -67,6 +67,8 @@ typedef struct ksvd_params {
     int precision;          /* KSVD_PREC_* */
     int geometry;           /* KSVD_GEOM_* */
     int device;             /* HIP device ordinal */
+    int proposer;           /* 1: learned support proposer with OMP fallback
+                               (BF16 form only); 0: plain OMP */
 } ksvd_params;
 
 /* Per-phase device time (HIP events on the context's stream) and counters. */
@@ -84,7 +86,13 @@ typedef struct ksvd_timings {
     int64_t eig_iters;      /* power iterations spent in rank-1 updates */
     int64_t omp_rescored;   /* BF16 form: patch-steps whose pick was re-scored in FP32 */
     int64_t omp_f64_refactors; /* BF16 form: support Grams re-factorised in FP64 */
-    int64_t omp_candidates_cut; /* BF16 form: patch-steps with more than 8 candidates */
+    int64_t omp_candidates_cut; /* BF16 form: patch-steps with a third candidate in one lane group */
+    double train_ms;        /* learned proposer: training (forward, backward, AdamW) */
+    int64_t proposed;       /* learned proposer: patches it proposed a support for */
+    int64_t fallback;       /* learned proposer: of those, patches left to OMP */
+    double train_loss;      /* learned proposer: mean loss of the last training batch */
+    int64_t proposer_gated; /* learned proposer: 1 once it stood down (a pass accepted
+                               under half the patches), plain OMP since */
 } ksvd_timings;
 
 typedef struct ksvd_ctx ksvd_ctx;
@@ -139,6 +147,11 @@ int ksvd_iterate(ksvd_ctx *ctx, int iters);
  * ksvd_iter iterations, then the final OMP. */
 int ksvd_run(ksvd_ctx *ctx);
 
+/* The final sparse code of kSvd (cuda_svd_denoiser.cu:400-401): ksvd_omp,
+ * or, in the BF16 + learned form once the proposer is trained,
+ * ksvd_proposer_omp. */
+int ksvd_run_final_code(ksvd_ctx *ctx);
+
 /* Overlap-add of D X, lambda from sigma2 (the noise variance that
  * Denoiser::loadImage estimates, denoiser.cpp:56), blend, and copy of the
  * m*n denoised buffer to host memory `out` (NULL: keep it on the device).
@@ -172,6 +185,39 @@ int ksvd_get_codes_dense(ksvd_ctx *ctx, float *X);
 int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t);
 int ksvd_reset_timings(ksvd_ctx *ctx);
 int ksvd_synchronize(ksvd_ctx *ctx);
+
+/* ---- learned support proposer (BF16 form, proposer = 1) ----------------
+ * No counterpart in the reference: these drive the BF16 + learned form.  A
+ * perceptron patch -> 512 -> 512 -> atoms (ReLU, BF16 weights, FP32
+ * accumulation) proposes each patch's whole support in one shot; one FP32
+ * least-squares solve gives its coefficients, and plain (BF16-form) OMP
+ * takes over whenever the proposed residual exceeds 1.05 x the patch's last
+ * OMP residual.  It learns from (patch, OMP support) pairs: ksvd_iterate
+ * trains it on every patch after the first OMP of a dictionary and on each
+ * fallback list afterwards.
+ *
+ * Input of a patch y with mean m: (y - m 1) / |y - m 1| (dim values), the DC
+ * fraction m sqrt(dim) / |y|, and 1.  Parameter vector theta: W1 [512 x KIN]
+ * | W2 [512 x 544] | W3 [KO x 544], row-major FP32; KIN = dim + 2 rounded up
+ * to 32 (column dim + 1 is the bias), KO = atoms rounded up to 32, column
+ * 512 of W2 and W3 is the bias. */
+
+/* Shapes: input pitch KIN, output rows KO, parameter count. */
+int ksvd_proposer_dims(ksvd_ctx *ctx, int *kin, int *ko, int64_t *nparams);
+/* Seeded initialisation U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0. */
+int ksvd_proposer_init(ksvd_ctx *ctx, uint64_t seed);
+int ksvd_proposer_get_params(ksvd_ctx *ctx, float *theta);
+int ksvd_proposer_set_params(ksvd_ctx *ctx, const float *theta);
+/* `steps` AdamW steps (batch 16384, learning rate lr) on the current codes of
+ * all patches as targets, in a seeded random order. */
+int ksvd_proposer_train(ksvd_ctx *ctx, int steps, float lr, uint64_t seed);
+/* One proposal pass over all patches followed by OMP on the rejected ones:
+ * a replacement for ksvd_omp once the proposer is trained. */
+int ksvd_proposer_omp(ksvd_ctx *ctx);
+/* Mean loss and summed gradient (layout of theta) of one batch given on the
+ * host: X [nb x KIN] inputs (rounded to BF16), T [nb x KO] 0/1 targets;
+ * nb a multiple of 32 up to 16384.  For gradient checks. */
+int ksvd_proposer_loss_grad(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad);
 
 /* ---- host-side helpers (no GPU) ---------------------------------------- */
 
