@@ -305,10 +305,14 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
 // Sums of a and of b over the T2 threads of a workgroup (fixed order),
 // returned to every thread; red2 holds 2 x (T2 / 64) float2 and `phase`
 // alternates between calls so that one barrier suffices.
-__device__ __forceinline__ float2 block_sum2(float a, float b, float2 *red2, int &phase)
+__device__ __forceinline__ float2 block_sum2(float a, float b, float2 *red2, int &phase, bool act = true)
 {
-    a = wave_sum_dpp(a);
-    b = wave_sum_dpp(b);
+    if (act) {
+        a = wave_sum_dpp(a);
+        b = wave_sum_dpp(b);
+    } else {
+        a = b = 0.f;
+    }
     float2 *r = red2 + phase * (T2 / 64);
     phase ^= 1;
     if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = make_float2(a, b);
@@ -326,51 +330,72 @@ __device__ __forceinline__ float2 block_sum2(float a, float b, float2 *red2, int
 // Power iteration u <- G u / |G u| from u (LDS) on the symmetric Gs (LDS,
 // pitch ldg), T2 threads, 4 per row, each holding its quarter of a row of G
 // in registers (quarters of ql = 4 * ceil(dim / 16) entries, ql <= QL; u must
-// hold 4 * QL floats, zero past dim).  Stops at the first iterate u_i with
-// i >= 2 and |u_i - u_{i-1}|^2 < tol2, or at i = maxit; a zero product keeps
-// u.  The test for u_i shares one block reduction with |G u_i|^2, so each step
-// costs one reduction.  Returns i.
+// hold 4 * QL floats, zero past dim; zb is scratch for 2 x 4 * QL floats).
+// Stops at the first iterate u_i with i >= 2 and |u_i - u_{i-1}|^2 < tol2,
+// or at i = maxit; a zero product keeps u_i.  One barrier per pass: pass i
+// reads z_i = G u_{i-1} unnormalised and its norm n_i (from the previous
+// pass's reduction), forms u_i = z_i / n_i on the fly, writes z_{i+1} =
+// G u_i to the other buffer, and one block reduction gives |z_{i+1}|^2 and
+// |u_i - u_{i-1}|^2 together.  Returns i; u holds u_i.
 template <int QL>
-__device__ int power_iteration_ps(const float *Gs, int ldg, float *u, int dim, float2 *red2, float tol2, int maxit)
+__device__ int power_iteration_ps(const float *Gs, int ldg, float *u, float *zb, int dim, float2 *red2, float tol2,
+                                  int maxit)
 {
     const int tid = threadIdx.x;
     const int row = tid >> 2, part = tid & 3;
     const int ql = (dim + 15) / 16 * 4;
     const int q0 = part * ql;
+    // only the wavefronts that hold rows compute; the others take part in
+    // the barriers with zero partial sums
+    const bool act = (tid >> 6) < (4 * dim + 63) / 64;
     float g[QL];
 #pragma unroll
     for (int i = 0; i < QL; i++) {
         const int q = q0 + i;
-        g[i] = (row < dim && i < ql && q < dim) ? Gs[q * ldg + row] : 0.f;
+        g[i] = (act && row < dim && i < ql && q < dim) ? Gs[q * ldg + row] : 0.f;
     }
     const bool owner = row < dim && part == 0;
-    float ucur = owner ? u[row] : 0.f, uprev = 0.f;
+    // the iterate buffers past dim stay 0 (g is 0 there, but 0 * NaN is not);
+    // the first pass's barrier orders these stores before any read
+    for (int e = dim + tid; e < 4 * QL; e += blockDim.x) {
+        zb[e] = 0.f;
+        zb[4 * QL + e] = 0.f;
+    }
+    const float *src = u;  // u_i = src * scale
+    float scale = 1.f, uprev = 0.f, ui = 0.f;
     int phase = 0;
     int it = 0;
     for (;;) {
-        // terms past the thread's range have g = 0 and add exactly nothing
+        // terms past the thread's range have g = 0 and add exactly nothing;
+        // four partial sums shorten the dependent chain
         float z = 0.f;
+        if (act) {
+            float z4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c = 0; c < QL / 4; c++) {
-            const f4 uu = *(const f4 *)(u + q0 + 4 * c);
-            z = fmaf(g[4 * c], uu.x, z);
-            z = fmaf(g[4 * c + 1], uu.y, z);
-            z = fmaf(g[4 * c + 2], uu.z, z);
-            z = fmaf(g[4 * c + 3], uu.w, z);
+            for (int c = 0; c < QL / 4; c++) {
+                const f4 uu = *(const f4 *)(src + q0 + 4 * c);
+                z4[0] = fmaf(g[4 * c], uu.x, z4[0]);
+                z4[1] = fmaf(g[4 * c + 1], uu.y, z4[1]);
+                z4[2] = fmaf(g[4 * c + 2], uu.z, z4[2]);
+                z4[3] = fmaf(g[4 * c + 3], uu.w, z4[3]);
+            }
+            z = quad_sum((z4[0] + z4[1]) + (z4[2] + z4[3])) * scale;
         }
-        z = quad_sum(z);
-        const float du = ucur - uprev;
-        const float2 sz = block_sum2(owner ? z * z : 0.f, owner ? du * du : 0.f, red2, phase);
+        ui = owner ? src[row] * scale : 0.f;
+        const float du = ui - uprev;
+        float *dst = zb + ((it + 1) & 1) * 4 * QL;
+        if (owner) dst[row] = z;
+        const float2 sz = block_sum2(owner ? z * z : 0.f, owner ? du * du : 0.f, red2, phase, act);
         if ((it >= 2 && sz.y < tol2) || it >= maxit) break;
         const float nz = sqrtf(sz.x);
-        if (nz == 0.f) break;  // E = 0: keep d_k (its coefficients become 0)
-        const float un = owner ? z / nz : 0.f;
-        uprev = ucur;
-        ucur = un;
-        if (owner) u[row] = un;
-        __syncthreads();
+        if (nz == 0.f) break;  // E = 0: keep u_i (d_k at the start; its coefficients become 0)
+        uprev = ui;
+        src = dst;
+        scale = 1.f / nz;
         it++;
     }
+    if (owner) u[row] = ui;
+    __syncthreads();
     return it;
 }
 
@@ -522,8 +547,10 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 //   P2 for each column whose patch is among atom k-1's first m_{k-1} users
 //      (rank r), x = e_{k-1,r} . d_{k-1} (column r from global memory) and
 //      e -= d_{k-1} x;
-//   P3 its E_k columns -> global, partial Gram of its columns on MFMA ->
-//      Gpart[h]; done[h] = k + 1;
+//   P3 partial Gram of its columns on MFMA -> Gpart[h]; done[h] = k + 1;
+//      then its E_k columns -> global (atom k+1's hand-over reads them after
+//      aflag = k + 1, which follows every rdone[*] = k + 1 and the store
+//      wait that precedes each rdone);
 //   RS once every done[*] = k + 1: sums the partials of the Gram tiles
 //      h, h + H, ... in helper order -> Gred; rdone[h] = k + 1;
 //   P4 atom k-1's final coefficients for its own columns of atom k-1 (its
@@ -671,6 +698,7 @@ __device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, const floa
     const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
     float *Gs = sm;               // dpad x dpad (rows and columns past dim are 0)
     float *u = Gs + dpad * dpad;  // 4 * QL, zero past dim
+    float *zb = u + 4 * QL;       // 2 x 4 * QL: the power iteration's unnormalised iterates
     const int nf4 = a.ntiles * 64;
     const unsigned rbytes = (unsigned)(nf4 * 16);
     unsigned long long iters = 0;
@@ -702,7 +730,7 @@ __device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, const floa
             }
             for (int e = tid; e < 4 * QL; e += PS_T) u[e] = e < dim ? ld_c(a.D + (int64_t)k * dim + e) : 0.f;
             __syncthreads();
-            iters += (unsigned long long)power_iteration_ps<QL>(Gs, dpad, u, dim, red2, a.tol2, a.maxit);
+            iters += (unsigned long long)power_iteration_ps<QL>(Gs, dpad, u, zb, dim, red2, a.tol2, a.maxit);
             for (int e = tid; e < dim; e += PS_T) a.D[(int64_t)k * dim + e] = u[e];
         }
         ps_publish(flags + PSF_AFLAG, (unsigned)(k + 1));
@@ -824,33 +852,48 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
             else if (nh > 0) ps_handover<1>(Es, dpad, dim, hl, hrank, nh, Eprev, uv);
         }
         __syncthreads();
-        // ---- P3: E_k columns to global, partial Gram ---------------------------
+        // ---- P3: partial Gram of this helper's columns --------------------------
         if (k < K) {
             if (nc > 0) {
-                // stores first: they drain during the MFMAs
-                float *Ecur = (k & 1) ? a.E1 : a.E0;
-                for (int q = tid; q < nc * dim; q += PS_T) {
-                    const int tl = q / dim, e = q - tl * dim;
-                    Ecur[(int64_t)(tb + tl) * dpad + e] = Es[tl * dpad + e];
-                }
                 f4 acc[TPW];
 #pragma unroll
                 for (int q = 0; q < TPW; q++) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
                 const int col16 = lane & 15, grp = lane >> 4;
                 const int nk = (nc + 3) >> 2;
+                // the wavefront's tiles advance together: UK k-steps of
+                // operands are read first, then the MFMA chains of the tiles
+                // interleave (independent accumulators)
+                int oa[TPW], ob[TPW];
+                bool tok[TPW];
 #pragma unroll
                 for (int q = 0; q < TPW; q++) {
                     const int ti = wave + PS_W * q;
-                    if (ti < a.ntiles) {
-                        int ba, bb;
-                        tile_coords(ti, a.nbt, &ba, &bb);
-                        for (int kk = 0; kk < nk; kk++) {
-                            const int t = 4 * kk + grp;
-                            const float av = Es[t * dpad + ba * 16 + col16];
-                            const float bv = Es[t * dpad + bb * 16 + col16];
-                            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
+                    tok[q] = ti < a.ntiles;
+                    int ba = 0, bb = 0;
+                    if (tok[q]) tile_coords(ti, a.nbt, &ba, &bb);
+                    oa[q] = ba * 16 + col16;
+                    ob[q] = bb * 16 + col16;
+                }
+                const float *Eg = Es + grp * dpad;
+                constexpr int UK = TPW <= 3 ? 4 : 2;  // k-steps per batch of reads (register budget)
+                for (int kk = 0; kk < nk; kk += UK) {
+                    float av[UK][TPW], bv[UK][TPW];
+#pragma unroll
+                    for (int u = 0; u < UK; u++) {
+                        // columns past nc4 hold stale values: kept out
+                        const bool in = kk + u < nk;
+                        const float *row = Eg + 4 * (kk + u) * dpad;
+#pragma unroll
+                        for (int q = 0; q < TPW; q++) {
+                            av[u][q] = in ? row[oa[q]] : 0.f;
+                            bv[u][q] = in ? row[ob[q]] : 0.f;
                         }
                     }
+#pragma unroll
+                    for (int u = 0; u < UK; u++)
+#pragma unroll
+                        for (int q = 0; q < TPW; q++)
+                            if (tok[q]) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u][q], bv[u][q], acc[q], 0, 0, 0);
                 }
                 float *gp = a.Gpart + (size_t)h * nf4 * 4;
 #pragma unroll
@@ -860,6 +903,16 @@ __device__ void ps_helper(const SweepArgs &a, unsigned *flags, int H, int h, flo
                 }
             }
             ps_publish(flags + PSF_DONE + h, (unsigned)(k + 1));
+            // E_k's columns to global memory for atom k+1's hand-over: issued
+            // after the flag (they are read only after atom k+1's vector is
+            // out, which follows this helper's next publish and its wait)
+            if (nc > 0) {
+                float *Ecur = (k & 1) ? a.E1 : a.E0;
+                for (int q = tid; q < nc * dim; q += PS_T) {
+                    const int tl = q / dim, e = q - tl * dim;
+                    Ecur[(int64_t)(tb + tl) * dpad + e] = Es[tl * dpad + e];
+                }
+            }
             // ---- RS: this helper's tiles of the Gram, summed over the partials
             if (m > 0) {
                 if (!ps_wait(flags, PSF_DONE, H, (unsigned)(k + 1), bail)) return;
@@ -979,14 +1032,14 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
 
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
-    int H = 16;
+    int H = 24;
     if (const char *hv = getenv("KSVD_SWEEP_HELPERS")) H = std::max(1, std::min(PS_HMAX, atoi(hv)));
     if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
     const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (persistent) {
         const size_t lds_helper = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (size_t)PS_CWMAX * PS_NOMAX) +
                                   sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 3 * PS_CWMAX + 1);
-        const size_t lds_master = sizeof(float) * ((size_t)dpad * dpad + 4 * 48);
+        const size_t lds_master = sizeof(float) * ((size_t)dpad * dpad + 3 * 4 * 48);
         const size_t lds = std::max(lds_helper, lds_master);
         typedef void (*PsFn)(SweepArgs, unsigned *, int, float *);
         PsFn ps = nullptr;
