@@ -1,22 +1,31 @@
 """K-SVD denoising benchmark on one MI355X.
 
-A step is one K-SVD iteration (batched OMP over every patch + one
-Gauss-Seidel sweep of rank-1 atom updates) over the synthetic image.  The
-workload is the plain FP32 form on the 512 x 512 grayscale synthetic image,
-sigma 25, 12 x 12 patches sliding 3 (27,889 patches of dim 144), 256 atoms,
-OMP limit 5.  Inputs are resident in HBM before the timed region; the timed
-region is exactly --steps iterations from the reference's dictionary init.
+A step is one K-SVD iteration (sparse coding of every patch + one
+Gauss-Seidel sweep of rank-1 atom updates) over one synthetic image.  The
+default workload is the 2048 x 2048 grayscale synthetic image (seeded
+gratings + rectangles, Gaussian sigma 20), 12 x 12 patches sliding 3
+(461,041 patches of dim 144), 256 atoms, OMP limit 5, 10 iterations, run in
+three forms on the same input:
 
-Prints one JSON line.  `roofline` is for the kernel pair that dominates the
-step: one atom of the Gauss-Seidel dictionary sweep (k1_kernel builds the
-restricted residual and partial Grams, k2_kernel reduces them and takes the
-top eigenvector).  Its algorithmic bytes are those of the reference's update
-for the atom's m users (re-read each user's patch and code, write its new
-coefficient); its duration is the library's own HIP-event span of the
-dictionary update on the library's stream, divided by the atoms swept.
-`roofline_omp` is the same record for the OMP pursuit kernel (correlation on
-MFMA FP32).  `cpu_baseline` times the CPU oracle (FP32, OpenMP) on one K-SVD
-iteration of the same workload on this host.
+  * plain FP32 (the drop-in for the reference's CUDA_K_GESVDJ path);
+  * BF16 (correlation on BF16 MFMA, FP32 re-score, FP32 least squares);
+  * BF16 + learned (the BF16 form whose sparse coding is proposed by the
+    trained perceptron, OMP as fallback; training is inside the timed
+    iterations).
+
+`value` is the BF16 + learned form's patch-iterations per second (patches x
+iterations / wall time of the iterations, inputs resident in HBM);
+`plain_fp32` and `bf16` give the other forms' rates, `speedup_vs_plain` the
+ratio, `delta_psnr_db` the denoised PSNR difference to the plain form.
+
+`roofline` is for the kernel that dominates the step: the persistent
+dictionary sweep (ps_sweep_kernel, one launch per iteration).  Its
+algorithmic bytes are those of the reference's update for the capped users
+of every atom (re-read each user's patch and code, write its new
+coefficient); its duration is the library's HIP-event span of the update on
+its stream.  `roofline_omp` is the same record for the plain form's OMP
+kernel (FP32 MFMA).  `cpu_baseline` times the CPU oracle (FP32, OpenMP) on
+one K-SVD iteration of the same workload on this host.
 """
 import argparse
 import json
@@ -31,24 +40,25 @@ PKG = os.path.join(ROOT, "cuda-k-svd-denoising_amd")
 ORACLE = os.path.join(ROOT, "oracle")
 sys.path.insert(0, PKG)
 
-# The three square workloads (seeded synthetic images; the 512^2 one is the
-# default benchmark line, the others are selectable with --config).
+# The three square workloads (seeded synthetic images); 2048 is the
+# benchmark line (the configuration the headline metric is quoted on), the
+# others are selectable with --config.
 WORKLOADS = {
     "128": dict(W=128, H=128, sigma=20.0, patch=(8, 8), slide=(2, 2), atoms=256, omp_iter=5, seed=1, iters=5),
     "512": dict(W=512, H=512, sigma=25.0, patch=(12, 12), slide=(3, 3), atoms=256, omp_iter=5, seed=2, iters=10),
     "2048": dict(W=2048, H=2048, sigma=20.0, patch=(12, 12), slide=(3, 3), atoms=256, omp_iter=5, seed=3, iters=10),
 }
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-# the 512 x 512 workload (profiles/hbm_traffic_ksvd_512sq.md), scaled by what
-# the same counters read for a device copy of known size (FETCH x 2, WRITE x 1).
+# HBM bytes per dispatch of the dominant kernels, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE (KiB) and scaled by what the same counters read for
+# a device copy of known size (FETCH x 2, WRITE x 1):
+# profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "512": {"omp": (9367.4 * 2 + 1198.5) * 1024,
-            "sweep_atom": (485.4 * 2 + 578.9 + 196.1 * 2 + 0.9) * 1024},
+    "2048": {"sweep": (130120.8 * 2 + 416736.1) * 1024, "omp": (132511.1 * 2 + 19816.8) * 1024},
 }
-
 
 def omp_flops_per_patch(n, K, k):
     """Algorithmic flops of one patch's pursuit (residual-correlation OMP):
@@ -97,14 +107,65 @@ def calibrate_copy(mib):
     print(json.dumps({"calibration": "d2d copy", "bytes_per_copy": nbytes, "copies": 3}))
 
 
+def run_form(K, wl, frame, steps, warmup, **kw):
+    """Time `steps` K-SVD iterations of one form on a fresh context (after
+    `warmup` untimed iterations on another context); returns the rate, the
+    library's timings and the context's dims."""
+    def make():
+        ks = K.KSvd(patch=wl["patch"], slide=wl["slide"], atoms=wl["atoms"], omp_iter=wl["omp_iter"],
+                    ksvd_iter=steps, **kw)
+        ks.set_frame(frame, wl["W"], wl["H"])
+        ks.extract()
+        ks.init_dictionary()
+        ks.synchronize()
+        return ks
+
+    if warmup > 0:
+        ks = make()
+        ks.iterate(warmup)
+        ks.synchronize()
+        ks.close()
+    ks = make()
+    ks.reset_timings()
+    t0 = time.perf_counter()
+    ks.iterate(steps)
+    ks.synchronize()
+    T = time.perf_counter() - t0
+    tm = ks.timings()
+    P = ks.P
+    ks.close()
+    return P * steps / T, T, tm, P
+
+
+def denoise(K, wl, frame, clean, steps, **kw):
+    """Whole image (gather -> iterations -> final code -> rebuild): images/s and
+    PSNR of the denoised picture (raw and min-max normalised)."""
+    s2 = K.noise_variance(K.frame_to_image(frame, wl["W"], wl["H"]))
+    ks = K.KSvd(patch=wl["patch"], slide=wl["slide"], atoms=wl["atoms"], omp_iter=wl["omp_iter"], ksvd_iter=steps,
+                **kw)
+    ks.set_frame(frame, wl["W"], wl["H"])
+    ks.synchronize()
+    t1 = time.perf_counter()
+    ks.extract()
+    ks.init_dictionary()
+    ks.run()
+    ks.rebuild(s2, copy=False)
+    ks.synchronize()
+    T_img = time.perf_counter() - t1
+    out = ks.rebuild_picture(s2)
+    ks.close()
+    return 1.0 / T_img, K.psnr(clean, out), K.psnr(clean, K.normalize(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", choices=sorted(WORKLOADS), default="512",
-                    help="image side of the workload (512: the benchmark line)")
+    ap.add_argument("--config", choices=sorted(WORKLOADS), default="2048",
+                    help="image side of the workload (2048: the benchmark line)")
     ap.add_argument("--steps", type=int, default=None, help="K-SVD iterations timed (default: the config's)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain-only", action="store_true", help="time the plain FP32 form only (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -122,72 +183,47 @@ def main():
         args.steps = wl["iters"]
     clean, noisy = K.synth_image(wl["seed"], wl["W"], wl["H"], wl["sigma"])
     frame = K.image_to_frame(noisy)
-
-    def make():
-        ks = K.KSvd(patch=wl["patch"], slide=wl["slide"], atoms=wl["atoms"], omp_iter=wl["omp_iter"],
-                    ksvd_iter=args.steps)
-        ks.set_frame(frame, wl["W"], wl["H"])
-        ks.extract()
-        ks.init_dictionary()
-        ks.synchronize()
-        return ks
-
-    # warmup on its own context: module load, first-touch, clocks
-    if args.warmup > 0:
-        ks = make()
-        ks.iterate(args.warmup)
-        ks.synchronize()
-        ks.close()
-
-    ks = make()
-    ks.reset_timings()
-    t0 = time.perf_counter()
-    ks.iterate(args.steps)
-    ks.synchronize()
-    T = time.perf_counter() - t0
-    tm = ks.timings()
-    P = ks.P
     n = wl["patch"][0] * wl["patch"][1]
     kk = wl["omp_iter"]
-    value = P * args.steps / T
 
-    # whole image (gather -> iterations -> final OMP -> rebuild), for images/s
-    s2 = K.noise_variance(noisy)
-    ks2 = K.KSvd(patch=wl["patch"], slide=wl["slide"], atoms=wl["atoms"], omp_iter=kk, ksvd_iter=args.steps)
-    ks2.set_frame(frame, wl["W"], wl["H"])
-    ks2.synchronize()
-    t1 = time.perf_counter()
-    ks2.extract()
-    ks2.init_dictionary()
-    ks2.run()
-    ks2.rebuild(s2, copy=False)
-    ks2.synchronize()
-    T_img = time.perf_counter() - t1
-    out = ks2.rebuild_picture(s2)
-    psnr_noisy = K.psnr(clean, noisy)
-    psnr_out = K.psnr(clean, K.normalize(out))
-    psnr_raw = K.psnr(clean, out)
-    ks2.close()
+    forms = {"plain": dict(precision=K.PREC_FP32)}
+    if not args.plain_only:
+        forms["bf16"] = dict(precision=K.PREC_BF16)
+        forms["learned"] = dict(precision=K.PREC_BF16, proposer=1)
+    res = {}
+    for name, kw in forms.items():
+        rate, T, tm, P = run_form(K, wl, frame, args.steps, args.warmup, **kw)
+        ips, praw, pnorm = denoise(K, wl, frame, clean, args.steps, **kw)
+        res[name] = dict(rate=rate, T=T, tm=tm, ips=ips, psnr_raw=praw, psnr_norm=pnorm)
+        print(json.dumps({"form": name, "patch_iters_per_s": round(rate, 1), "ms_per_step": round(T * 1e3 / args.steps, 4),
+                          "omp_ms_per_step": round(tm["omp_ms"] / args.steps, 4),
+                          "update_ms_per_step": round(tm["update_ms"] / args.steps, 4),
+                          "psnr_raw": round(praw, 4)}), file=sys.stderr, flush=True)
+    plain = res["plain"]
+    head = res.get("learned", plain)
+    tm = plain["tm"]
 
+    # the plain form's OMP kernel on FP32 MFMA
     omp_avg_ms = tm["omp_kernel_ms"] / max(tm["omp_calls"], 1)
     flops_launch = P * omp_flops_per_patch(n, wl["atoms"], kk)
     achieved = flops_launch / (omp_avg_ms * 1e-3) / 1e12
-    upd_avg_ms = tm["update_ms"] / max(tm["update_calls"], 1)
-    ks.close()
 
-    # Columns swept per iteration: replay the same (deterministic) iterations
-    # step by step, outside any timed region, and count each sweep's users.
-    ks3 = make()
+    # Columns swept per iteration: replay the plain form's (deterministic)
+    # iterations step by step, outside any timed region, and count each
+    # sweep's capped users.
+    ks3 = K.KSvd(patch=wl["patch"], slide=wl["slide"], atoms=wl["atoms"], omp_iter=kk, ksvd_iter=args.steps)
+    ks3.set_frame(frame, wl["W"], wl["H"])
+    ks3.extract()
+    ks3.init_dictionary()
     cols = 0
     for _ in range(args.steps):
         ks3.omp()
         cols += capped_users(*ks3.codes(), wl["atoms"], 1024)
         ks3.update()
     ks3.close()
-    atoms_swept = args.steps * wl["atoms"]
-    atom_ms = tm["update_ms"] / atoms_swept
-    atom_bytes = cols / atoms_swept * update_bytes_per_user(n, kk)
-    atom_gbs = atom_bytes / (atom_ms * 1e-3) / 1e9
+    sweep_ms = tm["update_ms"] / max(tm["update_calls"], 1)
+    sweep_bytes = cols / args.steps * update_bytes_per_user(n, kk)
+    sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     traffic = MEASURED_TRAFFIC.get(args.config, {})
 
     cpu = None
@@ -205,43 +241,60 @@ def main():
                "sample": f"one K-SVD iteration (OMP + dictionary sweep) of the same {wl['W']}x{wl['H']} workload, "
                          f"{P} patches, FP32 oracle with OpenMP, {Tc:.3f} s"}
 
+    lt = head["tm"]
     rec = {
-        "metric": "K-SVD patch-iters/sec, 1 GPU: plain FP32 form",
-        "value": round(value, 1),
+        "metric": "K-SVD patch-iters/sec, 1 GPU: BF16+learned vs plain FP32 at \u0394PSNR\u22640.05dB",
+        "value": round(head["rate"], 1),
         "unit": "patch-iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(T * 1e3 / args.steps, 4),
+        "ms_per_step": round(head["T"] * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16" if "learned" in res else "f32",
         "data": "synthetic (seeded gratings + rectangles + Gaussian noise)",
-        "config": {"workload": f"plain FP32 K-SVD, {wl['W']}x{wl['H']} sigma {wl['sigma']:g}, "
+        "config": {"workload": f"K-SVD denoising, {wl['W']}x{wl['H']} sigma {wl['sigma']:g}, "
                                f"{wl['patch'][0]}x{wl['patch'][1]} patches sliding {wl['slide'][0]}, "
                                f"{wl['atoms']} atoms, OMP limit {kk}, user cap 1024",
-                   "patches": P, "patch_dim": n, "atoms": wl["atoms"], "omp_iter": kk,
-                   "ksvd_iter": args.steps},
-        "roofline": {"bound": "hbm", "kernel": "k1_kernel + k2_kernel (one atom of the dictionary sweep)",
-                     "achieved": round(atom_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(atom_gbs / PEAK_HBM_GBS, 6),
-                     "traffic": round(traffic["sweep_atom"]) if "sweep_atom" in traffic else None,
-                     "avg_launch_ms": round(atom_ms, 5), "bytes_per_launch": round(atom_bytes),
-                     "users_per_atom": round(cols / atoms_swept, 1)},
-        "roofline_omp": {"bound": "mfma", "kernel": "omp_mfma_kernel", "achieved": round(achieved, 3),
-                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                   "patches": P, "patch_dim": n, "atoms": wl["atoms"], "omp_iter": kk, "ksvd_iter": args.steps},
+        "plain_fp32": {"value": round(plain["rate"], 1), "ms_per_step": round(plain["T"] * 1e3 / args.steps, 4),
+                       "images_per_s": round(plain["ips"], 3), "psnr_raw": round(plain["psnr_raw"], 4),
+                       "psnr_normalized": round(plain["psnr_norm"], 4)},
+        "roofline": {"bound": "hbm", "kernel": "ps_sweep_kernel (persistent dictionary sweep, plain form)",
+                     "achieved": round(sweep_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(sweep_gbs / PEAK_HBM_GBS, 6),
+                     "traffic": round(traffic["sweep"]) if "sweep" in traffic else None,
+                     "avg_launch_ms": round(sweep_ms, 5), "bytes_per_launch": round(sweep_bytes),
+                     "users_per_atom": round(cols / args.steps / wl["atoms"], 1)},
+        "roofline_omp": {"bound": "mfma", "kernel": "omp_mfma_kernel (plain form)", "achieved": round(achieved, 3),
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                          "traffic": round(traffic["omp"]) if "omp" in traffic else None,
                          "avg_launch_ms": round(omp_avg_ms, 5), "flops_per_launch": flops_launch},
-        "phase_ms_per_step": {"omp": round(tm["omp_ms"] / args.steps, 4),
-                              "update": round(upd_avg_ms, 4)},
         "power_iters_per_atom": round(tm["eig_iters"] / max(tm["update_calls"], 1) / wl["atoms"], 2),
-        "images_per_s": round(1.0 / T_img, 3),
-        "psnr": {"noisy": round(psnr_noisy, 4), "denoised_normalized": round(psnr_out, 4),
-                 "denoised_raw": round(psnr_raw, 4)},
+        "psnr_noisy": round(K.psnr(clean, noisy), 4),
         "cpu_baseline": cpu,
     }
+    if "learned" in res:
+        b = res["bf16"]
+        bt = b["tm"]
+        bomp_ms = bt["omp_kernel_ms"] / max(bt["omp_calls"], 1)
+        bach = flops_launch / (bomp_ms * 1e-3) / 1e12
+        rec["speedup_vs_plain"] = round(head["rate"] / plain["rate"], 4)
+        rec["delta_psnr_db"] = round(head["psnr_raw"] - plain["psnr_raw"], 4)
+        rec["images_per_s"] = round(head["ips"], 3)
+        rec["learned"] = {"fallback_rate": round(lt["fallback"] / max(lt["proposed"], 1), 4),
+                          "proposed": lt["proposed"], "train_ms": round(lt["train_ms"], 3),
+                          "train_loss": round(lt["train_loss"], 4), "stood_down": bool(lt["proposer_gated"]),
+                          "psnr_raw": round(head["psnr_raw"], 4)}
+        rec["bf16"] = {"value": round(b["rate"], 1), "ms_per_step": round(b["T"] * 1e3 / args.steps, 4),
+                       "speedup_vs_plain": round(b["rate"] / plain["rate"], 4),
+                       "delta_psnr_db": round(b["psnr_raw"] - plain["psnr_raw"], 4),
+                       "omp_kernel": {"kernel": "omp_bf16_kernel", "avg_launch_ms": round(bomp_ms, 5),
+                                      "achieved": round(bach, 3), "unit": "TFLOP/s (algorithmic)",
+                                      "peak_bf16": PEAK_BF16_TFLOPS, "frac": round(bach / PEAK_BF16_TFLOPS, 5),
+                                      "rescored_steps": bt["omp_rescored"], "f64_refactors": bt["omp_f64_refactors"]}}
     print(json.dumps(rec))
 
 

@@ -265,7 +265,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
                       &ctx->rout, &ctx->rpart,  &ctx->rlam, &ctx->Dbf, &ctx->Dq, &ctx->bfscal, &ctx->rbound,
                       &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
                       &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
-                      &ctx->mlp_lpart, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
+                      &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
                       &ctx->mlp_fb, &ctx->mlp_cnt};
     for (DevBuf *b : bufs)
         if (b->ptr) hipFree(b->ptr);
@@ -438,7 +438,7 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 // OMP pass, so when a pass accepts less than PROP_MIN_ACCEPT of the patches
 // the proposer stands down for the rest of this dictionary's iterations and
 // plain OMP runs instead.
-constexpr int PROP_EPOCHS0 = 2, PROP_MIN_STEPS0 = 24, PROP_MAX_STEPS0 = 96, PROP_STEPS1 = 4;
+constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4;
 constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
 constexpr uint64_t PROP_SEED = 0x5EEDull;
 

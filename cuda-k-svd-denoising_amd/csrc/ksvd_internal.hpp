@@ -68,6 +68,7 @@ struct ksvd_ctx {
     // learned support proposer (mlp.hip)
     DevBuf mlp_theta, mlp_m, mlp_v, mlp_wb, mlp_grad;        // weights (FP32 master, Adam moments, BF16), split-K grads
     DevBuf mlp_xb, mlp_h1b, mlp_h2b, mlp_zb, mlp_dz, mlp_dh2, mlp_dh1, mlp_lpart;  // one training batch
+    DevBuf mlp_tr;  // transposed operands of the backward products
     DevBuf mlp_x, mlp_perm;                                  // normalised patches (all), training order
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;

@@ -17,9 +17,9 @@
 //
 // One GEMM kernel (C = A B^T on v_mfma_f32_16x16x32_bf16, FP32
 // accumulation, 128 x 128 tiles, operands staged through LDS) serves all
-// eight products of a training step: A and B are each read either K-major
-// (stored rows x K) or M/N-major (stored K x rows), so the backward products
-// need no transposes:
+// eight products of a training step, every operand read K-major (stored
+// rows x K): the backward products take tiled transposes of the activations
+// and weights:
 //   dZ  = (sigmoid(Z) - T) / B                      (loss: mean over the batch
 //                                                    of the summed BCE)
 //   dW3 = dZ^T H2,  dH2 = (dZ W3) * [H2 > 0]
@@ -193,14 +193,33 @@ __global__ __launch_bounds__(256) void mlp_input_kernel(const float *__restrict_
     }
 }
 
-// Mean loss of the last training batch: the per-block partials summed in
-// block order.
-__global__ void loss_sum_kernel(const float *__restrict__ lpart, int n, int nb, float *__restrict__ out)
+// Mean loss of the last training batch: thread t sums partials t, t + 256,
+// ... in order, then a fixed tree over the 256 threads.
+__global__ __launch_bounds__(256) void loss_sum_kernel(const float *__restrict__ lpart, int n, int nb,
+                                                       float *__restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ float red[4];
     float s = 0.f;
-    for (int i = 0; i < n; i++) s += lpart[i];
-    *out = s / (float)nb;
+    for (int i = threadIdx.x; i < n; i += 256) s += lpart[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = ((red[0] + red[1]) + (red[2] + red[3])) / (float)nb;
+}
+
+// out [cols x rows] = in [rows x cols]^T (bf16, leading dimensions ldi / ldo),
+// 64 x 64 tiles through LDS.
+__global__ __launch_bounds__(256) void transpose_bf16_kernel(const __bf16 *__restrict__ in, int rows, int cols,
+                                                             int ldi, __bf16 *__restrict__ out, int ldo)
+{
+    __shared__ __bf16 t[64][66];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4)
+        t[r][tx] = (r0 + r < rows && c0 + tx < cols) ? in[(int64_t)(r0 + r) * ldi + c0 + tx] : (__bf16)0.f;
+    __syncthreads();
+    for (int c = ty; c < 64; c += 4)
+        if (c0 + c < cols && r0 + tx < rows) out[(int64_t)(c0 + c) * ldo + r0 + tx] = t[tx][c];
 }
 
 // rows [r0, r0 + n) of a [rows x ld] bf16 matrix: column c1 = 1, other
@@ -227,37 +246,40 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const __bf16 *__restri
 
 // dZ = (sigmoid(z) - t) / nb for atoms < K (0 for the padding), t = 1 when
 // the atom is in the support of patch perm[b0 + b] (codes idx/coef/cnt, a
-// nonzero coefficient); per-block partial sums of the loss
-// softplus(z) - t z.  One thread per (b, k).
+// nonzero coefficient); lpart[b] = the row's loss sum over the atoms of
+// softplus(z) - t z.  One block per row, thread k = atom k (ko <= 256).
 __global__ __launch_bounds__(256) void bce_grad_kernel(const float *__restrict__ Z, int ko, int K, int nb,
                                                        const int *__restrict__ perm, int b0,
                                                        const int *__restrict__ idx, const float *__restrict__ coef,
                                                        const int *__restrict__ cnt, int kmax,
                                                        __bf16 *__restrict__ dZ, float *__restrict__ lpart)
 {
+    __shared__ int sup[KSVD_MAX_SUPPORT];
+    __shared__ int ns;
     __shared__ float red[4];
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x, k = threadIdx.x;
+    const int p = perm[b0 + b];
+    if (k == 0) ns = min(cnt[p], kmax);
+    if (k < kmax) sup[k] = (k < cnt[p] && coef[(int64_t)p * kmax + k] != 0.f) ? idx[(int64_t)p * kmax + k] : -1;
+    __syncthreads();
     float l = 0.f;
-    if (q < (int64_t)nb * ko) {
-        const int b = (int)(q / ko), k = (int)(q - (int64_t)b * ko);
+    if (k < ko) {
         float d = 0.f;
         if (k < K) {
-            const int p = perm[b0 + b];
             float t = 0.f;
-            const int c = cnt[p];
-            for (int i = 0; i < c; i++)
-                if (idx[(int64_t)p * kmax + i] == k && coef[(int64_t)p * kmax + i] != 0.f) t = 1.f;
-            const float z = Z[q];
+            for (int i = 0; i < ns; i++)
+                if (sup[i] == k) t = 1.f;
+            const float z = Z[(int64_t)b * ko + k];
             const float sg = 1.f / (1.f + expf(-z));
             d = (sg - t) / (float)nb;
             l = fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))) - t * z;
         }
-        dZ[q] = (__bf16)d;
+        dZ[(int64_t)b * ko + k] = (__bf16)d;
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
     __syncthreads();
-    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) lpart[b] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // AdamW over the three weight matrices (contiguous FP32 master copy):
@@ -371,19 +393,23 @@ __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__
         const int e = sub + 16 * j;
         y[j] = (valid && e < dim) ? Y[p * dim + e] : 0.f;
     }
+    // the support's atoms (this lane's elements), read once
+    float dS[KM][RJ];
+#pragma unroll
+    for (int i = 0; i < KM; i++)
+#pragma unroll
+        for (int j = 0; j < RJ; j++) {
+            const int e = sub + 16 * j;
+            dS[i][j] = (i < kmax && S[i] >= 0 && e < dim) ? D[(int64_t)S[i] * dim + e] : 0.f;
+        }
     // b = D_S^T y and the Cholesky factor of the support Gram
     float bvec[KM], L[KM * KM], x[KM];
     bool ok = valid;
 #pragma unroll
     for (int i = 0; i < KM; i++) {
         float s = 0.f;
-        if (i < kmax && S[i] >= 0) {
 #pragma unroll
-            for (int j = 0; j < RJ; j++) {
-                const int e = sub + 16 * j;
-                if (e < dim) s = fmaf(D[(int64_t)S[i] * dim + e], y[j], s);
-            }
-        }
+        for (int j = 0; j < RJ; j++) s = fmaf(dS[i][j], y[j], s);
 #pragma unroll
         for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
         bvec[i] = s;
@@ -444,7 +470,7 @@ __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__
         float a = 0.f;
 #pragma unroll
         for (int i = 0; i < KM; i++)
-            if (i < kmax && ok && e < dim) a = fmaf(D[(int64_t)S[i] * dim + e], x[i], a);
+            if (i < kmax && ok && e < dim) a = fmaf(dS[i][j], x[i], a);
         const float r = y[j] - a;
         nr = fmaf(r, r, nr);
     }
@@ -523,7 +549,8 @@ static int mlp_alloc(ksvd_ctx *ctx)
         {&ctx->mlp_dz, sizeof(__bf16) * B * ko},
         {&ctx->mlp_dh2, sizeof(__bf16) * B * MLP_H},
         {&ctx->mlp_dh1, sizeof(__bf16) * B * MLP_H},
-        {&ctx->mlp_lpart, sizeof(float) * ((B * ko + 255) / 256)},
+        {&ctx->mlp_lpart, sizeof(float) * B},
+        {&ctx->mlp_tr, sizeof(__bf16) * ((size_t)(MLP_HP + MLP_H + kin) * B + (size_t)MLP_H * ko + (size_t)MLP_H * MLP_H)},
         {&ctx->mlp_cnt, 256},
     };
     for (auto &e : plan) {
@@ -640,41 +667,63 @@ static int mlp_forward_batch(ksvd_ctx *ctx, const __bf16 *Xb, int nb)
                                      ko, 0, nullptr, 0);
 }
 
+static int transpose(ksvd_ctx *ctx, const __bf16 *in, int rows, int cols, int ldi, __bf16 *out, int ldo)
+{
+    dim3 grid((cols + 63) / 64, (rows + 63) / 64);
+    hipLaunchKernelGGL(transpose_bf16_kernel, grid, dim3(256), 0, ctx->stream, in, rows, cols, ldi, out, ldo);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
 // Backward of the batch whose logits are in Zb and whose input is Xb: dZ is
-// already in mlp_dz.  Leaves the split-K weight-gradient partials in
-// mlp_grad.
+// already in mlp_dz.  Every product runs K-major: the activations and the
+// weights are transposed first (tiled, through LDS).  Leaves the split-K
+// weight-gradient partials in mlp_grad.
 static int mlp_backward_batch(ksvd_ctx *ctx, const __bf16 *Xb, int nb)
 {
     int kin, ko;
     mlp_dims(ctx, &kin, &ko);
     const __bf16 *W1 = (const __bf16 *)ctx->mlp_wb.ptr, *W2 = W1 + (size_t)MLP_H * kin,
                  *W3 = W2 + (size_t)MLP_H * MLP_HP;
-    (void)W1;
     const __bf16 *H1 = (const __bf16 *)ctx->mlp_h1b.ptr, *H2 = (const __bf16 *)ctx->mlp_h2b.ptr;
     const __bf16 *dZ = (const __bf16 *)ctx->mlp_dz.ptr;
     __bf16 *dH2 = (__bf16 *)ctx->mlp_dh2.ptr, *dH1 = (__bf16 *)ctx->mlp_dh1.ptr;
+    // transposed operands
+    __bf16 *T0 = (__bf16 *)ctx->mlp_tr.ptr;                  // [HP x nb]: H2^T, then H1^T
+    __bf16 *T1 = T0 + (size_t)MLP_HP * MLP_BATCH;             // [512 x nb]: dZ^T, dH2^T, dH1^T
+    __bf16 *T2 = T1 + (size_t)MLP_H * MLP_BATCH;              // [kin x nb]: X^T
+    __bf16 *W3t = T2 + (size_t)kin * MLP_BATCH;               // [512 x ko]
+    __bf16 *W2t = W3t + (size_t)MLP_H * ko;                   // [512 x 512]
     const int64_t n1 = (int64_t)MLP_H * kin, n2 = (int64_t)MLP_H * MLP_HP, n3 = (int64_t)ko * MLP_HP;
     float *g1 = (float *)ctx->mlp_grad.ptr, *g2 = g1 + MLP_SPLIT * n1, *g3 = g2 + MLP_SPLIT * n2;
     int rc;
     // dW3 [ko x HP] = dZ^T H2
-    if ((rc = gemm<false, false, EPI_F32>(ctx, dZ, ko, H2, MLP_HP, ko, MLP_HP, nb, MLP_SPLIT, g3, nullptr, MLP_HP, n3,
-                                          nullptr, 0)))
+    if ((rc = transpose(ctx, dZ, nb, ko, ko, T1, nb))) return rc;
+    if ((rc = transpose(ctx, H2, nb, MLP_HP, MLP_HP, T0, nb))) return rc;
+    if ((rc = gemm<true, true, EPI_F32>(ctx, T1, nb, T0, nb, ko, MLP_HP, nb, MLP_SPLIT, g3, nullptr, MLP_HP, n3,
+                                        nullptr, 0)))
         return rc;
     // dH2 [nb x 512] = (dZ W3) * [H2 > 0]
-    if ((rc = gemm<true, false, EPI_MASK_BF16>(ctx, dZ, ko, W3, MLP_HP, nb, MLP_H, ko, 1, nullptr, dH2, MLP_H, 0, H2,
-                                               MLP_HP)))
+    if ((rc = transpose(ctx, W3, ko, MLP_H, MLP_HP, W3t, ko))) return rc;
+    if ((rc = gemm<true, true, EPI_MASK_BF16>(ctx, dZ, ko, W3t, ko, nb, MLP_H, ko, 1, nullptr, dH2, MLP_H, 0, H2,
+                                              MLP_HP)))
         return rc;
     // dW2 [512 x HP] = dH2^T H1
-    if ((rc = gemm<false, false, EPI_F32>(ctx, dH2, MLP_H, H1, MLP_HP, MLP_H, MLP_HP, nb, MLP_SPLIT, g2, nullptr,
-                                          MLP_HP, n2, nullptr, 0)))
+    if ((rc = transpose(ctx, dH2, nb, MLP_H, MLP_H, T1, nb))) return rc;
+    if ((rc = transpose(ctx, H1, nb, MLP_HP, MLP_HP, T0, nb))) return rc;
+    if ((rc = gemm<true, true, EPI_F32>(ctx, T1, nb, T0, nb, MLP_H, MLP_HP, nb, MLP_SPLIT, g2, nullptr, MLP_HP, n2,
+                                        nullptr, 0)))
         return rc;
     // dH1 [nb x 512] = (dH2 W2) * [H1 > 0]
-    if ((rc = gemm<true, false, EPI_MASK_BF16>(ctx, dH2, MLP_H, W2, MLP_HP, nb, MLP_H, MLP_H, 1, nullptr, dH1, MLP_H,
-                                               0, H1, MLP_HP)))
+    if ((rc = transpose(ctx, W2, MLP_H, MLP_H, MLP_HP, W2t, MLP_H))) return rc;
+    if ((rc = gemm<true, true, EPI_MASK_BF16>(ctx, dH2, MLP_H, W2t, MLP_H, nb, MLP_H, MLP_H, 1, nullptr, dH1, MLP_H,
+                                              0, H1, MLP_HP)))
         return rc;
     // dW1 [512 x kin] = dH1^T X
-    return gemm<false, false, EPI_F32>(ctx, dH1, MLP_H, Xb, kin, MLP_H, kin, nb, MLP_SPLIT, g1, nullptr, kin, n1,
-                                       nullptr, 0);
+    if ((rc = transpose(ctx, dH1, nb, MLP_H, MLP_H, T1, nb))) return rc;
+    if ((rc = transpose(ctx, Xb, nb, kin, kin, T2, nb))) return rc;
+    return gemm<true, true, EPI_F32>(ctx, T1, nb, T2, nb, MLP_H, kin, nb, MLP_SPLIT, g1, nullptr, kin, n1, nullptr,
+                                     0);
 }
 
 static int mlp_adamw(ksvd_ctx *ctx, float lr)
@@ -769,8 +818,8 @@ int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, do
         cleanup();
         return rc;
     }
-    const int nblk = (int)(((int64_t)nb * ko + 255) / 256);
-    hipLaunchKernelGGL(bce_grad_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr, ko,
+    const int nblk = nb;
+    hipLaunchKernelGGL(bce_grad_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr, ko,
                        ctx->K, nb, (const int *)dperm.ptr, 0, (const int *)didx.ptr, (const float *)dcoef.ptr,
                        (const int *)dcnt.ptr, kmx, (__bf16 *)ctx->mlp_dz.ptr, (float *)ctx->mlp_lpart.ptr);
     if ((rc = mlp_backward_batch(ctx, Xb, nb))) {
@@ -819,15 +868,15 @@ static int mlp_train_list(ksvd_ctx *ctx, const int *perm, int64_t n, int steps, 
                            (__bf16 *)ctx->mlp_xb.ptr);
         const __bf16 *Xb = (const __bf16 *)ctx->mlp_xb.ptr;
         if ((rc = mlp_forward_batch(ctx, Xb, B))) return rc;
-        const int nblk = (int)(((int64_t)B * ko + 255) / 256);
-        hipLaunchKernelGGL(bce_grad_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr,
+        const int nblk = B;
+        hipLaunchKernelGGL(bce_grad_kernel, dim3(B), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr,
                            ko, ctx->K, B, perm, (int)b0, (const int *)ctx->idx.ptr, (const float *)ctx->coef.ptr,
                            (const int *)ctx->cnt.ptr, ctx->kmax, (__bf16 *)ctx->mlp_dz.ptr,
                            (float *)ctx->mlp_lpart.ptr);
         if ((rc = mlp_backward_batch(ctx, Xb, B))) return rc;
         if ((rc = mlp_adamw(ctx, lr))) return rc;
         if (st == steps - 1)
-            hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, (const float *)ctx->mlp_lpart.ptr,
+            hipLaunchKernelGGL(loss_sum_kernel, dim3(1), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_lpart.ptr,
                                nblk, B, (float *)ctx->mlp_cnt.ptr + 1);
     }
     KSVD_HIP_CHECK(ctx, hipGetLastError());
