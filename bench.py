@@ -262,7 +262,8 @@ def main():
         "plain_fp32": {"value": round(plain["rate"], 1), "ms_per_step": round(plain["T"] * 1e3 / args.steps, 4),
                        "images_per_s": round(plain["ips"], 3), "psnr_raw": round(plain["psnr_raw"], 4),
                        "psnr_normalized": round(plain["psnr_norm"], 4)},
-        "roofline": {"bound": "hbm", "kernel": "ps_sweep_kernel (persistent dictionary sweep, plain form)",
+        "roofline": {"bound": "hbm", "kernel": "dictionary update of the plain form: ps_sweep_kernel (persistent sweep) "
+                               "+ its users / column-table kernels",
                      "achieved": round(sweep_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(sweep_gbs / PEAK_HBM_GBS, 6),
                      "traffic": round(traffic["sweep"]) if "sweep" in traffic else None,
