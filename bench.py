@@ -19,7 +19,8 @@ iterations / wall time of the iterations, inputs resident in HBM);
 ratio, `delta_psnr_db` the denoised PSNR difference to the plain form.
 
 `roofline` is for the kernel that dominates the step: the persistent
-dictionary sweep (ps_sweep_kernel, one launch per iteration).  Its
+dictionary sweep (pt_sweep_kernel, the two-team sweep, one launch per
+iteration).  Its
 algorithmic bytes are those of the reference's update for the capped users
 of every atom (re-read each user's patch and code, write its new
 coefficient); its duration is the library's HIP-event span of the update on
@@ -57,7 +58,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (130120.8 * 2 + 416736.1) * 1024, "omp": (132511.1 * 2 + 19816.8) * 1024},
+    "2048": {"omp": (132511.1 * 2 + 19816.8) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
@@ -262,8 +263,8 @@ def main():
         "plain_fp32": {"value": round(plain["rate"], 1), "ms_per_step": round(plain["T"] * 1e3 / args.steps, 4),
                        "images_per_s": round(plain["ips"], 3), "psnr_raw": round(plain["psnr_raw"], 4),
                        "psnr_normalized": round(plain["psnr_norm"], 4)},
-        "roofline": {"bound": "hbm", "kernel": "dictionary update of the plain form: ps_sweep_kernel (persistent sweep) "
-                               "+ its users / column-table kernels",
+        "roofline": {"bound": "hbm", "kernel": "dictionary update of the plain form: pt_sweep_kernel (two-team persistent "
+                               "sweep) + its users / column-table / cap-zeroing kernels",
                      "achieved": round(sweep_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(sweep_gbs / PEAK_HBM_GBS, 6),
                      "traffic": round(traffic["sweep"]) if "sweep" in traffic else None,

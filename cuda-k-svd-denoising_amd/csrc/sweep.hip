@@ -1,8 +1,17 @@
-// The Gauss-Seidel dictionary sweep (cuda_svd_denoiser.cu:309-361) as two
-// kernels per atom, spread over CUs, with no grid-wide barrier (a kernel
-// boundary is the only cross-workgroup synchronisation).
+// The Gauss-Seidel dictionary sweep (cuda_svd_denoiser.cu:309-361).  Atom k
+// depends on atom k-1's new vector and coefficients, so the sweep is a chain
+// of 256 short dependent steps; three implementations, chosen in launch_sweep:
 //
-// K1(k), B1 workgroups x 1024 threads:
+//  * two-team persistent sweep (pt_sweep_kernel, the default for user caps up
+//    to 1024): two teams of 16 workgroups on XCD 0 take alternate atoms; a
+//    team runs its atom's power iteration as all-to-all exchanges of
+//    E_h (E_h^T u) vectors while the other team prepares the next atom;
+//  * master/helper persistent sweep (ps_sweep_kernel, user caps up to 1984):
+//    helpers form partial Gram matrices on MFMA, a master workgroup runs the
+//    power iteration on G = E E^T;
+//  * two kernels per atom (k1_kernel, k2_kernel: any cap, including none).
+//
+// The two-kernel form, K1(k), B1 workgroups x 1024 threads:
 //   (a) the new coefficients of atom k-1 for its users: x_t = e_t . d_{k-1}
 //       for its first min(c, cap) users, 0 for the rest;
 //   (b) the restricted residual E_k of atom k, a slice of columns per
@@ -979,11 +988,498 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
     else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, Gred, sm, &s_bail);
 }
 
-// Clears the sweep's flags (all but the sticky abort word) before a launch.
-__global__ void ps_reset_kernel(unsigned *flags)
+// ===========================================================================
+// Two-team persistent sweep: no master and no Gram matrix.  2 x PT_HT
+// workgroups of PT_T threads on XCD 0 form two teams; team t takes the atoms
+// k = t, t + 2, ...  The PT_HT helpers of a team hold atom k's capped columns
+// (helper h: [h*cw, (h+1)*cw), cw = ceil(m / PT_HT)) in LDS and run the power
+// iteration together.  Exchange i: each forms w_h = E_h (E_h^T u_{i-1}) from
+// its own columns and stores it as (sequence number, value) pairs, one 64-bit
+// store each; every team member reads the team's PT_HT vectors, rereading a
+// pair until it carries the exchange's number (no flag and no store wait on
+// this path), and sums them in a fixed order: z_i = E_k E_k^T u_{i-1},
+// u_i = z_i / |z_i|.  All members run the same code on the same values, so
+// they hold the same u_i and take the same stop decision bit for bit.  The
+// pairs alternate between two buffers per team by sequence parity: a member
+// reads exchange s's pairs before it stores its pairs of s + 1, so nobody
+// stores s + 2 while a member may still read s.  The buffers are cleared
+// before each launch, so no pair from an earlier sweep carries a current
+// number.
+//
+// While one team iterates on atom k, the other prepares its next atom k+1:
+// the final coefficients of its atom k-1, then atom k+1's columns without
+// atom k (atom k's share is handed over when d_k is out), and the columns of
+// E_k that this hand-over needs.  Atom k, team t, helper h:
+//   P2  once team 1-t has published d_{k-1} (done[1-t] = k): for each column
+//       whose patch is among atom k-1's first m_{k-1} users (rank r):
+//       x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x (column r prefetched);
+//   PI  exchanges i = 1, 2, ... until |u_i - u_{i-1}|^2 < tol2 with i >= 2,
+//       or i = maxit, or z_i = 0 (then u_{i-1} stays).  After exchange 1's
+//       store: its E_k columns to global memory and ecert[h] = k + 1 (after
+//       a store wait);
+//   out helper 0 stores d_k and sets done[t] = k + 1 (after a store wait);
+//   P4  atom k's final coefficients for its columns (x = e . d_k, the same
+//       arithmetic on the same values as team 1-t's P2, so both agree bit for
+//       bit); ccert[h] = k + 1 (after a store wait);
+//   P1  once every ccert of team t is k + 1, every ccert of team 1-t is k and
+//       done[t] = k + 1: atom k+2's column table without atom k+1, its
+//       patches, and every other atom's share subtracted (atoms <= k final,
+//       the others as they were); then, once every ecert of team 1-t is
+//       k + 2, the columns of E_{k+1} that atom k+2's hand-over needs.
+// Users beyond the cap get coefficient 0 after the sweep (pa_zero_kernel);
+// until then the column tables drop their entries.
+constexpr int PT_T = 1024;               // threads per helper workgroup
+constexpr int PT_W = PT_T / 64;          // wavefronts per helper
+constexpr int PT_CPW = PS_CWMAX / PT_W;  // columns per wavefront at most
+constexpr int PT_HT = 16;                // helpers per team (the two teams fill XCD 0's 32 CUs)
+constexpr int PT_GRP = 8;                // partial sums per element of the exchange
+constexpr int PT_XH = 4;                 // exchange pairs read per thread (PT_HT over >= 5 groups)
+// flag words: per-XCD registration; per-helper ecert and ccert (index
+// t * PT_HT + h); per-team done; the abort word and its sticky copy
+constexpr int PTF_ECERT = 32, PTF_CCERT = 64, PTF_DONE = 96;
+
+__device__ __forceinline__ unsigned long long ld64_c(const unsigned long long *p)
 {
-    const int i = threadIdx.x;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st64_c(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int cols_of(int m, int H, int h, int *tb)
+{
+    const int cw = (m + H - 1) / H;
+    *tb = h * cw;
+    return max(0, min(m, *tb + cw) - *tb);
+}
+
+// Capped user count of atom k from the LDS copy of uoff.
+__device__ __forceinline__ int pa_m(const int *suo, int k, int cap)
+{
+    const int c = suo[k + 1] - suo[k];
+    return c > cap ? cap : c;
+}
+
+// Column table of this helper's columns [tb, tb + nc) of atom kb: the patch
+// (sp), the column's own coefficient slot (ss), every support entry except
+// atom kb-1's whose coefficient is not 0 at this point of the sweep, as atom
+// and coefficient index (sat, sci; -1 for none), and the rank of the patch
+// among atom kb-1's first mx users where it is one (hrank, listed in hl in any
+// order).  An entry of an atom j < kb-1 (updated already) whose patch is not
+// among atom j's first m_j users has coefficient 0 from atom j's update on;
+// its coefficient word is cleared only after the sweep, so it is dropped here.
+template <int NO>
+__device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int kb, int tb, int nc, int mx, int *sp,
+                         int *ss, int *sat, int *sci, int *hrank, int *hl, int *nhp)
+{
+    const int no = a.kmax > 1 ? a.kmax - 1 : 1;
+    const int64_t cb = (int64_t)smo[kb] + tb;
+    for (int tl = threadIdx.x; tl < nc; tl += PT_T) {
+        const int64_t col = cb + tl;
+        sp[tl] = a.tab_p[col];
+        ss[tl] = a.tab_self[col];
+        // all loads unconditional (entries past `no` reread the last one and
+        // are then dropped), so none waits on a branch
+        int4 o[NO];
+#pragma unroll
+        for (int i = 0; i < NO; i++) o[i] = a.tab_oth[col * no + min(i, no - 1)];
+        int hr = -1;
+#pragma unroll
+        for (int i = 0; i < NO; i++) {
+            int at = i < no ? o[i].x : -1;
+            const int rk = o[i].z;
+            if (at >= 0 && at == kb - 1) {
+                if (rk >= 0 && rk < mx) hr = rk;
+                at = -1;
+            } else if (at >= 0 && (rk < 0 || (at < kb - 1 && rk >= pa_m(suo, at, a.cap)))) {
+                at = -1;
+            }
+            sat[tl * PS_NOMAX + i] = at;
+            sci[tl * PS_NOMAX + i] = at >= 0 ? o[i].y : -1;
+        }
+        hrank[tl] = hr;
+        if (hr >= 0) hl[atomicAdd(nhp, 1)] = tl;
+    }
+}
+
+// The columns e_t = y_p - sum_i d_{sat_i} x_i (x_i read at sci_i) of the
+// table into Es (pitch dpad), element-parallel: thread (tc, te) takes element
+// te of BU columns at once, with every load in flight before the first use.
+template <int NO, int BU>
+__device__ void pa_build(const SweepArgs &a, int nc, float *Es, const int *sp, const int *sat, const int *sci)
+{
+    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
+    const int cpt = PT_T / dim;
+    const int tc = tid / dim, te = tid - tc * dim;
+    if (tc >= cpt) return;
+    for (int t = tc; t < nc; t += BU * cpt) {
+        float v[BU], dv[BU][NO], xv[BU][NO];
+        // a column past nc rereads column nc-1, an empty entry reads atom 0
+        // and coefficient 0 and is then dropped
+#pragma unroll
+        for (int b = 0; b < BU; b++) {
+            const int tt = min(t + b * cpt, nc - 1);
+            v[b] = a.Y[(int64_t)sp[tt] * dim + te];
+#pragma unroll
+            for (int i = 0; i < NO; i++) {
+                dv[b][i] = ld_c(a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te);
+                xv[b][i] = ld_c(a.coef + max(sci[tt * PS_NOMAX + i], 0));
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < BU; b++) {
+            const int tt = t + b * cpt;
+            if (tt < nc) {
+#pragma unroll
+                for (int i = 0; i < NO; i++)
+                    if (sci[tt * PS_NOMAX + i] >= 0) v[b] = fmaf(-dv[b][i], xv[b][i], v[b]);
+                Es[tt * dpad + te] = v[b];
+            }
+        }
+    }
+}
+
+// After the sweep: coefficient 0 for every user of atom k beyond its first
+// min(c_k, cap) (one workgroup per atom).
+__global__ void pa_zero_kernel(SweepArgs a)
+{
+    const int k = blockIdx.x;
+    const int base = a.uoff[k], c = a.uoff[k + 1] - base;
+    for (int t = a.cap + threadIdx.x; t < c; t += blockDim.x) {
+        const int pos = base + t;
+        a.coef[(int64_t)a.users[pos] * a.kmax + a.uslot[pos]] = 0.f;
+    }
+}
+
+template <int NO, int BU>
+__device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, unsigned long long *Wx, float *sm,
+                          int *bail)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int K = a.K, cap = a.cap, H = PT_HT;
+    int dim = a.dim, dpad = a.dpad;
+    const unsigned ebytes = (unsigned)((size_t)cap * dpad * sizeof(float));
+    // LDS layout (pitch dpad)
+    float *Es = sm;                           // PS_CWMAX x dpad: this helper's columns of the team's atom
+    float *Eh = Es + PS_CWMAX * dpad;         // PS_CWMAX x dpad: columns of E_{k-1} for the hand-over (hl order)
+    float *dprev = Eh + PS_CWMAX * dpad;      // dpad: d_{k-1}, then d_k for P4
+    float *d0 = dprev + dpad;                 // dpad: atom k as it stands (the starting vector)
+    float *wpw = d0 + dpad;                   // PT_W x dpad: per-wavefront partial sums of E v
+    float *wx = wpw + PT_W * dpad;            // PT_GRP x dpad: partial sums of the exchange
+    int *sat = (int *)(wx + PT_GRP * dpad);   // PS_CWMAX x PS_NOMAX atoms (-1: none)
+    int *sci = sat + PS_CWMAX * PS_NOMAX;     // PS_CWMAX x PS_NOMAX coefficient indices (-1: none)
+    int *sp = sci + PS_CWMAX * PS_NOMAX;      // PS_CWMAX patches
+    int *hrank = sp + PS_CWMAX;               // PS_CWMAX: rank among atom k-1's users, -1: none
+    int *hl = hrank + PS_CWMAX;               // PS_CWMAX: columns with a hand-over
+    int *nhp = hl + PS_CWMAX;                 // their number
+    int *ss = nhp + 1;                        // PS_CWMAX: own coefficient slots of the columns
+    int *suo = ss + PS_CWMAX;                 // K + 1: uoff
+    int *smo = suo + K + 1;                   // K + 1: moff
+    unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
+    const int me = team * PT_HT + h, other = (1 - team) * PT_HT;
+    unsigned long long *Wt = Wx + (size_t)team * 2 * PT_HT * dpad;  // this team's two pair buffers
+    unsigned seq = 0;
+    unsigned long long iters = 0;
+
+    for (int i = tid; i <= K; i += PT_T) {
+        suo[i] = a.uoff[i];
+        smo[i] = a.moff[i];
+    }
+    if (tid == 0) *nhp = 0;
+    __syncthreads();
+    // the team's first atom k = team: its columns without atom k-1 (nothing
+    // is final yet) and atom k's vector
+    int k = team;
+    if (k < K) {
+        int tb0;
+        const int nc0 = cols_of(pa_m(suo, k, cap), H, h, &tb0);
+        pa_table<NO>(a, suo, smo, k, tb0, nc0, k >= 1 ? pa_m(suo, k - 1, cap) : 0, sp, ss, sat, sci, hrank, hl, nhp);
+        if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)k * dim + tid);
+        __syncthreads();
+        pa_build<NO, BU>(a, nc0, Es, sp, sat, sci);
+        __syncthreads();
+    }
+    bool eh_ready = false;  // Eh holds the hand-over columns for atom k
+    for (; k < K; k += 2) {
+        // dim and dpad pass through an opaque copy each atom, so that offsets
+        // derived from them are recomputed where they are used instead of
+        // being held in registers across the loop
+        asm volatile("" : "+s"(dim), "+s"(dpad));
+        int tb;
+        const int m = pa_m(suo, k, cap);
+        const int nc = cols_of(m, H, h, &tb);
+        float *Ecur = (k & 1) ? a.E1 : a.E0;
+        const float *Eprev = (k & 1) ? a.E0 : a.E1;
+        // ---- P2: atom k-1's share --------------------------------------------
+        if (k >= 1) {
+            const int nh = *nhp;
+            if (!eh_ready) {
+                // (the first atom of team 1) the columns of E_{k-1}
+                if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)k, bail)) return;
+                const int d4 = dpad / 4;
+                for (int q = tid; q < nh * d4; q += PT_T) {
+                    const int i = q / d4, c4 = q - i * d4;
+                    *(f4 *)(Eh + i * dpad + 4 * c4) =
+                        ld4_c(Eprev, ebytes, (unsigned)(((size_t)hrank[hl[i]] * dpad + 4 * c4) * sizeof(float)));
+                }
+            }
+            if (!ps_wait(flags, PTF_DONE + 1 - team, 1, (unsigned)k, bail)) return;
+            if (tid < dim) dprev[tid] = ld_c(a.D + (int64_t)(k - 1) * dim + tid);
+            __syncthreads();
+            float dv[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int e = lane + 64 * j;
+                dv[j] = e < dim ? dprev[e] : 0.f;
+            }
+            for (int i = wave; i < nh; i += PT_W) {
+                const int tl = hl[i];
+                float ev[3];
+#pragma unroll
+                for (int j = 0; j < 3; j++) ev[j] = Eh[i * dpad + min(lane + 64 * j, dim - 1)];
+                const float x = ps_dot(ev, dv, dim);
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int e = lane + 64 * j;
+                    if (e < dim) Es[tl * dpad + e] = fmaf(-dv[j], x, Es[tl * dpad + e]);
+                }
+            }
+            __syncthreads();
+        }
+        // ---- PI ------------------------------------------------------------------
+        float uj[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int e = lane + 64 * j;
+            uj[j] = e < dim ? d0[e] : 0.f;
+        }
+        // E_k's columns out for team 1-t's hand-over of atom k+1, and ecert
+        auto store_e = [&]() {
+            const int d4 = dpad / 4;
+            for (int q = tid; q < nc * d4; q += PT_T) {
+                const int tl = q / d4, c4 = q - tl * d4;
+                *(f4 *)(Ecur + (int64_t)(tb + tl) * dpad + 4 * c4) = *(const f4 *)(Es + tl * dpad + 4 * c4);
+            }
+            ps_publish(ecert + me, (unsigned)(k + 1));
+        };
+        // issue: v_t = e_t . u_{i-1} for the wavefront's columns, its partial
+        // sum of e_t v_t; w_h = the PT_W partials in wavefront order, stored
+        // as pairs
+        auto issue = [&](unsigned s) {
+            unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
+            float ev[PT_CPW][3];
+#pragma unroll
+            for (int g = 0; g < PT_CPW; g++) {
+                const int t = min(wave + PT_W * g, max(nc - 1, 0));
+#pragma unroll
+                for (int j = 0; j < 3; j++) ev[g][j] = Es[t * dpad + min(lane + 64 * j, dim - 1)];
+            }
+            float pw[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int g = 0; g < PT_CPW; g++) {
+                const float v = ps_dot(ev[g], uj, dim);
+                if (wave + PT_W * g < nc)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) pw[j] = fmaf(ev[g][j], v, pw[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int e = lane + 64 * j;
+                if (e < dim) wpw[wave * dpad + e] = pw[j];
+            }
+            __syncthreads();
+            if (tid < dim) {
+                float w = 0.f;
+#pragma unroll
+                for (int q = 0; q < PT_W; q++) w += wpw[q * dpad + tid];
+                st64_c(Wb + (size_t)h * dpad + tid, ((unsigned long long)s << 32) | __float_as_uint(w));
+            }
+        };
+        // collect: z_i = the team's vectors of exchange s (group gq sums
+        // helpers gq, gq + ngrp, ..., then the groups in order), then u_i;
+        // returns -1 to leave (abort), 1 when the iteration stops, else 0
+        int it = 0;
+        auto collect = [&](unsigned s) -> int {
+            const unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
+            const int ngrp = min(PT_T / dim, PT_GRP);
+            const int gq = tid / dim, ge = tid - gq * dim;
+            if (gq < ngrp) {
+                unsigned long long pv[PT_XH];
+#pragma unroll
+                for (int j = 0; j < PT_XH; j++) {
+                    const int hh = gq + ngrp * j;
+                    pv[j] = hh < H ? ld64_c(Wb + (size_t)hh * dpad + ge) : ((unsigned long long)s << 32);
+                }
+                // a thread still missing pairs polls its first missing one
+                // alone, then rereads the rest
+                long spins = 0;
+                for (;;) {
+                    int miss = -1;
+#pragma unroll
+                    for (int j = PT_XH - 1; j >= 0; j--)
+                        if ((unsigned)(pv[j] >> 32) != s) miss = j;
+                    if (miss < 0) break;
+                    const unsigned long long *pm = Wb + (size_t)(gq + ngrp * miss) * dpad + ge;
+                    while ((unsigned)(ld64_c(pm) >> 32) != s) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if ((++spins & 15) == 0) {
+                            if (ld_flag(flags + PSF_ABORT)) {
+                                *bail = 1;
+                                break;
+                            }
+                            if (spins > PS_SPIN_MAX / 2) {
+                                st_flag(flags + PSF_STICKY, 1u);
+                                st_flag(flags + PSF_ABORT, 1u);
+                                *bail = 1;
+                                break;
+                            }
+                        }
+                    }
+                    if (*bail) break;
+#pragma unroll
+                    for (int j = 0; j < PT_XH; j++) {
+                        const int hh = gq + ngrp * j;
+                        if (hh < H && (unsigned)(pv[j] >> 32) != s) pv[j] = ld64_c(Wb + (size_t)hh * dpad + ge);
+                    }
+                }
+                float z = __uint_as_float((unsigned)pv[0]);
+#pragma unroll
+                for (int j = 1; j < PT_XH; j++)
+                    if (gq + ngrp * j < H) z += __uint_as_float((unsigned)pv[j]);
+                wx[gq * dpad + ge] = z;
+            }
+            __syncthreads();
+            if (*bail) return -1;
+            // every wavefront forms z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
+            float zj[3], n2 = 0.f;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int e = lane + 64 * j;
+                float z = 0.f;
+                if (e < dim)
+                    for (int q = 0; q < ngrp; q++) z += wx[q * dpad + e];
+                zj[j] = z;
+                n2 = fmaf(z, z, n2);
+            }
+            n2 = wave_sum_dpp(n2);
+            if (n2 == 0.f) return 1;  // E = 0: keep u_{i-1} (d_k at the start; its coefficients become 0)
+            const float inv = 1.f / sqrtf(n2);
+            float d2 = 0.f;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float un = zj[j] * inv;
+                const float du = un - uj[j];
+                d2 = fmaf(du, du, d2);
+                uj[j] = un;
+            }
+            d2 = wave_sum_dpp(d2);
+            it++;
+            return ((it >= 2 && d2 < a.tol2) || it >= a.maxit) ? 1 : 0;
+        };
+        if (m > 0) {
+            issue(++seq);
+            store_e();
+            int r = collect(seq);
+            while (r == 0) {
+                issue(++seq);
+                r = collect(seq);
+            }
+            if (r < 0) return;
+            iters += (unsigned long long)it;
+        } else {
+            store_e();
+        }
+        // ---- d_k out ---------------------------------------------------------
+        __syncthreads();  // every wavefront has read wx
+        if (wave == 0) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int e = lane + 64 * j;
+                if (e < dim) {
+                    dprev[e] = uj[j];
+                    if (h == 0) a.D[(int64_t)k * dim + e] = uj[j];
+                }
+            }
+        }
+        if (h == 0) ps_publish(flags + PTF_DONE + team, (unsigned)(k + 1));
+        else __syncthreads();
+        // ---- P4: atom k's final coefficients ---------------------------------
+        {
+            float dv[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int e = lane + 64 * j;
+                dv[j] = e < dim ? dprev[e] : 0.f;
+            }
+#pragma unroll
+            for (int g = 0; g < PT_CPW; g++) {
+                const int tl = wave + PT_W * g;
+                if (tl >= nc) continue;
+                float ev[3];
+#pragma unroll
+                for (int j = 0; j < 3; j++) ev[j] = Es[tl * dpad + min(lane + 64 * j, dim - 1)];
+                const float x = ps_dot(ev, dv, dim);
+                if (lane == 0) a.coef[ss[tl]] = x;
+            }
+        }
+        ps_publish(ccert + me, (unsigned)(k + 1));
+        // ---- P1: atom k+2's columns without atom k+1 ------------------------------
+        const int kn = k + 2;
+        if (kn >= K) break;
+        if (!ps_wait(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), bail)) return;
+        if (k >= 1 && !ps_wait(flags, PTF_CCERT + other, H, (unsigned)k, bail)) return;
+        if (!ps_wait(flags, PTF_DONE + team, 1, (unsigned)(k + 1), bail)) return;
+        if (tid == 0) *nhp = 0;
+        __syncthreads();
+        int tbn;
+        const int ncn = cols_of(pa_m(suo, kn, cap), H, h, &tbn);
+        pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp);
+        if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
+        __syncthreads();
+        pa_build<NO, BU>(a, ncn, Es, sp, sat, sci);
+        // the columns of E_{k+1} that atom k+2's hand-over needs
+        if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
+        {
+            const int nh = *nhp, d4 = dpad / 4;
+            const float *Enx = (kn & 1) ? a.E0 : a.E1;  // E_{k+1}
+            for (int q = tid; q < nh * d4; q += PT_T) {
+                const int i = q / d4, c4 = q - i * d4;
+                *(f4 *)(Eh + i * dpad + 4 * c4) =
+                    ld4_c(Enx, ebytes, (unsigned)(((size_t)hrank[hl[i]] * dpad + 4 * c4) * sizeof(float)));
+            }
+        }
+        eh_ready = true;
+        __syncthreads();
+    }
+    if (h == 0 && tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], iters);
+}
+
+template <int NO, int BU>
+__global__ __launch_bounds__(PT_T) void pt_sweep_kernel(SweepArgs a, unsigned *flags, unsigned long long *Wx)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail;
+    if (threadIdx.x == 0) {
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+        s_role = (xcc == 0 && slot < 2u * PT_HT) ? (int)slot : -1;
+        s_bail = 0;
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    pt_helper<NO, BU>(a, flags, role & 1, role >> 1, Wx, sm, &s_bail);
+}
+
+// Clears the sweep's flags (all but the sticky abort word) and the nw
+// exchange pairs before a launch.
+__global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < PSF_WORDS && i != PSF_STICKY) flags[i] = 0u;
+    if (i < nw) wx[i] = 0ull;
 }
 
 }  // namespace
@@ -1032,6 +1528,27 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
 
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
+    if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL")) {
+        const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
+                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
+        if (lds <= 160 * 1024) {
+            typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
+            PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 3>;
+            KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)pt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            unsigned *flags = (unsigned *)ctx->ticket.ptr;
+            // the exchange pairs (two teams x two buffers of PT_HT x dpad) sit
+            // at the start of the partial-Gram buffer
+            unsigned long long *wx = (unsigned long long *)a.Gpart;
+            const int nw = 2 * 2 * PT_HT * dpad;
+            hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
+                               ctx->stream, flags, wx, nw);
+            // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on XCD 0
+            hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
+            hipLaunchKernelGGL(pa_zero_kernel, dim3(K), dim3(256), 0, ctx->stream, a);
+            KSVD_HIP_CHECK(ctx, hipGetLastError());
+            return KSVD_OK;
+        }
+    }
     int H = 24;
     if (const char *hv = getenv("KSVD_SWEEP_HELPERS")) H = std::max(1, std::min(PS_HMAX, atoi(hv)));
     if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
@@ -1051,7 +1568,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
         unsigned *flags = (unsigned *)ctx->ticket.ptr;
         // the reduced Gram follows the partials in the same buffer
         float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
-        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags);
+        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags, (unsigned long long *)nullptr, 0);
         // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on XCD 0
         hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
         KSVD_HIP_CHECK(ctx, hipGetLastError());

@@ -128,10 +128,12 @@ def signfix(D):
     (CFG1, 1024, "two_kernel"), (CFG2, 1024, "two_kernel"),
     (dict(CFG1, k=1), 1024, "persistent"), (dict(CFG1, k=8), 1024, "persistent"),
     (dict(CFG1, k=8), 1024, "two_kernel"), (dict(CFG1, k=1), 1024, "two_kernel"),
+    (CFG2, 1536, "persistent"), (dict(CFG2, k=8), 1536, "persistent"), (CFG2, 64, "persistent"),
 ], ids=["128sq_cap", "512sq_cap", "128sq_nocap", "128sq_cap_2k", "512sq_cap_2k", "128sq_k1", "128sq_k8",
-        "128sq_k8_2k", "128sq_k1_2k"])
+        "128sq_k8_2k", "128sq_k1_2k", "512sq_cap1536_master", "512sq_k8_cap1536_master", "512sq_cap64"])
 def test_update_parity(kl, orc, cfg, cap, sweep, monkeypatch):
-    # the persistent one-XCD sweep runs whenever the cap allows; the
+    # the persistent one-XCD sweeps run whenever the cap allows (the two-team
+    # sweep for caps up to 1024, the master/helper sweep above); the
     # two-kernel sweep is the one for no cap, or when forced
     monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1" if sweep == "two_kernel" else "0")
     clean, noisy, ks = make(kl, cfg, user_cap=cap)
