@@ -1010,14 +1010,18 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 // the final coefficients of its atom k-1, then atom k+1's columns without
 // atom k (atom k's share is handed over when d_k is out), and the columns of
 // E_k that this hand-over needs.  Atom k, team t, helper h:
-//   P2  once team 1-t has published d_{k-1} (done[1-t] = k): for each column
+//   P2  once team 1-t's d_{k-1} pairs (tagged k) are in: for each column
 //       whose patch is among atom k-1's first m_{k-1} users (rank r):
 //       x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x (column r prefetched);
 //   PI  exchanges i = 1, 2, ... until |u_i - u_{i-1}|^2 < tol2 with i >= 2,
 //       or i = maxit, or z_i = 0 (then u_{i-1} stays).  After exchange 1's
 //       store: its E_k columns to global memory and ecert[h] = k + 1 (after
-//       a store wait);
-//   out helper 0 stores d_k and sets done[t] = k + 1 (after a store wait);
+//       a store wait).  Only
+//       PT_PW wavefronts run the arithmetic on the vectors; the others take
+//       part in the exchange's reads and the barriers;
+//   out helper 0 stores d_k as pairs tagged k + 1 for team 1-t (no store
+//       wait on this path), then into the dictionary, and sets done[t] =
+//       k + 1 (after a store wait);
 //   P4  atom k's final coefficients for its columns (x = e . d_k, the same
 //       arithmetic on the same values as team 1-t's P2, so both agree bit for
 //       bit); ccert[h] = k + 1 (after a store wait);
@@ -1030,7 +1034,8 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 // until then the column tables drop their entries.
 constexpr int PT_T = 1024;               // threads per helper workgroup
 constexpr int PT_W = PT_T / 64;          // wavefronts per helper
-constexpr int PT_CPW = PS_CWMAX / PT_W;  // columns per wavefront at most
+constexpr int PT_PW = 8;                 // wavefronts that run the power iteration's arithmetic
+constexpr int PT_CPW = PS_CWMAX / PT_PW; // columns per such wavefront at most
 constexpr int PT_HT = 16;                // helpers per team (the two teams fill XCD 0's 32 CUs)
 constexpr int PT_GRP = 8;                // partial sums per element of the exchange
 constexpr int PT_XH = 4;                 // exchange pairs read per thread (PT_HT over >= 5 groups)
@@ -1175,11 +1180,14 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     int *hl = hrank + PS_CWMAX;               // PS_CWMAX: columns with a hand-over
     int *nhp = hl + PS_CWMAX;                 // their number
     int *ss = nhp + 1;                        // PS_CWMAX: own coefficient slots of the columns
-    int *suo = ss + PS_CWMAX;                 // K + 1: uoff
+    int *sdec = ss + PS_CWMAX;                // the power iteration's stop decision
+    int *suo = sdec + 1;                      // K + 1: uoff
     int *smo = suo + K + 1;                   // K + 1: moff
     unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
     const int me = team * PT_HT + h, other = (1 - team) * PT_HT;
     unsigned long long *Wt = Wx + (size_t)team * 2 * PT_HT * dpad;  // this team's two pair buffers
+    // d_k pairs of each team (after both teams' exchange buffers)
+    unsigned long long *Dp = Wx + (size_t)4 * PT_HT * dpad;
     unsigned seq = 0;
     unsigned long long iters = 0;
 
@@ -1225,9 +1233,31 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                         ld4_c(Eprev, ebytes, (unsigned)(((size_t)hrank[hl[i]] * dpad + 4 * c4) * sizeof(float)));
                 }
             }
-            if (!ps_wait(flags, PTF_DONE + 1 - team, 1, (unsigned)k, bail)) return;
-            if (tid < dim) dprev[tid] = ld_c(a.D + (int64_t)(k - 1) * dim + tid);
+            // d_{k-1}: team 1-t's pairs tagged k
+            if (tid < dim) {
+                const unsigned long long *pp = Dp + (size_t)(1 - team) * dpad + tid;
+                unsigned long long pv = ld64_c(pp);
+                long spins = 0;
+                while ((unsigned)(pv >> 32) != (unsigned)k) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if ((++spins & 15) == 0) {
+                        if (ld_flag(flags + PSF_ABORT)) {
+                            *bail = 1;
+                            break;
+                        }
+                        if (spins > PS_SPIN_MAX / 2) {
+                            st_flag(flags + PSF_STICKY, 1u);
+                            st_flag(flags + PSF_ABORT, 1u);
+                            *bail = 1;
+                            break;
+                        }
+                    }
+                    pv = ld64_c(pp);
+                }
+                dprev[tid] = __uint_as_float((unsigned)pv);
+            }
             __syncthreads();
+            if (*bail) return;
             float dv[3];
 #pragma unroll
             for (int j = 0; j < 3; j++) {
@@ -1264,36 +1294,41 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             }
             ps_publish(ecert + me, (unsigned)(k + 1));
         };
-        // issue: v_t = e_t . u_{i-1} for the wavefront's columns, its partial
-        // sum of e_t v_t; w_h = the PT_W partials in wavefront order, stored
-        // as pairs
+        // issue: wavefront w < PT_PW takes columns w, w + PT_PW, ...: v_t =
+        // e_t . u_{i-1} and its partial sum of e_t v_t; w_h = the PT_PW
+        // partials in wavefront order, stored as pairs
+        const bool piw = wave < PT_PW;
         auto issue = [&](unsigned s) {
             unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
-            float ev[PT_CPW][3];
+            if (piw) {
+                float pw[3] = {0.f, 0.f, 0.f};
+                for (int g0 = 0; wave + PT_PW * g0 < nc; g0 += 4) {
+                    float ev[4][3];
 #pragma unroll
-            for (int g = 0; g < PT_CPW; g++) {
-                const int t = min(wave + PT_W * g, max(nc - 1, 0));
+                    for (int b = 0; b < 4; b++) {
+                        const int t = min(wave + PT_PW * (g0 + b), nc - 1);
 #pragma unroll
-                for (int j = 0; j < 3; j++) ev[g][j] = Es[t * dpad + min(lane + 64 * j, dim - 1)];
-            }
-            float pw[3] = {0.f, 0.f, 0.f};
+                        for (int j = 0; j < 3; j++) ev[b][j] = Es[t * dpad + min(lane + 64 * j, dim - 1)];
+                    }
 #pragma unroll
-            for (int g = 0; g < PT_CPW; g++) {
-                const float v = ps_dot(ev[g], uj, dim);
-                if (wave + PT_W * g < nc)
+                    for (int b = 0; b < 4; b++) {
+                        const float v = ps_dot(ev[b], uj, dim);
+                        if (wave + PT_PW * (g0 + b) < nc)
 #pragma unroll
-                    for (int j = 0; j < 3; j++) pw[j] = fmaf(ev[g][j], v, pw[j]);
-            }
+                            for (int j = 0; j < 3; j++) pw[j] = fmaf(ev[b][j], v, pw[j]);
+                    }
+                }
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int e = lane + 64 * j;
-                if (e < dim) wpw[wave * dpad + e] = pw[j];
+                for (int j = 0; j < 3; j++) {
+                    const int e = lane + 64 * j;
+                    if (e < dim) wpw[wave * dpad + e] = pw[j];
+                }
             }
             __syncthreads();
             if (tid < dim) {
                 float w = 0.f;
 #pragma unroll
-                for (int q = 0; q < PT_W; q++) w += wpw[q * dpad + tid];
+                for (int q = 0; q < PT_PW; q++) w += wpw[q * dpad + tid];
                 st64_c(Wb + (size_t)h * dpad + tid, ((unsigned long long)s << 32) | __float_as_uint(w));
             }
         };
@@ -1352,31 +1387,42 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             }
             __syncthreads();
             if (*bail) return -1;
-            // every wavefront forms z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
-            float zj[3], n2 = 0.f;
+            // the PT_PW wavefronts form z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
+            // (each the same, bit for bit); wavefront 0 posts the decision
+            if (piw) {
+                float zj[3], n2 = 0.f;
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int e = lane + 64 * j;
-                float z = 0.f;
-                if (e < dim)
-                    for (int q = 0; q < ngrp; q++) z += wx[q * dpad + e];
-                zj[j] = z;
-                n2 = fmaf(z, z, n2);
-            }
-            n2 = wave_sum_dpp(n2);
-            if (n2 == 0.f) return 1;  // E = 0: keep u_{i-1} (d_k at the start; its coefficients become 0)
-            const float inv = 1.f / sqrtf(n2);
-            float d2 = 0.f;
+                for (int j = 0; j < 3; j++) {
+                    const int e = lane + 64 * j;
+                    float z = 0.f;
+                    if (e < dim)
+                        for (int q = 0; q < ngrp; q++) z += wx[q * dpad + e];
+                    zj[j] = z;
+                    n2 = fmaf(z, z, n2);
+                }
+                n2 = wave_sum_dpp(n2);
+                int dec;
+                if (n2 == 0.f) {
+                    dec = 1;  // E = 0: keep u_{i-1} (d_k at the start; its coefficients become 0)
+                } else {
+                    const float inv = 1.f / sqrtf(n2);
+                    float d2 = 0.f;
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const float un = zj[j] * inv;
-                const float du = un - uj[j];
-                d2 = fmaf(du, du, d2);
-                uj[j] = un;
+                    for (int j = 0; j < 3; j++) {
+                        const float un = zj[j] * inv;
+                        const float du = un - uj[j];
+                        d2 = fmaf(du, du, d2);
+                        uj[j] = un;
+                    }
+                    d2 = wave_sum_dpp(d2);
+                    dec = ((it + 1 >= 2 && d2 < a.tol2) || it + 1 >= a.maxit) ? 2 : 0;
+                }
+                if (tid == 0) *sdec = dec;
             }
-            d2 = wave_sum_dpp(d2);
-            it++;
-            return ((it >= 2 && d2 < a.tol2) || it >= a.maxit) ? 1 : 0;
+            __syncthreads();
+            const int dec = *sdec;
+            if (dec != 1) it++;
+            return dec != 0 ? 1 : 0;
         };
         if (m > 0) {
             issue(++seq);
@@ -1392,12 +1438,13 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             store_e();
         }
         // ---- d_k out ---------------------------------------------------------
-        __syncthreads();  // every wavefront has read wx
         if (wave == 0) {
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int e = lane + 64 * j;
                 if (e < dim) {
+                    if (h == 0)
+                        st64_c(Dp + (size_t)team * dpad + e, ((unsigned long long)(k + 1) << 32) | __float_as_uint(uj[j]));
                     dprev[e] = uj[j];
                     if (h == 0) a.D[(int64_t)k * dim + e] = uj[j];
                 }
@@ -1414,7 +1461,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                 dv[j] = e < dim ? dprev[e] : 0.f;
             }
 #pragma unroll
-            for (int g = 0; g < PT_CPW; g++) {
+            for (int g = 0; g < PS_CWMAX / PT_W; g++) {
                 const int tl = wave + PT_W * g;
                 if (tl >= nc) continue;
                 float ev[3];
@@ -1536,10 +1583,11 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
             PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 3>;
             KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)pt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned *flags = (unsigned *)ctx->ticket.ptr;
-            // the exchange pairs (two teams x two buffers of PT_HT x dpad) sit
-            // at the start of the partial-Gram buffer
+            // the exchange pairs (two teams x two buffers of PT_HT x dpad) and
+            // the d_k pairs (two teams x dpad) sit at the start of the
+            // partial-Gram buffer
             unsigned long long *wx = (unsigned long long *)a.Gpart;
-            const int nw = 2 * 2 * PT_HT * dpad;
+            const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;  // and each team's d_k pairs
             hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
                                ctx->stream, flags, wx, nw);
             // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on XCD 0
