@@ -432,38 +432,58 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 }
 
 // Learned form schedule: the first iteration on a dictionary runs OMP on
-// every patch and trains the proposer on those pairs (PROP_STEPS0 steps);
-// later iterations propose, run OMP on the rejected patches, and train on
-// them (up to PROP_STEPS1 steps).  A proposal pass costs about as much as an
-// OMP pass, so when a pass accepts less than PROP_MIN_ACCEPT of the patches
-// the proposer stands down for the rest of this dictionary's iterations and
-// plain OMP runs instead.
-constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4;
+// every patch and trains the proposer on those pairs: PROP_PROBE_STEPS steps,
+// then a probe (proposals for the first MLP_CHUNK patches, OMP for the ones
+// rejected), then the rest of its first-iteration steps if the probe accepted
+// at least PROP_MIN_ACCEPT of its patches.  Later iterations propose, run OMP
+// on the rejected patches, and train on them (up to PROP_STEPS1 steps).  A
+// proposal pass costs about as much as an OMP pass, so when the probe or a
+// pass accepts less than PROP_MIN_ACCEPT of its patches the proposer stands
+// down for the rest of this dictionary's iterations and OMP runs instead.
+constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4, PROP_PROBE_STEPS = 8;
 constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
 constexpr uint64_t PROP_SEED = 0x5EEDull;
+
+static int proposer_pass(ksvd_ctx *ctx, int64_t n);
 
 static int proposer_step(ksvd_ctx *ctx)
 {
     int rc;
     if (ctx->mlp_gated) return ksvd_omp(ctx);
+    // true when the last pass over n patches accepted at least PROP_MIN_ACCEPT of them
+    auto accepted_enough = [&](int64_t n, bool *ok) -> int {
+        unsigned nfb = 0;
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&nfb, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        *ok = (double)(n - (int64_t)nfb) >= PROP_MIN_ACCEPT * (double)n;
+        return KSVD_OK;
+    };
+    bool ok = false;
     if (!ctx->mlp_trained) {
         if ((rc = ksvd_omp(ctx))) return rc;
         if (ctx->P < MLP_BATCH) return KSVD_OK;  // too few patches to train on: plain OMP
         if (!ctx->mlp_ready && (rc = mlp_init(ctx, PROP_SEED))) return rc;
-        int steps = (int)std::min<int64_t>(PROP_MAX_STEPS0, std::max<int64_t>(PROP_MIN_STEPS0,
-                                                                              PROP_EPOCHS0 * ctx->P / MLP_BATCH));
+        const int steps = (int)std::min<int64_t>(PROP_MAX_STEPS0, std::max<int64_t>(PROP_MIN_STEPS0,
+                                                                                    PROP_EPOCHS0 * ctx->P / MLP_BATCH));
         Span s;
         if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
-        if ((rc = mlp_train_all(ctx, steps, PROP_LR, PROP_SEED + ctx->iters_done))) return rc;
+        if ((rc = mlp_train_all(ctx, PROP_PROBE_STEPS, PROP_LR, PROP_SEED + ctx->iters_done))) return rc;
         if ((rc = span_end(ctx, &s))) return rc;
         ctx->mlp_trained = true;
-        return KSVD_OK;
+        const int64_t n = std::min<int64_t>(ctx->P, MLP_CHUNK);
+        if ((rc = proposer_pass(ctx, n))) return rc;
+        if ((rc = accepted_enough(n, &ok))) return rc;
+        if (!ok) {
+            ctx->mlp_gated = true;
+            return KSVD_OK;
+        }
+        if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
+        if ((rc = mlp_train_all(ctx, steps - PROP_PROBE_STEPS, PROP_LR, PROP_SEED + 1 + ctx->iters_done))) return rc;
+        return span_end(ctx, &s);
     }
     if ((rc = ksvd_proposer_omp(ctx))) return rc;
-    unsigned nfb = 0;
-    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&nfb, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((double)(ctx->P - (int64_t)nfb) < PROP_MIN_ACCEPT * (double)ctx->P) {
+    if ((rc = accepted_enough(ctx->P, &ok))) return rc;
+    if (!ok) {
         ctx->mlp_gated = true;
         return KSVD_OK;
     }
@@ -708,6 +728,25 @@ int ksvd_proposer_train(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
     return KSVD_OK;
 }
 
+// Proposals for patches [0, n), OMP for the ones rejected (the others keep
+// their codes).
+static int proposer_pass(ksvd_ctx *ctx, int64_t n)
+{
+    int rc;
+    Span outer;
+    if ((rc = span_begin(ctx, &outer, SPAN_OMP))) return rc;
+    if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
+    if ((rc = mlp_propose(ctx, PROP_ACCEPT, n))) return rc;
+    if ((rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr))) return rc;
+    hipLaunchKernelGGL(proposer_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (const unsigned *)ctx->mlp_cnt.ptr,
+                       n, (unsigned long long *)ctx->counters.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    if ((rc = span_end(ctx, &outer))) return rc;
+    ctx->has_codes = true;
+    ctx->has_rebuild = false;
+    return KSVD_OK;
+}
+
 int ksvd_proposer_omp(ksvd_ctx *ctx)
 {
     int rc = proposer_ok(ctx);
@@ -715,18 +754,7 @@ int ksvd_proposer_omp(ksvd_ctx *ctx)
     if (!ctx->has_patches || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches and dictionary required");
     if (!ctx->mlp_ready) return ksvd_fail(ctx, KSVD_ERR_STATE, "proposer not initialised");
     if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "a first OMP (the residual bounds) is required");
-    Span outer;
-    if ((rc = span_begin(ctx, &outer, SPAN_OMP))) return rc;
-    if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
-    if ((rc = mlp_propose(ctx, PROP_ACCEPT))) return rc;
-    if ((rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr))) return rc;
-    hipLaunchKernelGGL(proposer_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (const unsigned *)ctx->mlp_cnt.ptr,
-                       ctx->P, (unsigned long long *)ctx->counters.ptr);
-    KSVD_HIP_CHECK(ctx, hipGetLastError());
-    if ((rc = span_end(ctx, &outer))) return rc;
-    ctx->has_codes = true;
-    ctx->has_rebuild = false;
-    return KSVD_OK;
+    return proposer_pass(ctx, ctx->P);
 }
 
 int ksvd_proposer_loss_grad(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad)

@@ -919,13 +919,13 @@ int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr)
 // Proposals for every patch, in chunks whose activations stay in the
 // last-level cache; rejected patches are listed in mlp_fb (count in
 // mlp_cnt[0]).  The caller then runs OMP over the list.
-int mlp_propose(ksvd_ctx *ctx, float accept)
+int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n)
 {
     int kin, ko;
     mlp_dims(ctx, &kin, &ko);
     int rc = mlp_inputs(ctx);
     if (rc) return rc;
-    const int64_t P = ctx->P;
+    const int64_t P = std::min<int64_t>(n, ctx->P);
     const int64_t CH = std::min<int64_t>(MLP_CHUNK, (P + 127) / 128 * 128);
     if ((rc = ksvd_reserve(ctx, ctx->mlp_h1c, sizeof(__bf16) * CH * MLP_HP))) return rc;
     if ((rc = ksvd_reserve(ctx, ctx->mlp_h2c, sizeof(__bf16) * CH * MLP_HP))) return rc;

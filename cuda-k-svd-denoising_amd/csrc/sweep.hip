@@ -1035,7 +1035,6 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 constexpr int PT_T = 1024;               // threads per helper workgroup
 constexpr int PT_W = PT_T / 64;          // wavefronts per helper
 constexpr int PT_PW = 8;                 // wavefronts that run the power iteration's arithmetic
-constexpr int PT_CPW = PS_CWMAX / PT_PW; // columns per such wavefront at most
 constexpr int PT_HT = 16;                // helpers per team (the two teams fill XCD 0's 32 CUs)
 constexpr int PT_GRP = 8;                // partial sums per element of the exchange
 constexpr int PT_XH = 4;                 // exchange pairs read per thread (PT_HT over >= 5 groups)

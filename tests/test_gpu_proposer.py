@@ -9,8 +9,8 @@
     1.05 x OMP-residual test more often than the untrained network's (what
     the training buys); every code, accepted proposal or OMP fallback, meets
     that bound;
-  * end to end, BF16 + learned: PSNR within 0.05 dB of the plain FP32 form,
-    fallback rate reported.
+  * end to end, BF16 + learned: PSNR no more than 0.05 dB below the plain FP32
+    form, fallback rate reported.
 """
 import numpy as np
 import pytest
@@ -122,5 +122,8 @@ def test_bf16_learned_end_to_end_psnr(kl):
     print(f"\nPSNR raw 512^2: plain FP32 {r32:.4f}, BF16 + learned {rl:.4f} (delta {rl - r32:+.4f} dB); "
           f"fallback rate {fb * 100:.2f}% of {t['proposed']} proposals; training {t['train_ms']:.2f} ms; "
           f"proposer stood down: {bool(t['proposer_gated'])}")
-    assert abs(rl - r32) <= 0.05
+    # the bar bounds the quality lost against the plain form; accepted
+    # proposals (residual within 1.05x of OMP's) change the codes and so the
+    # dictionary's path, which can land above the plain form's PSNR as well
+    assert rl >= r32 - 0.05
     assert t["proposed"] > 0
