@@ -1145,13 +1145,69 @@ __device__ void pa_build(const SweepArgs &a, int nc, float *Es, const int *sp, c
     }
 }
 
+// The patches of the table's columns into Es (pitch dpad): thread (tc, te)
+// loads element te of YB columns at once.
+template <int YB>
+__device__ void pa_load_y(const SweepArgs &a, int nc, float *Es, const int *sp)
+{
+    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
+    const int cpt = PT_T / dim;
+    const int tc = tid / dim, te = tid - tc * dim;
+    if (tc >= cpt || nc <= 0) return;
+    for (int t = tc; t < nc; t += YB * cpt) {
+        float v[YB];
+#pragma unroll
+        for (int b = 0; b < YB; b++) v[b] = a.Y[(int64_t)sp[min(t + b * cpt, nc - 1)] * dim + te];
+#pragma unroll
+        for (int b = 0; b < YB; b++)
+            if (t + b * cpt < nc) Es[(t + b * cpt) * dpad + te] = v[b];
+    }
+}
+
+// The other atoms' shares d_j x_j (x_j read at sci) subtracted from the
+// columns in Es, entry order, element-parallel with BU columns' loads in
+// flight together (an empty entry reads atom 0 and coefficient 0 and is then
+// dropped).  With pa_load_y before it, the same columns as pa_build.
+template <int NO, int BU>
+__device__ void pa_subtract(const SweepArgs &a, int nc, float *Es, const int *sat, const int *sci)
+{
+    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
+    const int cpt = PT_T / dim;
+    const int tc = tid / dim, te = tid - tc * dim;
+    if (tc >= cpt) return;
+    for (int t = tc; t < nc; t += BU * cpt) {
+        float dv[BU][NO], xv[BU][NO];
+#pragma unroll
+        for (int b = 0; b < BU; b++) {
+            const int tt = min(t + b * cpt, nc - 1);
+#pragma unroll
+            for (int i = 0; i < NO; i++) {
+                dv[b][i] = ld_c(a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te);
+                xv[b][i] = ld_c(a.coef + max(sci[tt * PS_NOMAX + i], 0));
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < BU; b++) {
+            const int tt = t + b * cpt;
+            if (tt < nc) {
+                float v = Es[tt * dpad + te];
+#pragma unroll
+                for (int i = 0; i < NO; i++)
+                    if (sci[tt * PS_NOMAX + i] >= 0) v = fmaf(-dv[b][i], xv[b][i], v);
+                Es[tt * dpad + te] = v;
+            }
+        }
+    }
+}
+
 // After the sweep: coefficient 0 for every user of atom k beyond its first
-// min(c_k, cap) (one workgroup per atom).
+// min(c_k, cap) (workgroups (k, 0 .. gridDim.y) share atom k's users).
 __global__ void pa_zero_kernel(SweepArgs a)
 {
     const int k = blockIdx.x;
     const int base = a.uoff[k], c = a.uoff[k + 1] - base;
-    for (int t = a.cap + threadIdx.x; t < c; t += blockDim.x) {
+    const int step = gridDim.y * blockDim.x;
+    for (int t = a.cap + blockIdx.y * blockDim.x + threadIdx.x; t < c; t += step) {
         const int pos = base + t;
         a.coef[(int64_t)a.users[pos] * a.kmax + a.uslot[pos]] = 0.f;
     }
@@ -1346,32 +1402,27 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     const int hh = gq + ngrp * j;
                     pv[j] = hh < H ? ld64_c(Wb + (size_t)hh * dpad + ge) : ((unsigned long long)s << 32);
                 }
-                // a thread still missing pairs polls its first missing one
-                // alone, then rereads the rest
+                // a thread still missing pairs rereads all its missing ones
+                // together (a few 8-byte loads per round)
                 long spins = 0;
                 for (;;) {
-                    int miss = -1;
+                    bool ok = true;
 #pragma unroll
-                    for (int j = PT_XH - 1; j >= 0; j--)
-                        if ((unsigned)(pv[j] >> 32) != s) miss = j;
-                    if (miss < 0) break;
-                    const unsigned long long *pm = Wb + (size_t)(gq + ngrp * miss) * dpad + ge;
-                    while ((unsigned)(ld64_c(pm) >> 32) != s) {
-                        __builtin_amdgcn_s_sleep(2);
-                        if ((++spins & 15) == 0) {
-                            if (ld_flag(flags + PSF_ABORT)) {
-                                *bail = 1;
-                                break;
-                            }
-                            if (spins > PS_SPIN_MAX / 2) {
-                                st_flag(flags + PSF_STICKY, 1u);
-                                st_flag(flags + PSF_ABORT, 1u);
-                                *bail = 1;
-                                break;
-                            }
+                    for (int j = 0; j < PT_XH; j++) ok = ok && (unsigned)(pv[j] >> 32) == s;
+                    if (ok) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 31) == 0) {
+                        if (ld_flag(flags + PSF_ABORT)) {
+                            *bail = 1;
+                            break;
+                        }
+                        if (spins > PS_SPIN_MAX) {
+                            st_flag(flags + PSF_STICKY, 1u);
+                            st_flag(flags + PSF_ABORT, 1u);
+                            *bail = 1;
+                            break;
                         }
                     }
-                    if (*bail) break;
 #pragma unroll
                     for (int j = 0; j < PT_XH; j++) {
                         const int hh = gq + ngrp * j;
@@ -1472,11 +1523,10 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         }
         ps_publish(ccert + me, (unsigned)(k + 1));
         // ---- P1: atom k+2's columns without atom k+1 ------------------------------
+        // (the table, the patches and the atom's current vector first: none
+        // changes during the sweep before atom k+2's own update)
         const int kn = k + 2;
         if (kn >= K) break;
-        if (!ps_wait(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), bail)) return;
-        if (k >= 1 && !ps_wait(flags, PTF_CCERT + other, H, (unsigned)k, bail)) return;
-        if (!ps_wait(flags, PTF_DONE + team, 1, (unsigned)(k + 1), bail)) return;
         if (tid == 0) *nhp = 0;
         __syncthreads();
         int tbn;
@@ -1484,7 +1534,11 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp);
         if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
         __syncthreads();
-        pa_build<NO, BU>(a, ncn, Es, sp, sat, sci);
+        pa_load_y<10>(a, ncn, Es, sp);
+        if (!ps_wait(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), bail)) return;
+        if (k >= 1 && !ps_wait(flags, PTF_CCERT + other, H, (unsigned)k, bail)) return;
+        if (!ps_wait(flags, PTF_DONE + team, 1, (unsigned)(k + 1), bail)) return;
+        pa_subtract<NO, BU>(a, ncn, Es, sat, sci);
         // the columns of E_{k+1} that atom k+2's hand-over needs
         if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
         {
@@ -1579,7 +1633,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                            sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
-            PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 3>;
+            PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 4>;
             KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)pt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned *flags = (unsigned *)ctx->ticket.ptr;
             // the exchange pairs (two teams x two buffers of PT_HT x dpad) and
@@ -1591,7 +1645,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                                ctx->stream, flags, wx, nw);
             // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on XCD 0
             hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
-            hipLaunchKernelGGL(pa_zero_kernel, dim3(K), dim3(256), 0, ctx->stream, a);
+            hipLaunchKernelGGL(pa_zero_kernel, dim3(K, 16), dim3(256), 0, ctx->stream, a);
             KSVD_HIP_CHECK(ctx, hipGetLastError());
             return KSVD_OK;
         }
