@@ -259,6 +259,8 @@ int ksvd_destroy(ksvd_ctx *ctx)
     }
     drop_timeline(ctx);
     drop_update_graph(ctx);
+    if (ctx->gate_ev) hipEventDestroy((hipEvent_t)ctx->gate_ev);
+    if (ctx->gate_host) hipHostFree(ctx->gate_host);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
@@ -286,6 +288,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     if (rc) return rc;
     rc = resolve(ctx);
     if (rc) return rc;
+    const int64_t oldP = ctx->P;
     ctx->m = m;
     ctx->n = n;
     ctx->Pr = (n - p.patch_h) / p.slide_h + 1;
@@ -329,15 +332,24 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
         {&ctx->rpart, sizeof(double) * (size_t)((N + 255) / 256)},
         {&ctx->rbound, sizeof(float) * (size_t)P},
     };
-    drop_update_graph(ctx);  // buffers may move
+    // the update's graph holds these buffers' addresses and P: it is kept
+    // only when neither changes
+    bool same = ctx->P == oldP;
     for (auto &e : plan) {
+        void *before = e.b->ptr;
         rc = ksvd_reserve(ctx, *e.b, e.bytes);
-        if (rc) return rc;
+        if (rc) {
+            drop_update_graph(ctx);
+            return rc;
+        }
+        same = same && e.b->ptr == before;
     }
+    if (!same) drop_update_graph(ctx);
     // flags of the persistent sweep (its reset kernel keeps the sticky abort word)
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     KSVD_HIP_CHECK(ctx, hipMemset(ctx->ticket.ptr, 0, ctx->ticket.bytes));
     KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
+    if ((rc = prepare_update_graph(ctx))) return rc;
     ctx->has_image = true;
     ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
     ctx->mlp_x_valid = false;
@@ -386,6 +398,7 @@ int ksvd_init_dictionary(ksvd_ctx *ctx)
     ctx->gram_valid = false;
     ctx->mlp_trained = false;  // a new dictionary lineage: its atoms are not the ones the proposer learned
     ctx->mlp_gated = false;
+    ctx->gate_pending = false;
     ctx->iters_done = 0;
     return KSVD_OK;
 }
@@ -432,62 +445,75 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 }
 
 // Learned form schedule: the first iteration on a dictionary runs OMP on
-// every patch and trains the proposer on those pairs: PROP_PROBE_STEPS steps,
-// then a probe (proposals for the first MLP_CHUNK patches, OMP for the ones
-// rejected), then the rest of its first-iteration steps if the probe accepted
-// at least PROP_MIN_ACCEPT of its patches.  Later iterations propose, run OMP
-// on the rejected patches, and train on them (up to PROP_STEPS1 steps).  A
-// proposal pass costs about as much as an OMP pass, so when the probe or a
-// pass accepts less than PROP_MIN_ACCEPT of its patches the proposer stands
-// down for the rest of this dictionary's iterations and OMP runs instead.
+// every patch and trains the proposer on those pairs for PROP_PROBE_STEPS
+// steps, then probes (proposals for the first MLP_CHUNK patches, OMP for the
+// ones rejected).  Later iterations propose, run OMP on the rejected patches,
+// and train on them (up to PROP_STEPS1 steps); the one after a probe that
+// passed first trains the rest of the first-iteration steps.  A proposal pass
+// costs about as much as an OMP pass, so when the probe or a pass accepts
+// less than PROP_MIN_ACCEPT of its patches the proposer stands down for the
+// rest of this dictionary's iterations and OMP runs instead.  Each pass's
+// fallback count reaches the host behind an event and is read at the next
+// iteration, so no iteration waits for the GPU.
 constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4, PROP_PROBE_STEPS = 8;
 constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
 constexpr uint64_t PROP_SEED = 0x5EEDull;
 
 static int proposer_pass(ksvd_ctx *ctx, int64_t n);
 
+// Queues the copy of the last pass's fallback count (n patches) to the host.
+static int post_gate(ksvd_ctx *ctx, int64_t n, bool probe)
+{
+    if (!ctx->gate_host) KSVD_HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->gate_host, sizeof(unsigned), hipHostMallocDefault));
+    if (!ctx->gate_ev) {
+        hipEvent_t e;
+        KSVD_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->gate_ev = e;
+    }
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->gate_host, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    KSVD_HIP_CHECK(ctx, hipEventRecord((hipEvent_t)ctx->gate_ev, ctx->stream));
+    ctx->gate_pending = true;
+    ctx->gate_n = n;
+    ctx->gate_probe = probe;
+    return KSVD_OK;
+}
+
 static int proposer_step(ksvd_ctx *ctx)
 {
     int rc;
+    Span s;
+    const int steps0 = (int)std::min<int64_t>(PROP_MAX_STEPS0, std::max<int64_t>(PROP_MIN_STEPS0,
+                                                                                PROP_EPOCHS0 * ctx->P / MLP_BATCH));
+    if (ctx->gate_pending) {
+        // the last pass's count (its copy was queued before at least one
+        // dictionary sweep, so this normally does not wait)
+        KSVD_HIP_CHECK(ctx, hipEventSynchronize((hipEvent_t)ctx->gate_ev));
+        ctx->gate_pending = false;
+        const int64_t n = ctx->gate_n;
+        if ((double)(n - (int64_t)*ctx->gate_host) < PROP_MIN_ACCEPT * (double)n) {
+            ctx->mlp_gated = true;
+        } else if (ctx->gate_probe) {
+            if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
+            if ((rc = mlp_train_all(ctx, steps0 - PROP_PROBE_STEPS, PROP_LR, PROP_SEED + 1 + ctx->iters_done))) return rc;
+            if ((rc = span_end(ctx, &s))) return rc;
+        }
+    }
     if (ctx->mlp_gated) return ksvd_omp(ctx);
-    // true when the last pass over n patches accepted at least PROP_MIN_ACCEPT of them
-    auto accepted_enough = [&](int64_t n, bool *ok) -> int {
-        unsigned nfb = 0;
-        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&nfb, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-        KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        *ok = (double)(n - (int64_t)nfb) >= PROP_MIN_ACCEPT * (double)n;
-        return KSVD_OK;
-    };
-    bool ok = false;
     if (!ctx->mlp_trained) {
         if ((rc = ksvd_omp(ctx))) return rc;
         if (ctx->P < MLP_BATCH) return KSVD_OK;  // too few patches to train on: plain OMP
         if (!ctx->mlp_ready && (rc = mlp_init(ctx, PROP_SEED))) return rc;
-        const int steps = (int)std::min<int64_t>(PROP_MAX_STEPS0, std::max<int64_t>(PROP_MIN_STEPS0,
-                                                                                    PROP_EPOCHS0 * ctx->P / MLP_BATCH));
-        Span s;
         if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
         if ((rc = mlp_train_all(ctx, PROP_PROBE_STEPS, PROP_LR, PROP_SEED + ctx->iters_done))) return rc;
         if ((rc = span_end(ctx, &s))) return rc;
         ctx->mlp_trained = true;
         const int64_t n = std::min<int64_t>(ctx->P, MLP_CHUNK);
         if ((rc = proposer_pass(ctx, n))) return rc;
-        if ((rc = accepted_enough(n, &ok))) return rc;
-        if (!ok) {
-            ctx->mlp_gated = true;
-            return KSVD_OK;
-        }
-        if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
-        if ((rc = mlp_train_all(ctx, steps - PROP_PROBE_STEPS, PROP_LR, PROP_SEED + 1 + ctx->iters_done))) return rc;
-        return span_end(ctx, &s);
+        return post_gate(ctx, n, true);
     }
     if ((rc = ksvd_proposer_omp(ctx))) return rc;
-    if ((rc = accepted_enough(ctx->P, &ok))) return rc;
-    if (!ok) {
-        ctx->mlp_gated = true;
-        return KSVD_OK;
-    }
-    Span s;
+    if ((rc = post_gate(ctx, ctx->P, false))) return rc;
     if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
     if ((rc = mlp_train_fallback(ctx, PROP_STEPS1, PROP_LR))) return rc;
     return span_end(ctx, &s);
@@ -600,6 +626,7 @@ int ksvd_set_dictionary(ksvd_ctx *ctx, const float *D)
     ctx->has_rebuild = false;
     ctx->mlp_trained = false;
     ctx->mlp_gated = false;
+    ctx->gate_pending = false;
     ctx->iters_done = 0;
     return KSVD_OK;
 }
@@ -693,6 +720,7 @@ int ksvd_proposer_init(ksvd_ctx *ctx, uint64_t seed)
     int rc = proposer_ok(ctx);
     if (rc) return rc;
     ctx->mlp_trained = false;
+    ctx->gate_pending = false;
     return mlp_init(ctx, seed);
 }
 

@@ -73,6 +73,12 @@ struct ksvd_ctx {
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
+    // the last proposal pass's fallback count, copied to pinned host memory
+    // behind gate_ev and read at the next learned-form iteration
+    unsigned *gate_host = nullptr;
+    void *gate_ev = nullptr;  // hipEvent_t
+    bool gate_pending = false, gate_probe = false;
+    int64_t gate_n = 0;
     int64_t mlp_chunk_rows = 0;
     int64_t mlp_step = 0;
     int64_t iters_done = 0;  // K-SVD iterations run on this image (learned form schedule)
@@ -118,6 +124,7 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n);  // patches [0, n)
 size_t bf16_dict_bytes(int dim);
 size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);
+int prepare_update_graph(ksvd_ctx *ctx);  // capture the update's launches for the current buffers
 void drop_update_graph(ksvd_ctx *ctx);
 size_t sweep_gpart_bytes(int dim);
 // Synchronises the stream; fails if a persistent sweep gave up waiting.

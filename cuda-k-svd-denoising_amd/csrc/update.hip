@@ -222,32 +222,43 @@ void drop_update_graph(ksvd_ctx *ctx)
     ctx->upd_graph = nullptr;
 }
 
-// The sweep's 2K + 8 launches are captured once per buffer layout into a HIP
-// graph and replayed (KSVD_NO_GRAPH=1 launches them one by one).
-int launch_update(ksvd_ctx *ctx)
+static bool graphs_off()
 {
     const char *ng = getenv("KSVD_NO_GRAPH");
-    const bool no_graph = ng && ng[0] && ng[0] != '0';
-    if (no_graph) {
+    return ng && ng[0] && ng[0] != '0';
+}
+
+// The update's launches depend only on the context's buffers and sizes, so
+// they are captured once per buffer layout into a HIP graph (KSVD_NO_GRAPH=1
+// launches them one by one).  ksvd_set_image prepares the graph right after
+// laying the buffers out, so that no K-SVD iteration pays the capture.
+int prepare_update_graph(ksvd_ctx *ctx)
+{
+    if (graphs_off() || ctx->upd_exec) return KSVD_OK;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    KSVD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_update(ctx);
+    hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return ksvd_fail(ctx, KSVD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    KSVD_HIP_CHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    ctx->upd_graph = g;
+    ctx->upd_exec = ge;
+    return KSVD_OK;
+}
+
+int launch_update(ksvd_ctx *ctx)
+{
+    if (graphs_off()) {
         int rc = enqueue_update(ctx);
         if (rc) return rc;
     } else {
-        if (!ctx->upd_exec) {
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
-            KSVD_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_update(ctx);
-            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (ec != hipSuccess)
-                return ksvd_fail(ctx, KSVD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
-            KSVD_HIP_CHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            ctx->upd_graph = g;
-            ctx->upd_exec = ge;
-        }
+        int rc = prepare_update_graph(ctx);
+        if (rc) return rc;
         KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
     }
     ctx->gram_valid = false;

@@ -296,3 +296,29 @@ def test_update_graph_replays_match_eager(kl, orc, monkeypatch):
         np.testing.assert_array_equal(ks.dictionary(), D_eager, err_msg=f"replay {rep}")
         np.testing.assert_array_equal(ks.codes()[1], c_eager, err_msg=f"replay {rep}")
     ks.close()
+
+
+def test_update_graph_follows_image_changes(kl):
+    """The update's graph is captured when an image is set and kept while the
+    buffers and the patch count stay; a second image of the same size reuses
+    it, one of another size gets a new one.  Each image's K-SVD must give the
+    bits of a fresh context on the same image."""
+    cfg = CFG1
+
+    def run(ks, noisy):
+        ks.set_picture(noisy)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        return ks.dictionary(), ks.codes()[1]
+
+    images = [kl.synth_image(s, w, h, 20.0)[1] for s, w, h in ((1, 128, 128), (4, 128, 128), (5, 96, 112))]
+    ks = kl.KSvd(patch=cfg["patch"], slide=cfg["slide"], atoms=cfg["atoms"], omp_iter=cfg["k"], ksvd_iter=3)
+    for i, noisy in enumerate(images):
+        d, c = run(ks, noisy)
+        fresh = kl.KSvd(patch=cfg["patch"], slide=cfg["slide"], atoms=cfg["atoms"], omp_iter=cfg["k"], ksvd_iter=3)
+        d0, c0 = run(fresh, noisy)
+        fresh.close()
+        np.testing.assert_array_equal(d, d0, err_msg=f"image {i}")
+        np.testing.assert_array_equal(c, c0, err_msg=f"image {i}")
+    ks.close()
