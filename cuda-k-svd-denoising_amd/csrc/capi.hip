@@ -446,8 +446,10 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 
 // Learned form schedule: the first iteration on a dictionary runs OMP on
 // every patch and trains the proposer on those pairs for PROP_PROBE_STEPS
-// steps, then probes (proposals for the first MLP_CHUNK patches, OMP for the
-// ones rejected).  Later iterations propose, run OMP on the rejected patches,
+// steps, then probes: proposals for the first MLP_CHUNK patches, counted
+// against the residual bound but not kept (the codes stay OMP's, so a
+// proposer that stands down leaves the BF16 form's codes).  Later iterations
+// propose, run OMP on the rejected patches,
 // and train on them (up to PROP_STEPS1 steps); the one after a probe that
 // passed first trains the rest of the first-iteration steps.  A proposal pass
 // costs about as much as an OMP pass, so when the probe or a pass accepts
@@ -459,7 +461,7 @@ constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP
 constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
 constexpr uint64_t PROP_SEED = 0x5EEDull;
 
-static int proposer_pass(ksvd_ctx *ctx, int64_t n);
+static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe);
 
 // Queues the copy of the last pass's fallback count (n patches) to the host.
 static int post_gate(ksvd_ctx *ctx, int64_t n, bool probe)
@@ -509,7 +511,7 @@ static int proposer_step(ksvd_ctx *ctx)
         if ((rc = span_end(ctx, &s))) return rc;
         ctx->mlp_trained = true;
         const int64_t n = std::min<int64_t>(ctx->P, MLP_CHUNK);
-        if ((rc = proposer_pass(ctx, n))) return rc;
+        if ((rc = proposer_pass(ctx, n, true))) return rc;
         return post_gate(ctx, n, true);
     }
     if ((rc = ksvd_proposer_omp(ctx))) return rc;
@@ -756,16 +758,18 @@ int ksvd_proposer_train(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
     return KSVD_OK;
 }
 
-// Proposals for patches [0, n), OMP for the ones rejected (the others keep
-// their codes).
-static int proposer_pass(ksvd_ctx *ctx, int64_t n)
+// Proposals for patches [0, n), OMP for the ones rejected.  A probe counts
+// the rejections only: the codes (OMP's, just computed) stay as they are.
+static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe)
 {
     int rc;
     Span outer;
     if ((rc = span_begin(ctx, &outer, SPAN_OMP))) return rc;
     if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
-    if ((rc = mlp_propose(ctx, PROP_ACCEPT, n))) return rc;
-    if ((rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr))) return rc;
+    if ((rc = mlp_propose(ctx, PROP_ACCEPT, n, !probe))) return rc;
+    if (!probe &&
+        (rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr)))
+        return rc;
     hipLaunchKernelGGL(proposer_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (const unsigned *)ctx->mlp_cnt.ptr,
                        n, (unsigned long long *)ctx->counters.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
@@ -782,7 +786,7 @@ int ksvd_proposer_omp(ksvd_ctx *ctx)
     if (!ctx->has_patches || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "patches and dictionary required");
     if (!ctx->mlp_ready) return ksvd_fail(ctx, KSVD_ERR_STATE, "proposer not initialised");
     if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "a first OMP (the residual bounds) is required");
-    return proposer_pass(ctx, ctx->P);
+    return proposer_pass(ctx, ctx->P, false);
 }
 
 int ksvd_proposer_loss_grad(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad)

@@ -120,7 +120,8 @@ int mlp_get_params(ksvd_ctx *ctx, float *theta);
 int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad);
 int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed);
 int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr);
-int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n);  // patches [0, n)
+// proposals for patches [0, n); write = false only counts the rejected ones
+int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write);
 size_t bf16_dict_bytes(int dim);
 size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);

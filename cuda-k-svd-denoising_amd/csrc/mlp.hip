@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__
                                                       const float *__restrict__ Y, int dim, int64_t p0, int64_t np,
                                                       const float *__restrict__ D, const float *__restrict__ G,
                                                       int kmax, const float *__restrict__ rbound, float accept,
-                                                      int *__restrict__ idx_out, float *__restrict__ coef_out,
+                                                      int write, int *__restrict__ idx_out, float *__restrict__ coef_out,
                                                       int *__restrict__ cnt_out, int *__restrict__ fb_list,
                                                       unsigned *__restrict__ fb_count)
 {
@@ -479,16 +479,18 @@ __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__
     const bool accept_it = ok && sqrtf(nr) <= accept * rbound[p];
     if (valid && sub == 0) {
         if (accept_it) {
-            cnt_out[p] = kmax;
+            if (write) {
+                cnt_out[p] = kmax;
 #pragma unroll
-            for (int i = 0; i < KM; i++)
-                if (i < kmax) {
-                    idx_out[p * kmax + i] = S[i];
-                    coef_out[p * kmax + i] = x[i];
-                }
+                for (int i = 0; i < KM; i++)
+                    if (i < kmax) {
+                        idx_out[p * kmax + i] = S[i];
+                        coef_out[p * kmax + i] = x[i];
+                    }
+            }
         } else {
             const unsigned slot = atomicAdd(fb_count, 1u);
-            fb_list[slot] = (int)p;
+            if (write) fb_list[slot] = (int)p;
         }
     }
 }
@@ -919,7 +921,7 @@ int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr)
 // Proposals for every patch, in chunks whose activations stay in the
 // last-level cache; rejected patches are listed in mlp_fb (count in
 // mlp_cnt[0]).  The caller then runs OMP over the list.
-int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n)
+int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write)
 {
     int kin, ko;
     mlp_dims(ctx, &kin, &ko);
@@ -959,7 +961,7 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n)
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, (const __bf16 *)Z, ko, ctx->K,
                                (const float *)ctx->Y.ptr, ctx->dim, p0, (int64_t)nb, (const float *)ctx->D.ptr,
                                (const float *)ctx->G.ptr, ctx->kmax, (const float *)ctx->rbound.ptr, accept,
-                               (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr, (int *)ctx->mlp_fb.ptr,
+                               write ? 1 : 0, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr, (int *)ctx->mlp_fb.ptr,
                                (unsigned *)ctx->mlp_cnt.ptr);
         };
         const int rj = (ctx->dim + 15) / 16;

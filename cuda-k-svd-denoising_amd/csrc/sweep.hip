@@ -1036,8 +1036,9 @@ constexpr int PT_T = 1024;               // threads per helper workgroup
 constexpr int PT_W = PT_T / 64;          // wavefronts per helper
 constexpr int PT_PW = 8;                 // wavefronts that run the power iteration's arithmetic
 constexpr int PT_HT = 16;                // helpers per team (the two teams fill XCD 0's 32 CUs)
-constexpr int PT_GRP = 8;                // partial sums per element of the exchange
-constexpr int PT_XH = 4;                 // exchange pairs read per thread (PT_HT over >= 5 groups)
+constexpr int PT_GRP = 4;                // groups reading the exchange, PT_HT / PT_GRP pairs each (4 groups: 3.0 ms per
+                                         // update at 2048x2048; 7 groups: 3.3 ms, 1 group: 5.1 ms)
+constexpr int PT_XH = PT_HT / PT_GRP;    // exchange pairs read per thread
 // flag words: per-XCD registration; per-helper ecert and ccert (index
 // t * PT_HT + h); per-team done; the abort word and its sticky copy
 constexpr int PTF_ECERT = 32, PTF_CCERT = 64, PTF_DONE = 96;
