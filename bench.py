@@ -58,7 +58,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (109901.6 * 2 + 161620.6) * 1024, "omp": (132554.5 * 2 + 19817.9) * 1024},
+    "2048": {"sweep": (109853.8 * 2 + 160905.9) * 1024, "omp": (131868.5 * 2 + 19817.2) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
