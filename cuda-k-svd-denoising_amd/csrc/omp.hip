@@ -6,9 +6,11 @@
 // for the kernel's lifetime.  Each wavefront owns a tile of 16 patches:
 //   * lane (col, g), col = lane % 16, g = lane / 16, holds quarter g of patch
 //     col: elements g*Q .. g*Q+Q-1 of y and of the residual r;
-//   * the correlation C (K x 16) = D^T R is v_mfma_f32_16x16x4_f32 with the
-//     reduction index permuted so that MFMA number kk takes element g*Q + kk
-//     from lane group g (A from LDS, B straight from r's registers);
+//   * step 0's correlation C0 (K x 16) = D^T Y is v_mfma_f32_16x16x4_f32
+//     with the reduction index permuted so that MFMA number kk takes element
+//     g*Q + kk from lane group g (A from LDS, B straight from y's registers);
+//     later steps take D^T r = C0 - G[:, S] x from the dictionary Gram
+//     (Batch-OMP), the rows of the support atoms read from L2;
 //   * |C| argmax per patch: in-lane scan in ascending atom order, then two
 //     cross-lane steps with the lowest index winning ties (cublasIsamax);
 //   * the support Gram grows by one Cholesky row per step (entries from the
@@ -72,23 +74,47 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
         int s = 0;
         bool active = valid;
 
+        f4 c0[KT];  // D^T y (step 0's correlation)
         for (int t = 0; t < kmax; t++) {
             if (__ballot(active) == 0) break;
-            // ---- correlation C = D^T R on MFMA --------------------------------
             f4 acc[KT];
+            if (t == 0) {
+                // ---- correlation C = D^T Y on MFMA ----------------------------
 #pragma unroll
-            for (int tt = 0; tt < KT; tt++) acc[tt] = f4{0.f, 0.f, 0.f, 0.f};
+                for (int tt = 0; tt < KT; tt++) acc[tt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kb = 0; kb < Q; kb += 4) {
-                f4 av[KT];
-#pragma unroll
-                for (int tt = 0; tt < KT; tt++)
-                    av[tt] = *(const f4 *)(Ds + (tt * 16 + col) * LD + e0 + kb);
-#pragma unroll
-                for (int sub = 0; sub < 4; sub++)
+                for (int kb = 0; kb < Q; kb += 4) {
+                    f4 av[KT];
 #pragma unroll
                     for (int tt = 0; tt < KT; tt++)
-                        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], r[kb + sub], acc[tt], 0, 0, 0);
+                        av[tt] = *(const f4 *)(Ds + (tt * 16 + col) * LD + e0 + kb);
+#pragma unroll
+                    for (int sub = 0; sub < 4; sub++)
+#pragma unroll
+                        for (int tt = 0; tt < KT; tt++)
+                            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], r[kb + sub], acc[tt], 0, 0, 0);
+                }
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++) c0[tt] = acc[tt];
+            } else {
+                // ---- correlation from the Gram: D^T r = D^T y - G[:, S] x ------
+                // (Batch-OMP; the lane's atoms tt*16 + 4g .. +3, rows S_q of
+                // the symmetric G, support entries in order)
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++) acc[tt] = c0[tt];
+#pragma unroll
+                for (int q = 0; q < KM; q++) {
+                    if (__ballot(q < s) == 0ull) break;
+                    const float xq = q < s ? x[q] : 0.f;
+                    const f4 *gr = (const f4 *)(G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g);
+                    f4 gv[KT];
+#pragma unroll
+                    for (int tt = 0; tt < KT; tt++) gv[tt] = gr[4 * tt];
+#pragma unroll
+                    for (int tt = 0; tt < KT; tt++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) acc[tt][i] = fmaf(-gv[tt][i], xq, acc[tt][i]);
+                }
             }
             // ---- first index of max |c| -------------------------------------
             float best = -1.f;
