@@ -25,7 +25,8 @@ algorithmic bytes are those of the reference's update for the capped users
 of every atom (re-read each user's patch and code, write its new
 coefficient); its duration is the library's HIP-event span of the update on
 its stream.  `roofline_omp` is the same record for the plain form's OMP
-kernel (FP32 MFMA).  `cpu_baseline` times the CPU oracle (FP32, OpenMP) on
+kernel (FP32 MFMA correlation in the first step, Gram updates after it),
+priced by the flops that kernel performs.  `cpu_baseline` times the CPU oracle (FP32, OpenMP) on
 one K-SVD iteration of the same workload on this host.
 """
 import argparse
@@ -58,7 +59,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (109853.8 * 2 + 160905.9) * 1024, "omp": (131868.5 * 2 + 19817.2) * 1024},
+    "2048": {"sweep": (109841.2 * 2 + 161096.8) * 1024, "omp": (131910.1 * 2 + 19811.5) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
@@ -66,6 +67,16 @@ def omp_flops_per_patch(n, K, k):
     sum over steps s of 2nK (correlation) + 2n(s-1) (Gram column) + 2n (rhs)
     + 2ns (approximation) + 3n (residual and norm) + 3s^2 (Cholesky)."""
     return sum(2 * n * K + 2 * n * (s - 1) + 2 * n + 2 * n * s + 3 * n + 3 * s * s for s in range(1, k + 1))
+
+
+def batch_omp_flops_per_patch(n, K, k):
+    """Flops of one patch's pursuit in the plain form's kernel (Batch-OMP):
+    the first step correlates D^T y (2nK); step s > 1 updates it from the
+    dictionary Gram, c = D^T y - G[:, S] x (2K(s-1)); every step forms
+    b_j = d_j . y (2n), the approximation and residual (2ns + 3n) and the
+    Cholesky row (3s^2).  The support Gram entries are read from G."""
+    return sum((2 * n * K if s == 1 else 2 * K * (s - 1)) + 2 * n + 2 * n * s + 3 * n + 3 * s * s
+               for s in range(1, k + 1))
 
 
 def update_flops_per_patch(n, k):
@@ -206,7 +217,7 @@ def main():
 
     # the plain form's OMP kernel on FP32 MFMA
     omp_avg_ms = tm["omp_kernel_ms"] / max(tm["omp_calls"], 1)
-    flops_launch = P * omp_flops_per_patch(n, wl["atoms"], kk)
+    flops_launch = P * batch_omp_flops_per_patch(n, wl["atoms"], kk)
     achieved = flops_launch / (omp_avg_ms * 1e-3) / 1e12
 
     # Columns swept per iteration: replay the plain form's (deterministic)
@@ -282,7 +293,8 @@ def main():
         b = res["bf16"]
         bt = b["tm"]
         bomp_ms = bt["omp_kernel_ms"] / max(bt["omp_calls"], 1)
-        bach = flops_launch / (bomp_ms * 1e-3) / 1e12
+        bflops = P * omp_flops_per_patch(n, wl["atoms"], kk)
+        bach = bflops / (bomp_ms * 1e-3) / 1e12
         rec["speedup_vs_plain"] = round(head["rate"] / plain["rate"], 4)
         rec["delta_psnr_db"] = round(head["psnr_raw"] - plain["psnr_raw"], 4)
         rec["images_per_s"] = round(head["ips"], 3)
