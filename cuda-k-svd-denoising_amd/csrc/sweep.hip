@@ -615,6 +615,20 @@ __device__ __forceinline__ f4 ld4_c(const float *base, unsigned bytes, unsigned 
     return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 16));
 }
 
+// Drops this CU's L1 lines, so that plain loads after it read what other
+// workgroups stored to the XCD's L2 before the flags this workgroup waited
+// for.  Wavefront 0 issues the invalidation; the barrier orders every
+// wavefront's later loads after it.  (The persistent sweep reads the atoms and
+// coefficients of the other atoms' shares this way: with L1-bypassing sc1
+// loads instead, the 2048 x 2048 sweep ran 20 % slower in about one context
+// in four, depending on where the dictionary and coefficient buffers sat in
+// memory.)
+__device__ __forceinline__ void l1_drop()
+{
+    if ((threadIdx.x >> 6) == 0) asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+}
+
 // This workgroup's global stores reach the XCD's L2, then *flag = v.
 __device__ __forceinline__ void ps_publish(unsigned *flag, unsigned v)
 {
@@ -1168,7 +1182,9 @@ __device__ void pa_load_y(const SweepArgs &a, int nc, float *Es, const int *sp)
 // The other atoms' shares d_j x_j (x_j read at sci) subtracted from the
 // columns in Es, entry order, element-parallel with BU columns' loads in
 // flight together (an empty entry reads atom 0 and coefficient 0 and is then
-// dropped).  With pa_load_y before it, the same columns as pa_build.
+// dropped).  With pa_load_y before it, the same columns as pa_build.  The
+// atoms and coefficients are read with plain loads: the caller drops its L1
+// lines (l1_drop) after the waits that certify them.
 template <int NO, int BU>
 __device__ void pa_subtract(const SweepArgs &a, int nc, float *Es, const int *sat, const int *sci)
 {
@@ -1183,8 +1199,8 @@ __device__ void pa_subtract(const SweepArgs &a, int nc, float *Es, const int *sa
             const int tt = min(t + b * cpt, nc - 1);
 #pragma unroll
             for (int i = 0; i < NO; i++) {
-                dv[b][i] = ld_c(a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te);
-                xv[b][i] = ld_c(a.coef + max(sci[tt * PS_NOMAX + i], 0));
+                dv[b][i] = a.D[(int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te];
+                xv[b][i] = a.coef[max(sci[tt * PS_NOMAX + i], 0)];
             }
         }
 #pragma unroll
@@ -1221,7 +1237,6 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int K = a.K, cap = a.cap, H = PT_HT;
     int dim = a.dim, dpad = a.dpad;
-    const unsigned ebytes = (unsigned)((size_t)cap * dpad * sizeof(float));
     // LDS layout (pitch dpad)
     float *Es = sm;                           // PS_CWMAX x dpad: this helper's columns of the team's atom
     float *Eh = Es + PS_CWMAX * dpad;         // PS_CWMAX x dpad: columns of E_{k-1} for the hand-over (hl order)
@@ -1282,11 +1297,11 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             if (!eh_ready) {
                 // (the first atom of team 1) the columns of E_{k-1}
                 if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)k, bail)) return;
+                l1_drop();
                 const int d4 = dpad / 4;
                 for (int q = tid; q < nh * d4; q += PT_T) {
                     const int i = q / d4, c4 = q - i * d4;
-                    *(f4 *)(Eh + i * dpad + 4 * c4) =
-                        ld4_c(Eprev, ebytes, (unsigned)(((size_t)hrank[hl[i]] * dpad + 4 * c4) * sizeof(float)));
+                    *(f4 *)(Eh + i * dpad + 4 * c4) = *(const f4 *)(Eprev + (size_t)hrank[hl[i]] * dpad + 4 * c4);
                 }
             }
             // d_{k-1}: team 1-t's pairs tagged k
@@ -1539,16 +1554,17 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         if (!ps_wait(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), bail)) return;
         if (k >= 1 && !ps_wait(flags, PTF_CCERT + other, H, (unsigned)k, bail)) return;
         if (!ps_wait(flags, PTF_DONE + team, 1, (unsigned)(k + 1), bail)) return;
+        l1_drop();
         pa_subtract<NO, BU>(a, ncn, Es, sat, sci);
         // the columns of E_{k+1} that atom k+2's hand-over needs
         if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
+        l1_drop();
         {
             const int nh = *nhp, d4 = dpad / 4;
             const float *Enx = (kn & 1) ? a.E0 : a.E1;  // E_{k+1}
             for (int q = tid; q < nh * d4; q += PT_T) {
                 const int i = q / d4, c4 = q - i * d4;
-                *(f4 *)(Eh + i * dpad + 4 * c4) =
-                    ld4_c(Enx, ebytes, (unsigned)(((size_t)hrank[hl[i]] * dpad + 4 * c4) * sizeof(float)));
+                *(f4 *)(Eh + i * dpad + 4 * c4) = *(const f4 *)(Enx + (size_t)hrank[hl[i]] * dpad + 4 * c4);
             }
         }
         eh_ready = true;
@@ -1631,7 +1647,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const int cap = a.cap;
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL")) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
-                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
+                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 2 + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
             PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 4>;
