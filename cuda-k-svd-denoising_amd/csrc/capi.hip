@@ -366,6 +366,24 @@ int ksvd_set_image_device(ksvd_ctx *ctx, const float *dbuf, int m, int n)
     return set_image_common(ctx, dbuf, m, n, hipMemcpyDeviceToDevice);
 }
 
+// learned form schedule constants (see proposer_step)
+constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4, PROP_PROBE_STEPS = 8;
+constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
+constexpr uint64_t PROP_SEED = 0x5EEDull;
+
+// The pinned word and the event through which a proposal pass's fallback
+// count reaches the host (created once per context).
+static int gate_setup(ksvd_ctx *ctx)
+{
+    if (!ctx->gate_host) KSVD_HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->gate_host, sizeof(unsigned), hipHostMallocDefault));
+    if (!ctx->gate_ev) {
+        hipEvent_t e;
+        KSVD_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->gate_ev = e;
+    }
+    return KSVD_OK;
+}
+
 int ksvd_extract_patches(ksvd_ctx *ctx)
 {
     if (!ctx) return KSVD_ERR_ARG;
@@ -379,6 +397,12 @@ int ksvd_extract_patches(ksvd_ctx *ctx)
     ctx->has_patches = true;
     ctx->has_codes = ctx->has_rebuild = false;
     ctx->mlp_x_valid = false;
+    // the learned form's network, inputs, buffers and first training order,
+    // and the pinned word its gate reads
+    if (ctx->prm.proposer && ctx->P >= MLP_BATCH) {
+        if ((rc = mlp_prepare(ctx, PROP_SEED, PROP_SEED))) return rc;
+        if ((rc = gate_setup(ctx))) return rc;
+    }
     return KSVD_OK;
 }
 
@@ -456,22 +480,17 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 // less than PROP_MIN_ACCEPT of its patches the proposer stands down for the
 // rest of this dictionary's iterations and OMP runs instead.  Each pass's
 // fallback count reaches the host behind an event and is read at the next
-// iteration, so no iteration waits for the GPU.
-constexpr int PROP_EPOCHS0 = 1, PROP_MIN_STEPS0 = 16, PROP_MAX_STEPS0 = 32, PROP_STEPS1 = 4, PROP_PROBE_STEPS = 8;
-constexpr float PROP_LR = 2e-3f, PROP_ACCEPT = 1.05f, PROP_MIN_ACCEPT = 0.5f;
-constexpr uint64_t PROP_SEED = 0x5EEDull;
+// iteration, so no iteration waits for the GPU.  The network, its inputs,
+// its buffers and the first training order are prepared with the patches
+// (ksvd_extract_patches), so that the first iteration does not pay for them.
 
 static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe);
 
 // Queues the copy of the last pass's fallback count (n patches) to the host.
 static int post_gate(ksvd_ctx *ctx, int64_t n, bool probe)
 {
-    if (!ctx->gate_host) KSVD_HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->gate_host, sizeof(unsigned), hipHostMallocDefault));
-    if (!ctx->gate_ev) {
-        hipEvent_t e;
-        KSVD_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        ctx->gate_ev = e;
-    }
+    int rc = gate_setup(ctx);
+    if (rc) return rc;
     KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->gate_host, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost,
                                        ctx->stream));
     KSVD_HIP_CHECK(ctx, hipEventRecord((hipEvent_t)ctx->gate_ev, ctx->stream));

@@ -70,6 +70,8 @@ struct ksvd_ctx {
     DevBuf mlp_xb, mlp_h1b, mlp_h2b, mlp_zb, mlp_dz, mlp_dh2, mlp_dh1, mlp_lpart;  // one training batch
     DevBuf mlp_tr;  // transposed operands of the backward products
     DevBuf mlp_x, mlp_perm;                                  // normalised patches (all), training order
+    uint64_t mlp_perm_seed = 0;   // mlp_perm holds the seeded permutation of mlp_perm_n patches
+    int64_t mlp_perm_n = -1;
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
@@ -119,6 +121,7 @@ int mlp_set_params(ksvd_ctx *ctx, const float *theta);
 int mlp_get_params(ksvd_ctx *ctx, float *theta);
 int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, double *loss, float *grad);
 int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed);
+int mlp_prepare(ksvd_ctx *ctx, uint64_t init_seed, uint64_t perm_seed);
 int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr);
 // proposals for patches [0, n); write = false only counts the rejected ones
 int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write);

@@ -887,11 +887,11 @@ static int mlp_train_list(ksvd_ctx *ctx, const int *perm, int64_t n, int steps, 
 
 // A seeded permutation of all patches (host Fisher-Yates), for training on
 // the current codes.
-int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
+// the training order of seed `seed` into mlp_perm, unless it is there already
+static int mlp_permutation(ksvd_ctx *ctx, uint64_t seed)
 {
-    int rc = mlp_alloc(ctx);
-    if (rc) return rc;
     const int64_t P = ctx->P;
+    if (ctx->mlp_perm_n == P && ctx->mlp_perm_seed == seed) return KSVD_OK;
     std::vector<int> perm((size_t)P);
     for (int64_t i = 0; i < P; i++) perm[i] = (int)i;
     uint64_t s = seed;
@@ -899,10 +899,43 @@ int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
         const int64_t j = (int64_t)(sm64(s) % (uint64_t)(i + 1));
         std::swap(perm[i], perm[j]);
     }
-    if ((rc = ksvd_reserve(ctx, ctx->mlp_perm, sizeof(int) * (size_t)P))) return rc;
+    ctx->mlp_perm_n = -1;
+    int rc = ksvd_reserve(ctx, ctx->mlp_perm, sizeof(int) * (size_t)P);
+    if (rc) return rc;
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->mlp_perm.ptr, perm.data(), sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
-    return mlp_train_list(ctx, (const int *)ctx->mlp_perm.ptr, P, steps, lr);
+    ctx->mlp_perm_n = P;
+    ctx->mlp_perm_seed = seed;
+    return KSVD_OK;
+}
+
+int mlp_train_all(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
+{
+    int rc = mlp_alloc(ctx);
+    if (rc) return rc;
+    if ((rc = mlp_permutation(ctx, seed))) return rc;
+    return mlp_train_list(ctx, (const int *)ctx->mlp_perm.ptr, ctx->P, steps, lr);
+}
+
+// Everything the learned form needs for an image that does not depend on the
+// codes: the network's buffers and initial weights (seed init_seed, once per
+// context), the normalised patches, the proposal passes' buffers and the
+// first training order (seed perm_seed).  Run when the patches are
+// extracted, so that the K-SVD iterations neither allocate nor wait for it.
+int mlp_prepare(ksvd_ctx *ctx, uint64_t init_seed, uint64_t perm_seed)
+{
+    int rc = 0;
+    if (!ctx->mlp_ready && (rc = mlp_init(ctx, init_seed))) return rc;
+    if ((rc = mlp_inputs(ctx))) return rc;
+    // the proposal passes' chunk buffers and fallback list, at their largest
+    int kin, ko;
+    mlp_dims(ctx, &kin, &ko);
+    const int64_t CH = std::min<int64_t>(MLP_CHUNK, (ctx->P + 127) / 128 * 128);
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_h1c, sizeof(__bf16) * CH * MLP_HP))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_h2c, sizeof(__bf16) * CH * MLP_HP))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_zc, sizeof(__bf16) * CH * ko))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_fb, sizeof(int) * (size_t)ctx->P))) return rc;
+    return mlp_permutation(ctx, perm_seed);
 }
 
 // Continual training on the last fallback list (patches whose codes plain
