@@ -396,10 +396,10 @@ __global__ __launch_bounds__(256) void omp_generic_kernel(
     if (ndeg) atomicAdd(&counters[CNT_DEGEN], ndeg);
 }
 
-template <int Q>
+template <int Q, int KM>
 int launch_mfma(ksvd_ctx *ctx)
 {
-    constexpr int KT = 16, KM = 8;
+    constexpr int KT = 16;
     constexpr int LD = 4 * Q + 4;
     const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
     auto kern = omp_mfma_kernel<Q, KT, KM>;
@@ -450,7 +450,12 @@ int launch_omp(ksvd_ctx *ctx)
         if (rc) return rc;
     }
     if (ctx->prm.precision == KSVD_PREC_BF16) return launch_omp_bf16(ctx);
-    if (mfma_ok(ctx)) return ctx->dim <= 64 ? launch_mfma<16>(ctx) : launch_mfma<36>(ctx);
+    // the support's registers (S, x, the Cholesky factor) are sized by KM:
+    // 5 covers the usual OMP limit with a 5 x 5 factor instead of 8 x 8
+    if (mfma_ok(ctx)) {
+        if (ctx->kmax <= 5) return ctx->dim <= 64 ? launch_mfma<16, 5>(ctx) : launch_mfma<36, 5>(ctx);
+        return ctx->dim <= 64 ? launch_mfma<16, 8>(ctx) : launch_mfma<36, 8>(ctx);
+    }
     const int rj = (ctx->dim + 63) / 64;
     if (rj <= 1) return launch_generic<1>(ctx);
     if (rj <= 2) return launch_generic<2>(ctx);
