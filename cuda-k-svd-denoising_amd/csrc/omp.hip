@@ -16,7 +16,9 @@
 //   * the support Gram grows by one Cholesky row per step (entries from the
 //     dictionary Gram G = D^T D), then two triangular solves give x; all four
 //     lanes of a patch carry identical copies of S, L, x;
-//   * r = y - D_S x and ||r|| end the step; ||r|| < 1e-3 retires the patch.
+//   * ||r||^2 = ||y||^2 - b_S . x ends the step; where that is too small to
+//     rule out ||r|| < 1e-3 (the retirement test), the explicit
+//     r = y - D_S x decides.
 // Semantics follow CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402).
 #include "ksvd_internal.hpp"
 
@@ -58,13 +60,25 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
     for (int tile = blockIdx.x * OMP_WAVES + wave; tile < ntiles; tile += gridDim.x * OMP_WAVES) {
         const int64_t p = (int64_t)tile * 16 + col;
         const bool valid = p < P;
-        float y[Q], r[Q];
+        float y[Q];
 #pragma unroll
-        for (int kk = 0; kk < Q; kk++) {
-            const int e = e0 + kk;
-            y[kk] = (valid && e < dim) ? Y[p * dim + e] : 0.f;
-            r[kk] = y[kk];
+        for (int kb = 0; kb < Q; kb += 4) {
+            if (valid && e0 + kb + 3 < dim) {
+                const f4 v = *(const f4 *)(Y + p * dim + e0 + kb);
+#pragma unroll
+                for (int c = 0; c < 4; c++) y[kb + c] = v[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) y[kb + c] = (valid && e0 + kb + c < dim) ? Y[p * dim + e0 + kb + c] : 0.f;
+            }
         }
+        // ||y||^2, and the band below which the retirement test needs the
+        // explicit residual (see the residual norm below)
+        float yy = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < Q; kk++) yy = fmaf(y[kk], y[kk], yy);
+        yy = patch_sum(yy);
+        const float band = 1e-6f + yy * 0x1p-10f + sqrtf(yy) * 0x1p-8f;
         int S[KM];
         float x[KM], bv[KM], L[KM * KM];
 #pragma unroll
@@ -92,7 +106,7 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                     for (int sub = 0; sub < 4; sub++)
 #pragma unroll
                         for (int tt = 0; tt < KT; tt++)
-                            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], r[kb + sub], acc[tt], 0, 0, 0);
+                            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], y[kb + sub], acc[tt], 0, 0, 0);
                 }
 #pragma unroll
                 for (int tt = 0; tt < KT; tt++) c0[tt] = acc[tt];
@@ -106,10 +120,20 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                 for (int q = 0; q < KM; q++) {
                     if (__ballot(q < s) == 0ull) break;
                     const float xq = q < s ? x[q] : 0.f;
-                    const f4 *gr = (const f4 *)(G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g);
+                    const float *gr = G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g;
+                    // row entries of atoms >= K (the zero rows of the LDS
+                    // image) are 0, so their correlations stay 0
                     f4 gv[KT];
 #pragma unroll
-                    for (int tt = 0; tt < KT; tt++) gv[tt] = gr[4 * tt];
+                    for (int tt = 0; tt < KT; tt++) {
+                        const int a0 = tt * 16 + 4 * g;
+                        if ((K & 3) == 0 && a0 + 3 < K) {
+                            gv[tt] = *(const f4 *)(gr + 16 * tt);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 4; i++) gv[tt][i] = a0 + i < K ? gr[16 * tt + i] : 0.f;
+                        }
+                    }
 #pragma unroll
                     for (int tt = 0; tt < KT; tt++)
 #pragma unroll
@@ -212,30 +236,43 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                     }
                 }
             }
-            // ---- residual and its norm ---------------------------------------
-            float nr = 0.f;
-            if (active) {
+            // ---- residual norm -------------------------------------------------
+            // For the least-squares x, ||r||^2 = ||y||^2 - b_S . x.  If x is
+            // off by dx, the explicit ||y - D_S x||^2 exceeds the exact one by
+            // dx^T G_S dx and the formula errs by at most ||y|| |G_S^1/2 dx|,
+            // so an explicit norm below 1e-6 (the retirement test ||r|| <
+            // 1e-3) always leaves the formula inside `band`.  Only there is
+            // r = y - D_S x formed, and its norm decides as before.
+            float nr = yy;
 #pragma unroll
-                for (int kb = 0; kb < Q; kb += 4) {
-                    f4 a4 = f4{0.f, 0.f, 0.f, 0.f};
+            for (int i = 0; i < KM; i++)
+                if (i < s) nr = fmaf(-bv[i], x[i], nr);
+            const bool near = active && nr < band;
+            if (__ballot(near) != 0ull) {
+                float ne = 0.f;
+                if (near) {
 #pragma unroll
-                    for (int i = 0; i < KM; i++)
-                        if (i < s) {
-                            const f4 dv = *(const f4 *)(Ds + S[i] * LD + e0 + kb);
-                            a4[0] = fmaf(dv[0], x[i], a4[0]);
-                            a4[1] = fmaf(dv[1], x[i], a4[1]);
-                            a4[2] = fmaf(dv[2], x[i], a4[2]);
-                            a4[3] = fmaf(dv[3], x[i], a4[3]);
+                    for (int kb = 0; kb < Q; kb += 4) {
+                        f4 a4 = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int i = 0; i < KM; i++)
+                            if (i < s) {
+                                const f4 dv = *(const f4 *)(Ds + S[i] * LD + e0 + kb);
+                                a4[0] = fmaf(dv[0], x[i], a4[0]);
+                                a4[1] = fmaf(dv[1], x[i], a4[1]);
+                                a4[2] = fmaf(dv[2], x[i], a4[2]);
+                                a4[3] = fmaf(dv[3], x[i], a4[3]);
+                            }
+#pragma unroll
+                        for (int c4 = 0; c4 < 4; c4++) {
+                            const float rv = y[kb + c4] - a4[c4];
+                            ne = fmaf(rv, rv, ne);
                         }
-#pragma unroll
-                    for (int c4 = 0; c4 < 4; c4++) {
-                        const float rv = y[kb + c4] - a4[c4];
-                        r[kb + c4] = rv;
-                        nr = fmaf(rv, rv, nr);
                     }
                 }
+                ne = patch_sum(ne);
+                if (near) nr = ne;
             }
-            nr = patch_sum(nr);
             if (active && sqrtf(nr) < 1e-3f) active = false;
         }
         if (valid && g == 0) {

@@ -101,6 +101,76 @@ def test_omp_parity(kl, orc, cfg):
     ks.close()
 
 
+@pytest.mark.parametrize("atoms", [250, 97])
+def test_omp_parity_fewer_atoms(kl, orc, atoms):
+    """Fewer atoms than the MFMA kernel's 256-atom tile (and a count that is
+    not a multiple of 4): the correlations of the missing atoms must stay 0
+    in every step, including the steps that take them from the Gram."""
+    cfg = dict(CFG1, atoms=atoms)
+    clean, noisy, ks = make(kl, cfg)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    io, co, no, _ = orc.omp(ks.patches(), ks.dictionary(), cfg["k"], real="f32")
+    same = support_agreement(idx, cnt, io, no)
+    err = coef_rel_err(idx, coef, io, co, cnt, same)
+    print(f"\nOMP {atoms} atoms: supports equal {same.mean() * 100:.3f}%, coef rel err {err.max():.2e}")
+    assert idx.max() < atoms
+    assert same.mean() >= 0.999 and err.max() <= 1e-4
+    ks.close()
+
+
+def test_omp_retirement_exact_patches(kl, orc):
+    """Patches the dictionary represents exactly, or nearly, stop the pursuit
+    through the retirement test ||r|| < 1e-3 (cublas_matrix_omp.cu's
+    threshold).  Atoms are unit vectors (exact in FP32), patches are 1-3 of
+    them with scale 1e-2 .. 1e3, some with a perturbation of norm 4e-4
+    (retired) or 4e-3 (not retired), and some are zero.  The kernel decides
+    from ||y||^2 - b.x unless that is near zero; the counts and supports must
+    be the oracle's, which forms the residual every step."""
+    rng = np.random.default_rng(11)
+    dim, K, P = 64, 32, 400
+    D = np.zeros((K, dim), np.float32)
+    for k in range(K):
+        D[k, 2 * k] = 1.0
+    Y = np.zeros((P, dim), np.float64)
+    kind = rng.integers(0, 5, P)
+    for p in range(P):
+        if kind[p] == 0:
+            continue  # zero patch
+        n = rng.integers(1, 4)
+        scale = 10.0 ** rng.uniform(-2, 3)
+        for a in rng.choice(K, n, replace=False):
+            Y[p] += scale * rng.uniform(0.5, 1.0) * D[a]
+        if kind[p] in (3, 4):
+            e = np.zeros(dim)
+            e[1::2] = rng.standard_normal(dim // 2)  # off every atom
+            Y[p] += (4e-4 if kind[p] == 3 else 4e-3) * e / np.linalg.norm(e)
+    Y = Y.astype(np.float32)
+    n_side = int(np.ceil(np.sqrt(P)))
+    m = n = 8 * n_side
+    frame = np.zeros((n, m), np.float32)
+    for p in range(P):
+        a, b = divmod(p, n_side)
+        frame[8 * a:8 * a + 8, 8 * b:8 * b + 8] = Y[p].reshape(8, 8)
+    ks = kl.KSvd(patch=(8, 8), slide=(8, 8), atoms=K, omp_iter=5)
+    ks.set_frame(frame, m, n)
+    ks.extract()
+    Yg = ks.patches()
+    ks.set_dictionary(D)
+    ks.omp()
+    idx, coef, cnt = ks.codes()
+    io, co, no, _ = orc.omp(Yg, D, 5, real="f32")
+    same = support_agreement(idx, cnt, io, no)
+    short = int((no[:P] < 5).sum())
+    print(f"\nretirement: {short} of {P} patches stop early in the oracle; supports equal {same.mean() * 100:.2f}%")
+    assert short > P // 2
+    np.testing.assert_array_equal(cnt, no)
+    assert same.all()
+    ks.close()
+
+
 def test_omp_generic_kernel_parity(kl, orc):
     # ompIter 10 is beyond the MFMA kernel's support limit -> generic kernel
     cfg = dict(CFG1, k=10, atoms=300, W=96, H=96, seed=5)
