@@ -787,7 +787,9 @@ static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe)
     if (!ctx->gram_valid && (rc = launch_dict_gram(ctx))) return rc;
     if ((rc = mlp_propose(ctx, PROP_ACCEPT, n, !probe))) return rc;
     if (!probe &&
-        (rc = launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr)))
+        (rc = omp_uses_fp32(ctx)
+                  ? launch_omp_fp32_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr)
+                  : launch_omp_bf16_list(ctx, (const int *)ctx->mlp_fb.ptr, (const unsigned *)ctx->mlp_cnt.ptr)))
         return rc;
     hipLaunchKernelGGL(proposer_count_kernel, dim3(1), dim3(1), 0, ctx->stream, (const unsigned *)ctx->mlp_cnt.ptr,
                        n, (unsigned long long *)ctx->counters.ptr);

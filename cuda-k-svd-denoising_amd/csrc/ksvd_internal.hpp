@@ -113,6 +113,11 @@ int launch_omp(ksvd_ctx *ctx);
 int launch_omp_bf16(ksvd_ctx *ctx);
 // BF16 OMP over the patches listed in `plist` (length *plist_n on the device).
 int launch_omp_bf16_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n);
+// The sparse-coding kernel of a context: the FP32 pursuit for the plain form
+// and for the learned form (its fallback and the judge of its proposals),
+// the BF16-correlation pursuit for the BF16 form alone.
+bool omp_uses_fp32(const ksvd_ctx *ctx);
+int launch_omp_fp32_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n);
 // learned support proposer (mlp.hip)
 int mlp_dims(const ksvd_ctx *ctx, int *kin, int *ko);
 int64_t mlp_param_count(const ksvd_ctx *ctx);

@@ -40,8 +40,9 @@ __device__ __forceinline__ float patch_sum(float v)
 template <int Q, int KT, int KM>
 __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
     const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
-    const float *__restrict__ G, int kmax, int ntiles, int *__restrict__ idx_out,
-    float *__restrict__ coef_out, int *__restrict__ cnt_out, unsigned long long *__restrict__ counters)
+    const float *__restrict__ G, int kmax, int ntiles, const int *__restrict__ plist,
+    const unsigned *__restrict__ plist_n, int *__restrict__ idx_out, float *__restrict__ coef_out,
+    int *__restrict__ cnt_out, float *__restrict__ rnorm_out, unsigned long long *__restrict__ counters)
 {
     constexpr int LD = 4 * Q + 4;
     constexpr int KP = 16 * KT;
@@ -57,8 +58,12 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
     const int e0 = g * Q;
     unsigned long long ndup = 0, ndeg = 0;
 
-    for (int tile = blockIdx.x * OMP_WAVES + wave; tile < ntiles; tile += gridDim.x * OMP_WAVES) {
-        const int64_t p = (int64_t)tile * 16 + col;
+    // list mode: the tiles cover plist[0 .. *plist_n)
+    const int64_t nl = plist ? (int64_t)*plist_n : 0;
+    const int ntl = plist ? (int)((nl + 15) / 16) : ntiles;
+    for (int tile = blockIdx.x * OMP_WAVES + wave; tile < ntl; tile += gridDim.x * OMP_WAVES) {
+        const int64_t slot = (int64_t)tile * 16 + col;
+        const int64_t p = plist ? (slot < nl ? (int64_t)plist[slot] : P) : slot;
         const bool valid = p < P;
         float y[Q];
 #pragma unroll
@@ -87,6 +92,7 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
         for (int i = 0; i < KM * KM; i++) L[i] = 0.f;
         int s = 0;
         bool active = valid;
+        float rlast = yy;  // squared residual norm of the current support
 
         f4 c0[KT];  // D^T y (step 0's correlation)
         for (int t = 0; t < kmax; t++) {
@@ -273,9 +279,11 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                 ne = patch_sum(ne);
                 if (near) nr = ne;
             }
+            if (active) rlast = nr;
             if (active && sqrtf(nr) < 1e-3f) active = false;
         }
         if (valid && g == 0) {
+            if (rnorm_out) rnorm_out[p] = sqrtf(fmaxf(rlast, 0.f));
             cnt_out[p] = s;
 #pragma unroll
             for (int i = 0; i < KM; i++)
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(256) void omp_generic_kernel(
 }
 
 template <int Q, int KM>
-int launch_mfma(ksvd_ctx *ctx)
+int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float *rnorm)
 {
     constexpr int KT = 16;
     constexpr int LD = 4 * Q + 4;
@@ -446,8 +454,8 @@ int launch_mfma(ksvd_ctx *ctx)
     if (grid > ctx->num_cus) grid = ctx->num_cus;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * OMP_WAVES), lds, ctx->stream, (const float *)ctx->Y.ptr,
                        ctx->dim, ctx->P, (const float *)ctx->D.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
-                       ntiles, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
-                       (unsigned long long *)ctx->counters.ptr);
+                       ntiles, plist, plist_n, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                       rnorm, (unsigned long long *)ctx->counters.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
     return KSVD_OK;
 }
@@ -480,19 +488,37 @@ int launch_generic(ksvd_ctx *ctx)
 // True when the MFMA pursuit kernel covers the context's shapes.
 static bool mfma_ok(const ksvd_ctx *ctx) { return ctx->dim <= 144 && ctx->K <= 256 && ctx->kmax <= 8; }
 
+// The FP32 pursuit over every patch, or over the patches plist[0 ..
+// *plist_n); each patch's final residual norm into rnorm when it is given.
+static int launch_mfma_any(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float *rnorm)
+{
+    // the support's registers (S, x, the Cholesky factor) are sized by KM:
+    // 5 covers the usual OMP limit with a 5 x 5 factor instead of 8 x 8
+    if (ctx->kmax <= 5)
+        return ctx->dim <= 64 ? launch_mfma<16, 5>(ctx, plist, plist_n, rnorm)
+                              : launch_mfma<36, 5>(ctx, plist, plist_n, rnorm);
+    return ctx->dim <= 64 ? launch_mfma<16, 8>(ctx, plist, plist_n, rnorm)
+                          : launch_mfma<36, 8>(ctx, plist, plist_n, rnorm);
+}
+
+bool omp_uses_fp32(const ksvd_ctx *ctx) { return ctx->prm.precision == KSVD_PREC_FP32 || ctx->prm.proposer; }
+
+int launch_omp_fp32_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n)
+{
+    if (!mfma_ok(ctx)) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "the listed pursuit needs dim <= 144, atoms <= 256, ompIter <= 8");
+    return launch_mfma_any(ctx, plist, plist_n, (float *)ctx->rbound.ptr);
+}
+
 int launch_omp(ksvd_ctx *ctx)
 {
     if (!ctx->gram_valid) {
         int rc = launch_dict_gram(ctx);
         if (rc) return rc;
     }
-    if (ctx->prm.precision == KSVD_PREC_BF16) return launch_omp_bf16(ctx);
-    // the support's registers (S, x, the Cholesky factor) are sized by KM:
-    // 5 covers the usual OMP limit with a 5 x 5 factor instead of 8 x 8
-    if (mfma_ok(ctx)) {
-        if (ctx->kmax <= 5) return ctx->dim <= 64 ? launch_mfma<16, 5>(ctx) : launch_mfma<36, 5>(ctx);
-        return ctx->dim <= 64 ? launch_mfma<16, 8>(ctx) : launch_mfma<36, 8>(ctx);
-    }
+    if (!omp_uses_fp32(ctx)) return launch_omp_bf16(ctx);
+    // the learned form keeps each patch's residual norm: its proposals are
+    // accepted against it
+    if (mfma_ok(ctx)) return launch_mfma_any(ctx, nullptr, nullptr, ctx->prm.proposer ? (float *)ctx->rbound.ptr : nullptr);
     const int rj = (ctx->dim + 63) / 64;
     if (rj <= 1) return launch_generic<1>(ctx);
     if (rj <= 2) return launch_generic<2>(ctx);
