@@ -59,7 +59,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (109841.2 * 2 + 161096.8) * 1024, "omp": (131910.1 * 2 + 19811.5) * 1024},
+    "2048": {"sweep": (110009.2 * 2 + 161647.3) * 1024, "omp": (132038.9 * 2 + 19817.3) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
@@ -70,13 +70,14 @@ def omp_flops_per_patch(n, K, k):
 
 
 def batch_omp_flops_per_patch(n, K, k):
-    """Flops of one patch's pursuit in the plain form's kernel (Batch-OMP):
-    the first step correlates D^T y (2nK); step s > 1 updates it from the
-    dictionary Gram, c = D^T y - G[:, S] x (2K(s-1)); every step forms
-    b_j = d_j . y (2n), the approximation and residual (2ns + 3n) and the
-    Cholesky row (3s^2).  The support Gram entries are read from G."""
-    return sum((2 * n * K if s == 1 else 2 * K * (s - 1)) + 2 * n + 2 * n * s + 3 * n + 3 * s * s
-               for s in range(1, k + 1))
+    """Flops of one patch's pursuit in the FP32 kernel (Batch-OMP): |y|^2
+    once (2n); the first step correlates D^T y (2nK); step s > 1 updates it
+    from the dictionary Gram, c = D^T y - G[:, S] x (2K(s-1)); every step
+    forms b_j = d_j . y (2n), the Cholesky row and solves (3s^2) and the
+    residual norm |y|^2 - b.x (2s).  The support Gram entries are read from
+    G; the explicit residual is formed only for near-zero norms."""
+    return 2 * n + sum((2 * n * K if s == 1 else 2 * K * (s - 1)) + 2 * n + 3 * s * s + 2 * s
+                       for s in range(1, k + 1))
 
 
 def update_flops_per_patch(n, k):
