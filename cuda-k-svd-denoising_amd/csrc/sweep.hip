@@ -646,11 +646,16 @@ __device__ __forceinline__ void ps_publish(unsigned *flag, unsigned v)
 }
 
 // Threads 0 .. count-1 each wait for flags[first + t] >= v.  Returns false
-// (for the whole workgroup) after an abort or a wait that ran out.
-__device__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *bail)
+// (for the whole workgroup) after an abort or a wait that ran out.  Up to
+// three groups of flags at once: thread t < n0 waits for flags[f0 + t] >= v0,
+// the next n1 threads for flags[f1 + ..] >= v1, the next n2 for f2, v2.
+__device__ bool ps_wait3(unsigned *flags, int f0, int n0, unsigned v0, int f1, int n1, unsigned v1, int f2, int n2,
+                         unsigned v2, int *bail)
 {
-    if ((int)threadIdx.x < count) {
-        const unsigned *f = flags + first + threadIdx.x;
+    const int t = (int)threadIdx.x;
+    if (t < n0 + n1 + n2) {
+        const unsigned *f = t < n0 ? flags + f0 + t : t < n0 + n1 ? flags + f1 + (t - n0) : flags + f2 + (t - n0 - n1);
+        const unsigned v = t < n0 ? v0 : t < n0 + n1 ? v1 : v2;
         long spins = 0;
         while (ld_flag(f) < v) {
             __builtin_amdgcn_s_sleep(2);
@@ -672,6 +677,11 @@ __device__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *
     const bool ok = *bail == 0;
     __syncthreads();
     return ok;
+}
+
+__device__ __forceinline__ bool ps_wait(unsigned *flags, int first, int count, unsigned v, int *bail)
+{
+    return ps_wait3(flags, first, count, v, 0, 0, 0u, 0, 0, 0u, bail);
 }
 
 // e . u for one column held by a wavefront as ev[j] = e[lane + 64j], with u
@@ -1557,9 +1567,9 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
         __syncthreads();
         pa_load_y<10>(a, ncn, Es, sp);
-        if (!ps_wait(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), bail)) return;
-        if (k >= 1 && !ps_wait(flags, PTF_CCERT + other, H, (unsigned)k, bail)) return;
-        if (!ps_wait(flags, PTF_DONE + team, 1, (unsigned)(k + 1), bail)) return;
+        if (!ps_wait3(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), PTF_CCERT + other, k >= 1 ? H : 0,
+                      (unsigned)k, PTF_DONE + team, 1, (unsigned)(k + 1), bail))
+            return;
         l1_drop();
         pa_subtract<NO, BU>(a, ncn, Es, sat, sci);
         // the columns of E_{k+1} that atom k+2's hand-over needs
