@@ -684,6 +684,28 @@ __device__ __forceinline__ bool ps_wait(unsigned *flags, int first, int count, u
     return ps_wait3(flags, first, count, v, 0, 0, 0u, 0, 0, 0u, bail);
 }
 
+// Sums over the 64 lanes of four values a[0..3] at once, returned to every
+// lane in t[0..3] (fixed order): exchanges over lane bits 0 and 1 leave each
+// lane one column's quad sum (column 2 (lane & 1) + ((lane >> 1) & 1)), then
+// rotations by 8 and 4 within rows and exchanges over lane bits 4 and 5 sum
+// the 16 lanes of that column, read back from lanes 0, 2, 1, 3.
+__device__ __forceinline__ void reduce4_cols(const float *a, float *t)
+{
+    const int lane = threadIdx.x & 63;
+    const bool odd = (lane & 1) != 0, hi = (lane & 2) != 0;
+    const float x = (odd ? a[2] : a[0]) + dpp_f<0xB1, 0xF>(odd ? a[0] : a[2]);
+    const float y = (odd ? a[3] : a[1]) + dpp_f<0xB1, 0xF>(odd ? a[1] : a[3]);
+    float v = (hi ? y : x) + dpp_f<0x4E, 0xF>(hi ? x : y);
+    v += dpp_f<0x128, 0xF>(v);  // row_ror:8
+    v += dpp_f<0x124, 0xF>(v);  // row_ror:4
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    t[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    t[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 2));
+    t[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 1));
+    t[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 3));
+}
+
 // e . u for one column held by a wavefront as ev[j] = e[lane + 64j], with u
 // held the same way (0 past dim): lane partials in j order with fmaf, then
 // wave_sum_dpp.  The hand-over (P2) and the stored coefficient (P4) both use
@@ -1142,7 +1164,10 @@ __device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int
 // The columns e_t = y_p - sum_i d_{sat_i} x_i (x_i read at sci_i) of the
 // table into Es (pitch dpad), element-parallel: thread (tc, te) takes element
 // te of BU columns at once, with every load in flight before the first use.
-template <int NO, int BU>
+// The atoms and coefficients are read with L1-bypassing loads (SC1), or with
+// plain loads after the caller has dropped its L1 lines (l1_drop) behind the
+// waits that certify them.
+template <int NO, int BU, bool SC1 = true>
 __device__ void pa_build(const SweepArgs &a, int nc, float *Es, const int *sp, const int *sat, const int *sci)
 {
     const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
@@ -1159,8 +1184,10 @@ __device__ void pa_build(const SweepArgs &a, int nc, float *Es, const int *sp, c
             v[b] = a.Y[(int64_t)sp[tt] * dim + te];
 #pragma unroll
             for (int i = 0; i < NO; i++) {
-                dv[b][i] = ld_c(a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te);
-                xv[b][i] = ld_c(a.coef + max(sci[tt * PS_NOMAX + i], 0));
+                const float *dp = a.D + (int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te;
+                const float *cp = a.coef + max(sci[tt * PS_NOMAX + i], 0);
+                dv[b][i] = SC1 ? ld_c(dp) : *dp;
+                xv[b][i] = SC1 ? ld_c(cp) : *cp;
             }
         }
 #pragma unroll
@@ -1171,63 +1198,6 @@ __device__ void pa_build(const SweepArgs &a, int nc, float *Es, const int *sp, c
                 for (int i = 0; i < NO; i++)
                     if (sci[tt * PS_NOMAX + i] >= 0) v[b] = fmaf(-dv[b][i], xv[b][i], v[b]);
                 Es[tt * dpad + te] = v[b];
-            }
-        }
-    }
-}
-
-// The patches of the table's columns into Es (pitch dpad): thread (tc, te)
-// loads element te of YB columns at once.
-template <int YB>
-__device__ void pa_load_y(const SweepArgs &a, int nc, float *Es, const int *sp)
-{
-    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
-    const int cpt = PT_T / dim;
-    const int tc = tid / dim, te = tid - tc * dim;
-    if (tc >= cpt || nc <= 0) return;
-    for (int t = tc; t < nc; t += YB * cpt) {
-        float v[YB];
-#pragma unroll
-        for (int b = 0; b < YB; b++) v[b] = a.Y[(int64_t)sp[min(t + b * cpt, nc - 1)] * dim + te];
-#pragma unroll
-        for (int b = 0; b < YB; b++)
-            if (t + b * cpt < nc) Es[(t + b * cpt) * dpad + te] = v[b];
-    }
-}
-
-// The other atoms' shares d_j x_j (x_j read at sci) subtracted from the
-// columns in Es, entry order, element-parallel with BU columns' loads in
-// flight together (an empty entry reads atom 0 and coefficient 0 and is then
-// dropped).  With pa_load_y before it, the same columns as pa_build.  The
-// atoms and coefficients are read with plain loads: the caller drops its L1
-// lines (l1_drop) after the waits that certify them.
-template <int NO, int BU>
-__device__ void pa_subtract(const SweepArgs &a, int nc, float *Es, const int *sat, const int *sci)
-{
-    const int tid = threadIdx.x, dim = a.dim, dpad = a.dpad;
-    const int cpt = PT_T / dim;
-    const int tc = tid / dim, te = tid - tc * dim;
-    if (tc >= cpt) return;
-    for (int t = tc; t < nc; t += BU * cpt) {
-        float dv[BU][NO], xv[BU][NO];
-#pragma unroll
-        for (int b = 0; b < BU; b++) {
-            const int tt = min(t + b * cpt, nc - 1);
-#pragma unroll
-            for (int i = 0; i < NO; i++) {
-                dv[b][i] = a.D[(int64_t)max(sat[tt * PS_NOMAX + i], 0) * dim + te];
-                xv[b][i] = a.coef[max(sci[tt * PS_NOMAX + i], 0)];
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < BU; b++) {
-            const int tt = t + b * cpt;
-            if (tt < nc) {
-                float v = Es[tt * dpad + te];
-#pragma unroll
-                for (int i = 0; i < NO; i++)
-                    if (sci[tt * PS_NOMAX + i] >= 0) v = fmaf(-dv[b][i], xv[b][i], v);
-                Es[tt * dpad + te] = v;
             }
         }
     }
@@ -1397,13 +1367,21 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
 #pragma unroll
                         for (int j = 0; j < 3; j++) ev[b][j] = Es[t * dpad + min(lane + 64 * j, dim - 1)];
                     }
+                    // the four columns' dots (u is 0 past dim) reduced together
+                    float a4[4];
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const float v = ps_dot(ev[b], uj, dim);
+                        a4[b] = 0.f;
+#pragma unroll
+                        for (int j = 0; j < 3; j++) a4[b] = fmaf(ev[b][j], uj[j], a4[b]);
+                    }
+                    float vt[4];
+                    reduce4_cols(a4, vt);
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
                         if (wave + PT_PW * (g0 + b) < nc)
 #pragma unroll
-                            for (int j = 0; j < 3; j++) pw[j] = fmaf(ev[b][j], v, pw[j]);
-                    }
+                            for (int j = 0; j < 3; j++) pw[j] = fmaf(ev[b][j], vt[b], pw[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
@@ -1555,8 +1533,9 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         }
         ps_publish(ccert + me, (unsigned)(k + 1));
         // ---- P1: atom k+2's columns without atom k+1 ------------------------------
-        // (the table, the patches and the atom's current vector first: none
-        // changes during the sweep before atom k+2's own update)
+        // (the table and the atom's current vector first, before the waits:
+        // neither changes during the sweep before atom k+2's own update; then
+        // the patches and the other atoms' shares in one pass)
         const int kn = k + 2;
         if (kn >= K) break;
         if (tid == 0) *nhp = 0;
@@ -1566,12 +1545,11 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp);
         if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
         __syncthreads();
-        pa_load_y<10>(a, ncn, Es, sp);
         if (!ps_wait3(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), PTF_CCERT + other, k >= 1 ? H : 0,
                       (unsigned)k, PTF_DONE + team, 1, (unsigned)(k + 1), bail))
             return;
         l1_drop();
-        pa_subtract<NO, BU>(a, ncn, Es, sat, sci);
+        pa_build<NO, BU, false>(a, ncn, Es, sp, sat, sci);
         // the columns of E_{k+1} that atom k+2's hand-over needs
         if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
         l1_drop();
