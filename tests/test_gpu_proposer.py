@@ -9,6 +9,8 @@
     1.05 x OMP-residual test more often than the untrained network's (what
     the training buys); every code, accepted proposal or OMP fallback, meets
     that bound;
+  * the fallback list through the FP32 pursuit gives the full pass's codes
+    bit for bit;
   * end to end, BF16 + learned: PSNR no more than 0.05 dB below the plain FP32
     form, fallback rate reported.
 """
@@ -99,6 +101,39 @@ def test_training_gains_and_bound(kl):
           f"{worst:.4f}")
     assert loss < 0.25 * loss0
     assert acc_trained >= acc_untrained + 0.02
+    assert worst <= 1.05 * (1 + 1e-4)
+    ks.close()
+
+
+def test_learned_fallback_list_matches_full_pursuit(kl):
+    """The learned form codes with the FP32 pursuit, and its fallback runs
+    that kernel over a list of patches.  With an untrained network almost
+    every proposal fails the 1.05 x bound, so almost every patch goes through
+    the list: its codes must be the full pass's bit for bit, and every
+    accepted proposal must meet the bound."""
+    cfg = dict(CFG2, W=256, H=256)
+    clean, noisy, ks = make(kl, cfg, precision=kl.PREC_BF16, proposer=1)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()                                    # the FP32 pass: codes and residual bounds
+    idx0, coef0, cnt0 = ks.codes()
+    ks.proposer_init(5)
+    ks.reset_timings()
+    ks.proposer_omp()
+    idx, coef, cnt = ks.codes()
+    t = ks.timings()
+    same = (cnt == cnt0) & np.all(idx == idx0, axis=1) & np.all(coef == coef0, axis=1)
+    Y = ks.patches().astype(np.float64)
+    D = ks.dictionary().astype(np.float64)
+    worst = 0.0
+    for p in np.nonzero(~same)[0]:
+        r_new = np.linalg.norm(Y[p] - coef[p, :cnt[p]] @ D[idx[p, :cnt[p]]])
+        r_omp = np.linalg.norm(Y[p] - coef0[p, :cnt0[p]] @ D[idx0[p, :cnt0[p]]])
+        worst = max(worst, r_new / max(r_omp, 1e-12))
+    print(f"\nfallback {t['fallback']} of {t['proposed']}; codes equal to the full pass on {same.mean() * 100:.2f}%; "
+          f"worst accepted residual / OMP residual {worst:.4f}")
+    assert t["fallback"] >= 0.9 * t["proposed"]
+    assert same.sum() >= t["fallback"]
     assert worst <= 1.05 * (1 + 1e-4)
     ks.close()
 
