@@ -59,7 +59,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (110009.2 * 2 + 161647.3) * 1024, "omp": (132038.9 * 2 + 19817.3) * 1024},
+    "2048": {"sweep": (99701.5 * 2 + 157479.8) * 1024, "omp": (132038.4 * 2 + 19818.8) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
