@@ -352,6 +352,7 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     if ((rc = prepare_update_graph(ctx))) return rc;
     ctx->has_image = true;
     ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
+    ctx->cap_zero_pending = false;
     ctx->mlp_x_valid = false;
     return KSVD_OK;
 }
@@ -396,6 +397,7 @@ int ksvd_extract_patches(ksvd_ctx *ctx)
     if ((rc = span_end(ctx, &s))) return rc;
     ctx->has_patches = true;
     ctx->has_codes = ctx->has_rebuild = false;
+    ctx->cap_zero_pending = false;
     ctx->mlp_x_valid = false;
     // the learned form's network, inputs, buffers and first training order,
     // and the pinned word its gate reads
@@ -442,6 +444,7 @@ int ksvd_omp(ksvd_ctx *ctx)
     if ((rc = span_end(ctx, &outer))) return rc;
     ctx->has_codes = true;
     ctx->has_rebuild = false;
+    ctx->cap_zero_pending = false;  // every code was rewritten
     return KSVD_OK;
 }
 
@@ -515,6 +518,7 @@ static int proposer_step(ksvd_ctx *ctx)
         if ((double)(n - (int64_t)*ctx->gate_host) < PROP_MIN_ACCEPT * (double)n) {
             ctx->mlp_gated = true;
         } else if (ctx->gate_probe) {
+            if ((rc = flush_cap_zero(ctx))) return rc;
             if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
             if ((rc = mlp_train_all(ctx, steps0 - PROP_PROBE_STEPS, PROP_LR, PROP_SEED + 1 + ctx->iters_done))) return rc;
             if ((rc = span_end(ctx, &s))) return rc;
@@ -578,6 +582,7 @@ int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "sigma2 must be a positive finite noise variance");
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = flush_cap_zero(ctx))) return rc;
     Span s;
     if ((rc = span_begin(ctx, &s, SPAN_REBUILD))) return rc;
     if ((rc = launch_rebuild(ctx, sigma2))) return rc;
@@ -657,7 +662,9 @@ int ksvd_get_codes(ksvd_ctx *ctx, int32_t *idx, float *coef, int32_t *cnt)
     if (!ctx) return KSVD_ERR_ARG;
     if (!ctx->has_codes) return ksvd_fail(ctx, KSVD_ERR_STATE, "no codes");
     const size_t pk = (size_t)ctx->P * ctx->kmax;
-    int rc;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = flush_cap_zero(ctx))) return rc;
     if (idx && (rc = sync_copy_d2h(ctx, idx, ctx->idx.ptr, sizeof(int) * pk))) return rc;
     if (coef && (rc = sync_copy_d2h(ctx, coef, ctx->coef.ptr, sizeof(float) * pk))) return rc;
     if (cnt && (rc = sync_copy_d2h(ctx, cnt, ctx->cnt.ptr, sizeof(int) * (size_t)ctx->P))) return rc;
@@ -688,6 +695,7 @@ int ksvd_set_codes(ksvd_ctx *ctx, const int32_t *idx, const float *coef, const i
     KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->cnt.ptr, cnt, sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
     ctx->has_codes = true;
     ctx->has_rebuild = false;
+    ctx->cap_zero_pending = false;
     return KSVD_OK;
 }
 
@@ -769,6 +777,7 @@ int ksvd_proposer_train(ksvd_ctx *ctx, int steps, float lr, uint64_t seed)
     if (steps < 0 || !(lr > 0.f)) return ksvd_fail(ctx, KSVD_ERR_ARG, "steps >= 0 and lr > 0 required");
     if (ctx->P < MLP_BATCH) return ksvd_fail(ctx, KSVD_ERR_ARG, "fewer patches than one training batch (16384)");
     if (!ctx->mlp_ready && (rc = mlp_init(ctx, PROP_SEED))) return rc;
+    if ((rc = flush_cap_zero(ctx))) return rc;
     Span s;
     if ((rc = span_begin(ctx, &s, SPAN_TRAIN))) return rc;
     if ((rc = mlp_train_all(ctx, steps, lr, seed))) return rc;
@@ -797,6 +806,7 @@ static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe)
     if ((rc = span_end(ctx, &outer))) return rc;
     ctx->has_codes = true;
     ctx->has_rebuild = false;
+    if (!probe) ctx->cap_zero_pending = false;  // proposals and fallback rewrote every code
     return KSVD_OK;
 }
 

@@ -75,6 +75,10 @@ struct ksvd_ctx {
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
+    // The two-team sweep leaves the coefficients of the users beyond the cap
+    // to one clearing pass; the pass is run only before something reads the
+    // codes (they are rewritten by the next OMP otherwise)
+    bool sweep_defers_zero = false, cap_zero_pending = false;
     // the last proposal pass's fallback count, copied to pinned host memory
     // behind gate_ev and read at the next learned-form iteration
     unsigned *gate_host = nullptr;
@@ -118,6 +122,8 @@ int launch_omp_bf16_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_
 // the BF16-correlation pursuit for the BF16 form alone.
 bool omp_uses_fp32(const ksvd_ctx *ctx);
 int launch_omp_fp32_list(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n);
+// Runs the clearing pass the last sweep left pending, if any (sweep.hip).
+int flush_cap_zero(ksvd_ctx *ctx);
 // learned support proposer (mlp.hip)
 int mlp_dims(const ksvd_ctx *ctx, int *kin, int *ko);
 int64_t mlp_param_count(const ksvd_ctx *ctx);

@@ -1601,8 +1601,25 @@ static bool env_flag(const char *name)
     return v && v[0] && v[0] != '0';
 }
 
+int flush_cap_zero(ksvd_ctx *ctx)
+{
+    if (!ctx->cap_zero_pending) return KSVD_OK;
+    SweepArgs a = {};
+    a.coef = (float *)ctx->coef.ptr;
+    a.uoff = (const int *)ctx->uoff.ptr;
+    a.users = (const int *)ctx->users.ptr;
+    a.uslot = (const int *)ctx->uslot.ptr;
+    a.kmax = ctx->kmax;
+    a.cap = ctx->prm.user_cap;
+    hipLaunchKernelGGL(pa_zero_kernel, dim3(ctx->K, 16), dim3(256), 0, ctx->stream, a);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    ctx->cap_zero_pending = false;
+    return KSVD_OK;
+}
+
 int launch_sweep(ksvd_ctx *ctx, float tol)
 {
+    ctx->sweep_defers_zero = false;
     const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
     const int dpad = (dim + 15) / 16 * 16;
     const int nbt = dpad / 16;
@@ -1656,8 +1673,9 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                                ctx->stream, flags, wx, nw);
             // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on XCD 0
             hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
-            hipLaunchKernelGGL(pa_zero_kernel, dim3(K, 16), dim3(256), 0, ctx->stream, a);
             KSVD_HIP_CHECK(ctx, hipGetLastError());
+            // the users beyond the cap get coefficient 0 in flush_cap_zero
+            ctx->sweep_defers_zero = true;
             return KSVD_OK;
         }
     }

@@ -253,14 +253,18 @@ int prepare_update_graph(ksvd_ctx *ctx)
 
 int launch_update(ksvd_ctx *ctx)
 {
+    // the users lists read the codes: the last sweep's clearing first
+    int rc = flush_cap_zero(ctx);
+    if (rc) return rc;
     if (graphs_off()) {
-        int rc = enqueue_update(ctx);
+        rc = enqueue_update(ctx);
         if (rc) return rc;
     } else {
-        int rc = prepare_update_graph(ctx);
+        rc = prepare_update_graph(ctx);
         if (rc) return rc;
         KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
     }
+    ctx->cap_zero_pending = ctx->sweep_defers_zero;
     ctx->gram_valid = false;
     return KSVD_OK;
 }

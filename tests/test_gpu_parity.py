@@ -241,6 +241,35 @@ def test_update_parity(kl, orc, cfg, cap, sweep, monkeypatch):
     ks.close()
 
 
+def test_cap_clearing_before_reuse(kl, orc):
+    """The two-team sweep clears the coefficients of the users beyond the cap
+    in a pass that runs only before the codes are read again.  Two updates in
+    a row (the second reads the first's codes through its users lists) must
+    give the bits of an update, a read-back of the codes, and an update from
+    those codes; a rebuild right after an update must read cleared codes."""
+    cfg = dict(CFG1)
+    runs = []
+    for readback in (False, True):
+        clean, noisy, ks = make(kl, cfg, user_cap=64)
+        ks.extract()
+        ks.init_dictionary()
+        io, co, no, _ = orc.omp(ks.patches(), ks.dictionary(), cfg["k"], real="f32")
+        ks.set_codes(io, co, no)
+        ks.update()
+        if readback:
+            i1, c1, n1 = ks.codes()
+            ks.set_codes(i1, c1, n1)
+        ks.update()
+        sig2 = kl.noise_variance(noisy)
+        out = ks.rebuild_picture(sig2)
+        runs.append((ks.dictionary(), ks.codes()[1], out))
+        ks.close()
+    (d0, c0, o0), (d1, c1, o1) = runs
+    np.testing.assert_array_equal(d0, d1)
+    np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_array_equal(o0, o1)
+
+
 def test_rebuild_bitexact_sums(kl, orc):
     cfg = CFG2
     clean, noisy, ks = make(kl, cfg)
