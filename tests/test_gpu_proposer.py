@@ -138,6 +138,30 @@ def test_learned_fallback_list_matches_full_pursuit(kl):
     ks.close()
 
 
+def test_learned_run_deterministic(kl):
+    """Two runs of the learned form on the same image (training, probe,
+    stand-down decision, iterations, rebuild) give the same bits."""
+    cfg = CFG2
+    res = []
+    for _ in range(2):
+        clean, noisy, ks = make(kl, cfg, precision=kl.PREC_BF16, proposer=1)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        t = ks.timings()
+        res.append((ks.dictionary(), ks.codes(), ks.rebuild_picture(kl.noise_variance(noisy)), t["proposed"],
+                    t["fallback"], t["train_loss"]))
+        ks.close()
+    a, b = res
+    print(f"\nlearned run twice: proposed {a[3]} / {b[3]}, fallback {a[4]} / {b[4]}, loss {a[5]:.5f} / {b[5]:.5f}")
+    assert a[3] > 0
+    np.testing.assert_array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[2], b[2])
+    assert a[3:] == b[3:]
+
+
 def test_bf16_learned_end_to_end_psnr(kl):
     cfg = CFG2
     out, tm = {}, {}
