@@ -1068,9 +1068,8 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 //   PI  exchanges i = 1, 2, ... until |u_i - u_{i-1}|^2 < tol2 with i >= 2,
 //       or i = maxit, or z_i = 0 (then u_{i-1} stays).  After exchange 1's
 //       store: its E_k columns to global memory and ecert[h] = k + 1 (after
-//       a store wait).  Only
-//       PT_PW wavefronts run the arithmetic on the vectors; the others take
-//       part in the exchange's reads and the barriers;
+//       a store wait).  PT_PW wavefronts form w_h; every wavefront takes
+//       part in the exchange's reads and forms u_i and the stop decision;
 //   out helper 0 stores d_k as pairs tagged k + 1 for team 1-t (no store
 //       wait on this path), then into the dictionary, and sets done[t] =
 //       k + 1 (after a store wait);
@@ -1237,8 +1236,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     int *hl = hrank + PS_CWMAX;               // PS_CWMAX: columns with a hand-over
     int *nhp = hl + PS_CWMAX;                 // their number
     int *ss = nhp + 1;                        // PS_CWMAX: own coefficient slots of the columns
-    int *sdec = ss + PS_CWMAX;                // the power iteration's stop decision
-    int *suo = sdec + 1;                      // K + 1: uoff
+    int *suo = ss + PS_CWMAX;                 // K + 1: uoff
     int *smo = suo + K + 1;                   // K + 1: moff
     unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
     const int me = team * PT_HT + h, other = (1 - team) * PT_HT;
@@ -1447,9 +1445,11 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             }
             __syncthreads();
             if (*bail) return -1;
-            // the PT_PW wavefronts form z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
-            // (each the same, bit for bit); wavefront 0 posts the decision
-            if (piw) {
+            // every wavefront forms z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
+            // (each the same, bit for bit) and takes the decision itself, so
+            // no barrier passes it on
+            int dec;
+            {
                 float zj[3], n2 = 0.f;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
@@ -1461,7 +1461,6 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     n2 = fmaf(z, z, n2);
                 }
                 n2 = wave_sum_dpp(n2);
-                int dec;
                 if (n2 == 0.f) {
                     dec = 1;  // E = 0: keep u_{i-1} (d_k at the start; its coefficients become 0)
                 } else {
@@ -1477,10 +1476,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     d2 = wave_sum_dpp(d2);
                     dec = ((it + 1 >= 2 && d2 < a.tol2) || it + 1 >= a.maxit) ? 2 : 0;
                 }
-                if (tid == 0) *sdec = dec;
             }
-            __syncthreads();
-            const int dec = *sdec;
             if (dec != 1) it++;
             return dec != 0 ? 1 : 0;
         };
@@ -1658,7 +1654,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const int cap = a.cap;
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL")) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
-                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 2 + 2 * ((size_t)K + 1));
+                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
             PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 4>;
