@@ -1067,8 +1067,9 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 //       x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x (column r prefetched);
 //   PI  exchanges i = 1, 2, ... until |u_i - u_{i-1}|^2 < tol2 with i >= 2,
 //       or i = maxit, or z_i = 0 (then u_{i-1} stays).  After exchange 1's
-//       store: its E_k columns to global memory and ecert[h] = k + 1 (after
-//       a store wait).  PT_PW wavefronts form w_h; every wavefront takes
+//       store: its E_k columns to global memory; ecert[h] = k + 1 once
+//       exchange 1's vectors are in (after a store wait, which by then finds
+//       the columns out).  PT_PW wavefronts form w_h; every wavefront takes
 //       part in the exchange's reads and forms u_i and the stop decision;
 //   out helper 0 stores d_k as pairs tagged k + 1 for team 1-t (no store
 //       wait on this path), then into the dictionary, and sets done[t] =
@@ -1341,13 +1342,13 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             uj[j] = e < dim ? d0[e] : 0.f;
         }
         // E_k's columns out for team 1-t's hand-over of atom k+1, and ecert
-        auto store_e = [&]() {
+        auto store_e = [&](bool publish) {
             const int d4 = dpad / 4;
             for (int q = tid; q < nc * d4; q += PT_T) {
                 const int tl = q / d4, c4 = q - tl * d4;
                 *(f4 *)(Ecur + (int64_t)(tb + tl) * dpad + 4 * c4) = *(const f4 *)(Es + tl * dpad + 4 * c4);
             }
-            ps_publish(ecert + me, (unsigned)(k + 1));
+            if (publish) ps_publish(ecert + me, (unsigned)(k + 1));
         };
         // issue: wavefront w < PT_PW takes columns w, w + PT_PW, ...: v_t =
         // e_t . u_{i-1} and its partial sum of e_t v_t; w_h = the PT_PW
@@ -1399,7 +1400,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         // helpers gq, gq + ngrp, ..., then the groups in order), then u_i;
         // returns -1 to leave (abort), 1 when the iteration stops, else 0
         int it = 0;
-        auto collect = [&](unsigned s) -> int {
+        auto collect = [&](unsigned s, bool cert) -> int {
             const unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
             const int ngrp = min(PT_T / dim, PT_GRP);
             const int gq = tid / dim, ge = tid - gq * dim;
@@ -1443,7 +1444,14 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     if (gq + ngrp * j < H) z += __uint_as_float((unsigned)pv[j]);
                 wx[gq * dpad + ge] = z;
             }
+            if (cert) {
+                // exchange 1: the E_k columns stored before it are out by now
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
             __syncthreads();
+            if (cert && tid == 0) st_flag(ecert + me, (unsigned)(k + 1));
             if (*bail) return -1;
             // every wavefront forms z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
             // (each the same, bit for bit) and takes the decision itself, so
@@ -1482,16 +1490,16 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         };
         if (m > 0) {
             issue(++seq);
-            store_e();
-            int r = collect(seq);
+            store_e(false);
+            int r = collect(seq, true);
             while (r == 0) {
                 issue(++seq);
-                r = collect(seq);
+                r = collect(seq, false);
             }
             if (r < 0) return;
             iters += (unsigned long long)it;
         } else {
-            store_e();
+            store_e(true);
         }
         // ---- d_k out ---------------------------------------------------------
         if (wave == 0) {
