@@ -79,6 +79,8 @@ struct ksvd_ctx {
     // to one clearing pass; the pass is run only before something reads the
     // codes (they are rewritten by the next OMP otherwise)
     bool sweep_defers_zero = false, cap_zero_pending = false;
+    // rebuild: the LDS size the tiled rebuild kernel was last allowed (0: none yet)
+    size_t rebuild_lds_set = 0;
     // the last proposal pass's fallback count, copied to pinned host memory
     // behind gate_ev and read at the next learned-form iteration
     unsigned *gate_host = nullptr;

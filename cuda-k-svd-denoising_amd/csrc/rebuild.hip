@@ -1,11 +1,17 @@
 // Deterministic overlap-add rebuild, lambda and blend on gfx950
 // (CudaKSvdDenoiser::createImage, cuda_svd_denoiser.cu:413-498).
 //
-// rebuild_pixels: one thread per pixel gathers the patches that cover it in
+// rebuild_tiles: one thread per pixel gathers the patches that cover it in
 //   ascending patch order (no atomics).  Each covering patch's value at that
 //   pixel is D x restricted to the support, summed in ascending atom order
 //   with fused multiply-adds.  Writes the sum and the weight (count) and a
-//   per-workgroup double partial of (sum/w - y)^2 over covered pixels.
+//   per-workgroup double partial of (sum/w - y)^2 over covered pixels.  The
+//   workgroups are persistent (one per CU) and hold the whole dictionary in
+//   LDS; per 16 x 64-pixel tile the supports of the covering patches are
+//   sorted once into LDS, so a patch's codes are read once per tile instead
+//   of once per pixel it covers.
+// rebuild_pixels: the same sums from global memory, one pixel per thread, for
+//   dictionaries that do not fit in LDS.
 // rebuild_lambda: one workgroup folds the partials in a fixed order and forms
 //   lambda = |sqrtf(d / (sigma2 * N)) - 1| in float as the reference does.
 // rebuild_blend: out = (y * lambda + sum) / (lambda + w); uncovered pixels
@@ -76,6 +82,156 @@ __global__ __launch_bounds__(256) void rebuild_pixels_kernel(
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+constexpr int RB_TA = 16, RB_TB = 64;  // tile: 16 rows x 64 consecutive pixels, one per thread
+
+// Covering-patch slots a tile can need along one axis: the patches whose
+// window [p*step, p*step + ext) meets [a0, a0 + t).
+__host__ __device__ inline int rb_span(int t, int ext, int step) { return (t - 1 + ext) / step + 1; }
+
+struct RbTile {
+    int a0, b0, PR0, PC0, nc, ns;
+};
+
+__device__ inline RbTile rb_tile(int t, int ntb, int m, int n, int Pr, int Pc, int rext, int cext, int rstep,
+                                 int cstep)
+{
+    RbTile g;
+    g.a0 = (t / ntb) * RB_TA;
+    g.b0 = (t - (t / ntb) * ntb) * RB_TB;
+    const int a1 = min(g.a0 + RB_TA, n) - 1, b1 = min(g.b0 + RB_TB, m) - 1;
+    g.PR0 = g.a0 >= rext ? (g.a0 - rext) / rstep + 1 : 0;
+    g.PC0 = g.b0 >= cext ? (g.b0 - cext) / cstep + 1 : 0;
+    const int nr = min(a1 / rstep, Pr - 1) - g.PR0 + 1;
+    g.nc = min(b1 / cstep, Pc - 1) - g.PC0 + 1;
+    g.ns = (nr > 0 && g.nc > 0) ? nr * g.nc : 0;
+    return g;
+}
+
+// KM: bound on the support size (kmax <= KM), and the stride of a slot's
+// support in LDS.  nslot: bound on a tile's
+// covering patches, at most RB_TA * RB_TB (one per thread, checked at launch).
+template <int KM>
+__global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
+    const int *__restrict__ idx, const float *__restrict__ coef, const int *__restrict__ cnt,
+    const float *__restrict__ D, int dim, int K, int kmax, int m, int n, int Pr, int Pc, int rext, int cext,
+    int rstep, int cstep, int ntb, int ntiles, int nslot, const float *__restrict__ img, float *__restrict__ sum,
+    float *__restrict__ wgt, double *__restrict__ part)
+{
+    extern __shared__ __attribute__((aligned(16))) float Ds[];  // K x dim, then the tile's supports
+    __shared__ double red[RB_TA * RB_TB / 64];
+    const int tid = threadIdx.x;
+    const int kd = K * dim;
+    float2 *sup = (float2 *)(Ds + ((kd + 3) & ~3));  // [slot][KM]: (atom * dim as bits, coefficient)
+    int *sn = (int *)(sup + (size_t)nslot * KM);  // [slot]: support size
+    // this thread's covering patch of the next tile: its codes, loaded a tile ahead
+    int np = 0, at[KM];
+    float cf[KM];
+    auto fetch = [&](int t) {
+        np = 0;
+        if (t >= ntiles) return;
+        const RbTile g = rb_tile(t, ntb, m, n, Pr, Pc, rext, cext, rstep, cstep);
+        if (tid >= g.ns) return;
+        const int64_t p = (int64_t)(g.PR0 + tid / g.nc) * Pc + (g.PC0 + tid % g.nc);
+        np = cnt[p];
+#pragma unroll
+        for (int i = 0; i < KM; i++) {
+            if (i < kmax) {
+                at[i] = idx[p * kmax + i];
+                cf[i] = coef[p * kmax + i];
+            }
+        }
+    };
+    fetch(blockIdx.x);
+    if ((kd & 3) == 0) {
+        for (int i = tid; i < kd / 4; i += RB_TA * RB_TB) ((float4 *)Ds)[i] = ((const float4 *)D)[i];
+    } else {
+        for (int i = tid; i < kd; i += RB_TA * RB_TB) Ds[i] = D[i];
+    }
+    double acc = 0.0;
+    const int ta = tid / RB_TB, tb = tid - ta * RB_TB;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const RbTile g = rb_tile(t, ntb, m, n, Pr, Pc, rext, cext, rstep, cstep);
+        const int a0 = g.a0, b0 = g.b0, PR0 = g.PR0, PC0 = g.PC0, nc = g.nc;
+        __syncthreads();  // the previous tile's readers (and the dictionary's writers) are done
+        if (tid < g.ns) {
+            // support sorted by atom index, stably (the order an insertion sort
+            // gives); entries np..KM-1 padded with (atom 0, coefficient 0)
+            float2 *sj = sup + (size_t)tid * KM;
+#pragma unroll
+            for (int i = 0; i < KM; i++) {
+                int rank = i;
+                if (i < np) {
+                    rank = 0;
+#pragma unroll
+                    for (int j = 0; j < KM; j++)
+                        rank += (j < np && (at[j] < at[i] || (at[j] == at[i] && j < i))) ? 1 : 0;
+                }
+                sj[rank] = i < np ? make_float2(__int_as_float(at[i] * dim), cf[i]) : make_float2(0.f, 0.f);
+            }
+            sn[tid] = np;
+        }
+        __syncthreads();
+        fetch(t + gridDim.x);
+        const int a = a0 + ta, b = b0 + tb;
+        if (a < n && b < m) {
+            const int pr_lo = a >= rext ? (a - rext) / rstep + 1 : 0;
+            const int pr_hi = min(a / rstep, Pr - 1);
+            const int pc_lo = b >= cext ? (b - cext) / cstep + 1 : 0;
+            const int pc_hi = min(b / cstep, Pc - 1);
+            float s = 0.f, w = 0.f;
+            for (int pr = pr_lo; pr <= pr_hi; pr++) {
+                const int r = a - pr * rstep;
+                for (int pc = pc_lo; pc <= pc_hi; pc++) {
+                    const int e = r * cext + (b - pc * cstep);
+                    const int j = (pr - PR0) * nc + (pc - PC0);
+                    const float2 *sj = sup + (size_t)j * KM;
+                    const int np = sn[j];
+                    // every load of the support first, then the products in
+                    // ascending atom order (the padding is never summed)
+                    float2 u[KM];
+                    float dv[KM];
+#pragma unroll
+                    for (int i = 0; i < KM; i++) u[i] = sj[i];
+#pragma unroll
+                    for (int i = 0; i < KM; i++) dv[i] = Ds[__float_as_int(u[i].x) + e];
+                    float v = 0.f;
+#pragma unroll
+                    for (int i = 0; i < KM; i++)
+                        if (i < np) v = fmaf(dv[i], u[i].y, v);
+                    s = s + v;
+                    w = w + 1.f;
+                }
+            }
+            const int64_t q = (int64_t)a * m + b;
+            sum[q] = s;
+            wgt[q] = w;
+            if (w > 0.f) {
+                const float first = s / w;
+                const float diff = first - img[q];
+                const float sq = diff * diff;
+                acc += (double)sq;
+            }
+        }
+    }
+    // fixed-order fold: lanes by a butterfly, then the wavefronts in order
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double tsum = 0.0;
+        for (int i = 0; i < RB_TA * RB_TB / 64; i++) tsum += red[i];
+        part[blockIdx.x] = tsum;
+    }
+}
+
+// LDS bytes of rebuild_tiles_kernel: the dictionary and nslot supports of
+// stride km.
+size_t rebuild_tiles_lds(int K, int dim, int km, int64_t nslot)
+{
+    const size_t slots = (size_t)nslot;
+    return sizeof(float) * (((size_t)K * dim + 3) & ~(size_t)3) + slots * km * sizeof(float2) + slots * sizeof(int);
+}
+
 __global__ __launch_bounds__(256) void rebuild_lambda_kernel(const double *__restrict__ part, int nparts,
                                                              float sigma2, int64_t N, float *__restrict__ lam_out,
                                                              double *__restrict__ dsum_out)
@@ -134,13 +290,43 @@ int launch_rebuild(ksvd_ctx *ctx, float sigma2)
         return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED,
                          "compat geometry: the rebuild walk does not visit as many patches as the gather");
     const int64_t N = (int64_t)ctx->m * ctx->n;
-    const int nblk = (int)((N + 255) / 256);
-    hipLaunchKernelGGL(rebuild_pixels_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const int *)ctx->idx.ptr,
-                       (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr, (const float *)ctx->D.ptr, ctx->dim,
-                       ctx->kmax, ctx->m, ctx->n, Pr, Pc, rext, cext, rstep, cstep, (const float *)ctx->img.ptr,
-                       (float *)ctx->rsum.ptr, (float *)ctx->rweight.ptr, (double *)ctx->rpart.ptr);
+    const int nblk = (int)((N + 255) / 256);  // blend grid, and the capacity of the partials
+    const int64_t slots = (int64_t)rb_span(RB_TA, rext, rstep) * rb_span(RB_TB, cext, cstep);
+    const int km = ctx->kmax <= 4 ? 4 : ctx->kmax <= 5 ? 5 : ctx->kmax <= 8 ? 8 : 16;
+    const size_t lds = rebuild_tiles_lds(ctx->K, ctx->dim, km, slots);
+    int nparts = nblk;
+    if (ctx->kmax <= 16 && slots <= RB_TA * RB_TB && lds <= 160 * 1024 - sizeof(double) * (RB_TA * RB_TB / 64)) {
+        const int ntb = (ctx->m + RB_TB - 1) / RB_TB;
+        const int64_t nt64 = (int64_t)((ctx->n + RB_TA - 1) / RB_TA) * ntb;
+        if (nt64 > INT32_MAX) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "rebuild: too many tiles");
+        const int ntiles = (int)nt64;
+        int grid = ctx->num_cus;
+        if (grid > ntiles) grid = ntiles;
+        if (grid > nblk) grid = nblk;
+        nparts = grid;
+        auto kern = km == 4 ? rebuild_tiles_kernel<4>
+                  : km == 5 ? rebuild_tiles_kernel<5>
+                  : km == 8 ? rebuild_tiles_kernel<8>
+                            : rebuild_tiles_kernel<16>;
+        if (ctx->rebuild_lds_set != lds) {  // one host call per context, not per rebuild
+            KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)lds));
+            ctx->rebuild_lds_set = lds;
+        }
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(RB_TA * RB_TB), lds, ctx->stream,
+                           (const int *)ctx->idx.ptr, (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr,
+                           (const float *)ctx->D.ptr, ctx->dim, ctx->K, ctx->kmax, ctx->m, ctx->n, Pr, Pc, rext,
+                           cext, rstep, cstep, ntb, ntiles, (int)slots, (const float *)ctx->img.ptr, (float *)ctx->rsum.ptr,
+                           (float *)ctx->rweight.ptr, (double *)ctx->rpart.ptr);
+    } else {
+        hipLaunchKernelGGL(rebuild_pixels_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const int *)ctx->idx.ptr,
+                           (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr, (const float *)ctx->D.ptr,
+                           ctx->dim, ctx->kmax, ctx->m, ctx->n, Pr, Pc, rext, cext, rstep, cstep,
+                           (const float *)ctx->img.ptr, (float *)ctx->rsum.ptr, (float *)ctx->rweight.ptr,
+                           (double *)ctx->rpart.ptr);
+    }
     hipLaunchKernelGGL(rebuild_lambda_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->rpart.ptr,
-                       nblk, sigma2, N, (float *)ctx->rlam.ptr, (double *)((float *)ctx->rlam.ptr + 2));
+                       nparts, sigma2, N, (float *)ctx->rlam.ptr, (double *)((float *)ctx->rlam.ptr + 2));
     hipLaunchKernelGGL(rebuild_blend_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const float *)ctx->img.ptr,
                        (const float *)ctx->rsum.ptr, (const float *)ctx->rweight.ptr, (const float *)ctx->rlam.ptr, N,
                        (float *)ctx->rout.ptr);

@@ -297,6 +297,50 @@ def test_rebuild_bitexact_sums(kl, orc):
     ks.close()
 
 
+# (m, n, patch, slide, atoms, k): the tiled rebuild (dictionary and the
+# sorted supports of a 16 x 64-pixel tile's covering patches in LDS) runs when
+# they fit; otherwise the per-pixel rebuild does.
+REBUILD_CASES = [
+    (150, 97, (7, 5), (2, 3), 64, 3),     # tiled, supports of stride 4, ragged tiles
+    (200, 131, (6, 6), (2, 2), 128, 8),   # tiled, stride 8, 385 covering patches a tile
+    (300, 211, (12, 12), (3, 3), 256, 5), # tiled, stride 5 (the 2048 x 2048 workload's shape)
+    (120, 90, (8, 8), (2, 2), 64, 12),    # tiled, stride 16
+    (96, 80, (8, 8), (1, 1), 64, 5),      # per-pixel: 1,728 covering patches a tile > 1,024 threads
+    (130, 70, (12, 12), (3, 3), 256, 12), # per-pixel: supports of stride 16 overflow LDS beside D
+]
+
+
+@pytest.mark.parametrize("case", REBUILD_CASES, ids=lambda c: f"{c[0]}x{c[1]}_{c[2][0]}x{c[2][1]}s{c[3][0]}{c[3][1]}_K{c[4]}_k{c[5]}")
+def test_rebuild_paths_bitexact(kl, orc, case):
+    """Sums, weights and output bit-exact, lambda equal, against the oracle on
+    the GPU's own codes, for both rebuild kernels."""
+    m, n, (pw, ph), (sw, sh), K, k = case
+    rng = np.random.default_rng(m * 1000 + n)
+    frame = (rng.uniform(0, 255, (n, m)) + 30 * np.sin(np.arange(m) / 7.0)[None, :]).astype(np.float32)
+    ks = kl.KSvd(patch=(pw, ph), slide=(sw, sh), atoms=K, omp_iter=k)
+    ks.set_frame(frame, m, n)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    if k <= 8:  # the dictionary sweep takes OMP limits up to 8
+        ks.update()
+        ks.omp()
+    idx, coef, cnt = ks.codes()
+    D = ks.dictionary()
+    s2 = 400.0
+    out = ks.rebuild(s2)
+    sg, wg, lamg, dsg = ks.rebuild_parts()
+    oo, so, wo, dso, lamo = orc.rebuild(idx, coef, cnt, D, frame, m, n, pw, ph, sw, sh, s2)
+    np.testing.assert_array_equal(wg.ravel(), wo.ravel())
+    np.testing.assert_array_equal(sg.ravel(), so.ravel())
+    assert abs(dsg - dso) <= 1e-9 * dso
+    assert np.float32(lamg) == np.float32(lamo)
+    np.testing.assert_array_equal(out.ravel(), oo.ravel())
+    # a second rebuild of the same context gives the same bits
+    np.testing.assert_array_equal(ks.rebuild(s2), out)
+    ks.close()
+
+
 def run_gpu(kl, cfg):
     clean, noisy, ks = make(kl, cfg)
     ks.extract()
