@@ -22,12 +22,22 @@ namespace {
 constexpr int UB = 256;           // patches per users_* workgroup
 constexpr int UW = UB / 64;       // 64-bit mask words per atom per workgroup
 
+// Patch block of this workgroup.  Workgroups go to the 8 XCDs round robin;
+// XCD x takes a contiguous range of blocks, so the runs that neighbouring
+// blocks write into an atom's list share that XCD's L2 and leave it as whole
+// lines (a bijection on [0, nb)).
+__device__ inline int users_block(int nb)
+{
+    const int b = blockIdx.x, x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 __device__ void build_masks(unsigned long long *msk, const int *__restrict__ idx, const float *__restrict__ coef,
-                            const int *__restrict__ cnt, int64_t P, int K, int kmax)
+                            const int *__restrict__ cnt, int64_t P, int K, int kmax, int blk)
 {
     for (int q = threadIdx.x; q < K * UW; q += UB) msk[q] = 0ull;
     __syncthreads();
-    const int64_t p = (int64_t)blockIdx.x * UB + threadIdx.x;
+    const int64_t p = (int64_t)blk * UB + threadIdx.x;
     if (p < P) {
         const int c = cnt[p];
         for (int i = 0; i < c; i++)
@@ -42,12 +52,13 @@ __global__ __launch_bounds__(UB) void users_count_kernel(const int *__restrict__
                                                          int *__restrict__ blockcnt)
 {
     extern __shared__ unsigned long long msk[];
-    build_masks(msk, idx, coef, cnt, P, K, kmax);
+    const int blk = users_block(gridDim.x);
+    build_masks(msk, idx, coef, cnt, P, K, kmax, blk);
     for (int a = threadIdx.x; a < K; a += UB) {
         int c = 0;
 #pragma unroll
         for (int w = 0; w < UW; w++) c += __popcll(msk[a * UW + w]);
-        blockcnt[(int64_t)blockIdx.x * K + a] = c;
+        blockcnt[(int64_t)blk * K + a] = c;
     }
 }
 
@@ -100,8 +111,9 @@ __global__ __launch_bounds__(UB) void users_scatter_kernel(const int *__restrict
                                                            int *__restrict__ uslot, int *__restrict__ srank)
 {
     extern __shared__ unsigned long long msk[];
-    build_masks(msk, idx, coef, cnt, P, K, kmax);
-    const int64_t p = (int64_t)blockIdx.x * UB + threadIdx.x;
+    const int blk = users_block(gridDim.x);
+    build_masks(msk, idx, coef, cnt, P, K, kmax, blk);
+    const int64_t p = (int64_t)blk * UB + threadIdx.x;
     if (p >= P) return;
     const int w = threadIdx.x >> 6, bit = threadIdx.x & 63;
     const int c = cnt[p];
@@ -111,7 +123,7 @@ __global__ __launch_bounds__(UB) void users_scatter_kernel(const int *__restrict
         const int a = idx[p * kmax + i];
         int rank = __popcll(msk[a * UW + w] & ((1ull << bit) - 1ull));
         for (int ww = 0; ww < w; ww++) rank += __popcll(msk[a * UW + ww]);
-        const int pos = uoff[a] + blockoff[(int64_t)blockIdx.x * K + a] + rank;
+        const int pos = uoff[a] + blockoff[(int64_t)blk * K + a] + rank;
         users[pos] = (int)p;
         uslot[pos] = i;
         srank[p * kmax + i] = pos - uoff[a];  // rank among atom a's users
