@@ -130,14 +130,19 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                     // row entries of atoms >= K (the zero rows of the LDS
                     // image) are 0, so their correlations stay 0
                     f4 gv[KT];
+                    if (K == KP) {  // whole rows: no per-lane bounds
 #pragma unroll
-                    for (int tt = 0; tt < KT; tt++) {
-                        const int a0 = tt * 16 + 4 * g;
-                        if ((K & 3) == 0 && a0 + 3 < K) {
-                            gv[tt] = *(const f4 *)(gr + 16 * tt);
-                        } else {
+                        for (int tt = 0; tt < KT; tt++) gv[tt] = *(const f4 *)(gr + 16 * tt);
+                    } else {
 #pragma unroll
-                            for (int i = 0; i < 4; i++) gv[tt][i] = a0 + i < K ? gr[16 * tt + i] : 0.f;
+                        for (int tt = 0; tt < KT; tt++) {
+                            const int a0 = tt * 16 + 4 * g;
+                            if ((K & 3) == 0 && a0 + 3 < K) {
+                                gv[tt] = *(const f4 *)(gr + 16 * tt);
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < 4; i++) gv[tt][i] = a0 + i < K ? gr[16 * tt + i] : 0.f;
+                            }
                         }
                     }
 #pragma unroll
@@ -191,13 +196,18 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
                 if (g == 0) ndeg++;
             }
             if (active) {
+                // every Gram entry of the new Cholesky row, loaded together
+                // (one round trip to the L2, not one per support entry)
+                float gS[KM];
+#pragma unroll
+                for (int i = 0; i < KM; i++) gS[i] = G[(int64_t)(i < s ? S[i] : j) * K + j];
                 float w[KM];
                 float piv = G[(int64_t)j * K + j];
 #pragma unroll
                 for (int i = 0; i < KM; i++) {
                     w[i] = 0.f;
                     if (i < s) {
-                        float gi = G[(int64_t)S[i] * K + j];
+                        float gi = gS[i];
 #pragma unroll
                         for (int q = 0; q < i; q++) gi = fmaf(-L[i * KM + q], w[q], gi);
                         w[i] = gi / L[i * KM + i];
