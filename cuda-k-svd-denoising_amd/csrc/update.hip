@@ -6,7 +6,8 @@
 //                 the block's patches that use it (coef != 0); popcounts give
 //                 per-block counts;
 //   users_scan    per atom, exclusive scan of the block counts;
-//   users_offsets exclusive scan over atoms;
+//   users_offsets exclusive scans over atoms: the users lists' offsets and
+//                 the capped columns' offsets in the column table;
 //   users_scatter rank of a patch among the block's users of an atom = popcount
 //                 of the mask below it; writes the patch, its support slot and,
 //                 per support entry, the patch's rank in that atom's list.
@@ -62,46 +63,89 @@ __global__ __launch_bounds__(UB) void users_count_kernel(const int *__restrict__
     }
 }
 
+// Per atom (one workgroup each), exclusive scan of its block counts in
+// place; ucount[a] = the total.  Thread t takes UQ consecutive blocks per
+// chunk of 256 UQ (all loads in flight at once), then the 256 thread sums are
+// scanned across the wavefront by shuffles and across the 4 wavefronts in LDS.
+constexpr int UQ = 8;
 __global__ __launch_bounds__(256) void users_scan_kernel(int *__restrict__ blockcnt, int nb, int K,
                                                          int *__restrict__ ucount)
 {
-    __shared__ int sh[256];
-    const int a = blockIdx.x;
+    __shared__ int wsum[4];
+    const int a = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int carry = 0;
-    for (int b0 = 0; b0 < nb; b0 += 256) {
-        const int b = b0 + threadIdx.x;
-        const int v = b < nb ? blockcnt[(int64_t)b * K + a] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += 256 * UQ) {
+        const int bt = b0 + tid * UQ;
+        int v[UQ];
+#pragma unroll
+        for (int q = 0; q < UQ; q++) v[q] = bt + q < nb ? blockcnt[(int64_t)(bt + q) * K + a] : 0;
+        int tsum = 0;
+#pragma unroll
+        for (int q = 0; q < UQ; q++) tsum += v[q];
+        // inclusive scan of the thread sums across the wavefront
+        int incl = tsum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
         }
-        if (b < nb) blockcnt[(int64_t)b * K + a] = carry + sh[threadIdx.x] - v;
-        carry += sh[255];
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wave; w++) before += wsum[w];
+        const int chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        int run = before + incl - tsum;
+#pragma unroll
+        for (int q = 0; q < UQ; q++) {
+            if (bt + q < nb) blockcnt[(int64_t)(bt + q) * K + a] = run;
+            run += v[q];
+        }
+        carry += chunk;
+        __syncthreads();  // wsum is rewritten by the next chunk
     }
-    if (threadIdx.x == 0) ucount[a] = carry;
+    if (tid == 0) ucount[a] = carry;
 }
 
-__global__ __launch_bounds__(1024) void users_offsets_kernel(const int *__restrict__ ucount, int K,
-                                                             int *__restrict__ uoff)
+// Exclusive scan over the 1024 threads of a workgroup (K <= 1024 atoms):
+// shuffles within each wavefront, the 16 wavefront sums in LDS.  Returns the
+// exclusive prefix of v; *total gets the sum.
+__device__ int block_scan_1024(int v, int *wsum, int *total)
 {
-    __shared__ int sh[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < 16; w++) {
+        before += w < wave ? wsum[w] : 0;
+        all += wsum[w];
+    }
+    __syncthreads();  // wsum is reused by the next scan
+    *total = all;
+    return before + incl - v;
+}
+
+// Offsets of every atom's users list (uoff) and of its capped columns in the
+// column table (moff, counts capped at cap when cap > 0), in one launch.
+__global__ __launch_bounds__(1024) void users_offsets_kernel(const int *__restrict__ ucount, int K, int cap,
+                                                             int *__restrict__ uoff, int *__restrict__ moff)
+{
+    __shared__ int wsum[16];
     const int a = threadIdx.x;
     const int v = a < K ? ucount[a] : 0;
-    sh[a] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int t = a >= off ? sh[a - off] : 0;
-        __syncthreads();
-        sh[a] += t;
-        __syncthreads();
-    }
-    if (a < K) uoff[a] = sh[a] - v;
-    if (a == 0) uoff[K] = sh[1023];
+    int tot;
+    const int ex = block_scan_1024(v, wsum, &tot);
+    if (a < K) uoff[a] = ex;
+    if (a == 0) uoff[K] = tot;
+    const int vc = (cap > 0 && v > cap) ? cap : v;
+    const int exc = block_scan_1024(vc, wsum, &tot);
+    if (a < K) moff[a] = exc;
+    if (a == 0) moff[K] = tot;
 }
 
 __global__ __launch_bounds__(UB) void users_scatter_kernel(const int *__restrict__ idx, const float *__restrict__ coef,
@@ -137,24 +181,6 @@ __global__ __launch_bounds__(UB) void users_scatter_kernel(const int *__restrict
 // support entries.  Only coefficient VALUES change during the sweep, so the
 // table holds for all of it and the sweep's loads stop depending on each
 // other.  moff[k] = sum_{j<k} m_j.
-__global__ __launch_bounds__(1024) void cols_offsets_kernel(const int *__restrict__ ucount, int K, int cap,
-                                                            int *__restrict__ moff)
-{
-    __shared__ int sh[1024];
-    const int a = threadIdx.x;
-    int v = a < K ? ucount[a] : 0;
-    if (cap > 0 && v > cap) v = cap;
-    sh[a] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int t = a >= off ? sh[a - off] : 0;
-        __syncthreads();
-        sh[a] += t;
-        __syncthreads();
-    }
-    if (a < K) moff[a] = sh[a] - v;
-    if (a == 0) moff[K] = sh[1023];
-}
 
 __global__ __launch_bounds__(256) void cols_table_kernel(int K, int kmax, int cap, const int *__restrict__ idx,
                                                          const int *__restrict__ cnt, const int *__restrict__ uoff,
@@ -204,15 +230,13 @@ static int enqueue_update(ksvd_ctx *ctx)
     hipLaunchKernelGGL(users_count_kernel, dim3(nb), dim3(UB), mlds, ctx->stream, (const int *)ctx->idx.ptr,
                        (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr, P, K, kmax, blockcnt);
     hipLaunchKernelGGL(users_scan_kernel, dim3(K), dim3(256), 0, ctx->stream, blockcnt, nb, K, (int *)ctx->ucount.ptr);
+    int *moff = (int *)ctx->moff.ptr;
     hipLaunchKernelGGL(users_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const int *)ctx->ucount.ptr, K,
-                       (int *)ctx->uoff.ptr);
+                       cap, (int *)ctx->uoff.ptr, moff);
     hipLaunchKernelGGL(users_scatter_kernel, dim3(nb), dim3(UB), mlds, ctx->stream, (const int *)ctx->idx.ptr,
                        (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr, P, K, kmax,
                        (const int *)blockcnt, (const int *)ctx->uoff.ptr, (int *)ctx->users.ptr,
                        (int *)ctx->uslot.ptr, (int *)ctx->srank.ptr);
-    int *moff = (int *)ctx->moff.ptr;
-    hipLaunchKernelGGL(cols_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const int *)ctx->ucount.ptr, K,
-                       cap, moff);
     hipLaunchKernelGGL(cols_table_kernel, dim3(4, K), dim3(256), 0, ctx->stream, K, kmax, cap,
                        (const int *)ctx->idx.ptr, (const int *)ctx->cnt.ptr, (const int *)ctx->uoff.ptr,
                        (const int *)moff, (const int *)ctx->users.ptr, (const int *)ctx->uslot.ptr,
