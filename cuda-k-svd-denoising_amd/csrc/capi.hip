@@ -37,10 +37,10 @@ void drop_timeline(ksvd_ctx *ctx)
     if (!ctx->timeline) return;
     Timeline *tl = (Timeline *)ctx->timeline;
     for (auto &s : tl->pending) {
-        hipEventDestroy(s.a);
-        hipEventDestroy(s.b);
+        (void)hipEventDestroy(s.a);
+        (void)hipEventDestroy(s.b);
     }
-    for (auto e : tl->pool) hipEventDestroy(e);
+    for (auto e : tl->pool) (void)hipEventDestroy(e);
     delete tl;
     ctx->timeline = nullptr;
 }
@@ -254,13 +254,13 @@ int ksvd_destroy(ksvd_ctx *ctx)
 {
     if (!ctx) return KSVD_OK;
     if (ctx->stream) {
-        hipSetDevice(ctx->device);
-        hipStreamSynchronize(ctx->stream);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
     }
     drop_timeline(ctx);
     drop_update_graph(ctx);
-    if (ctx->gate_ev) hipEventDestroy((hipEvent_t)ctx->gate_ev);
-    if (ctx->gate_host) hipHostFree(ctx->gate_host);
+    if (ctx->gate_ev) (void)hipEventDestroy((hipEvent_t)ctx->gate_ev);
+    if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
@@ -270,8 +270,8 @@ int ksvd_destroy(ksvd_ctx *ctx)
                       &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
                       &ctx->mlp_fb, &ctx->mlp_cnt};
     for (DevBuf *b : bufs)
-        if (b->ptr) hipFree(b->ptr);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (b->ptr) (void)hipFree(b->ptr);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return KSVD_OK;
 }
