@@ -222,6 +222,33 @@ __global__ __launch_bounds__(256) void transpose_bf16_kernel(const __bf16 *__res
         if (c0 + c < cols && r0 + tx < rows) out[(int64_t)(c0 + c) * ldo + r0 + tx] = t[tx][c];
 }
 
+// The same transpose with 8 x 8 blocks in registers (no LDS): a thread
+// loads 8 rows of 8 elements (16 bytes each) and stores 8 rows of 8, so
+// every access is a 16-byte vector and 8 threads cover a 128-byte segment.
+// rows, cols, ldi and ldo multiples of 8 (checked by the caller).  256
+// threads cover a 128 x 128 tile.
+__global__ __launch_bounds__(256) void transpose8_bf16_kernel(const __bf16 *__restrict__ in, int rows, int cols,
+                                                              int ldi, __bf16 *__restrict__ out, int ldo)
+{
+    const int bc = threadIdx.x & 15, br = threadIdx.x >> 4;
+    const int r = blockIdx.y * 128 + 8 * br, c = blockIdx.x * 128 + 8 * bc;
+    if (r >= rows || c >= cols) return;
+    uint4 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = *(const uint4 *)(in + (int64_t)(r + i) * ldi + c);
+    auto word = [](const uint4 &q, int k) { return k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w; };
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        unsigned w[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const unsigned a = word(v[2 * p], j >> 1), b = word(v[2 * p + 1], j >> 1);
+            w[p] = (j & 1) ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+        }
+        *(uint4 *)(out + (int64_t)(c + j) * ldo + r) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // rows [r0, r0 + n) of a [rows x ld] bf16 matrix: column c1 = 1, other
 // columns from c1 on = 0 (the constant-1 column and its padding)
 __global__ void ones_column_kernel(__bf16 *__restrict__ H, int ld, int c1, int64_t rows)
@@ -671,6 +698,13 @@ static int mlp_forward_batch(ksvd_ctx *ctx, const __bf16 *Xb, int nb)
 
 static int transpose(ksvd_ctx *ctx, const __bf16 *in, int rows, int cols, int ldi, __bf16 *out, int ldo)
 {
+    const bool v16 = ((rows | cols | ldi | ldo) & 7) == 0 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0;
+    if (v16) {
+        dim3 grid((cols + 127) / 128, (rows + 127) / 128);
+        hipLaunchKernelGGL(transpose8_bf16_kernel, grid, dim3(256), 0, ctx->stream, in, rows, cols, ldi, out, ldo);
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        return KSVD_OK;
+    }
     dim3 grid((cols + 63) / 64, (rows + 63) / 64);
     hipLaunchKernelGGL(transpose_bf16_kernel, grid, dim3(256), 0, ctx->stream, in, rows, cols, ldi, out, ldo);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
