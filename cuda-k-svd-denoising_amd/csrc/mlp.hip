@@ -274,39 +274,49 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const __bf16 *__restri
 // dZ = (sigmoid(z) - t) / nb for atoms < K (0 for the padding), t = 1 when
 // the atom is in the support of patch perm[b0 + b] (codes idx/coef/cnt, a
 // nonzero coefficient); lpart[b] = the row's loss sum over the atoms of
-// softplus(z) - t z.  One block per row, thread k = atom k (ko <= 256).
+// softplus(z) - t z.  One wavefront per row, 4 rows a block (ko <= 256):
+// lane L holds atoms L + 64 q, q = 0..3, each group of 64 atoms is summed by
+// an xor butterfly and the group sums of lane 0 as (g0 + g1) + (g2 + g3).
 __global__ __launch_bounds__(256) void bce_grad_kernel(const float *__restrict__ Z, int ko, int K, int nb,
                                                        const int *__restrict__ perm, int b0,
                                                        const int *__restrict__ idx, const float *__restrict__ coef,
                                                        const int *__restrict__ cnt, int kmax,
                                                        __bf16 *__restrict__ dZ, float *__restrict__ lpart)
 {
-    __shared__ int sup[KSVD_MAX_SUPPORT];
-    __shared__ int ns;
-    __shared__ float red[4];
-    const int b = blockIdx.x, k = threadIdx.x;
-    const int p = perm[b0 + b];
-    if (k == 0) ns = min(cnt[p], kmax);
-    if (k < kmax) sup[k] = (k < cnt[p] && coef[(int64_t)p * kmax + k] != 0.f) ? idx[(int64_t)p * kmax + k] : -1;
-    __syncthreads();
-    float l = 0.f;
-    if (k < ko) {
-        float d = 0.f;
-        if (k < K) {
-            float t = 0.f;
-            for (int i = 0; i < ns; i++)
-                if (sup[i] == k) t = 1.f;
-            const float z = Z[(int64_t)b * ko + k];
-            const float sg = 1.f / (1.f + expf(-z));
-            d = (sg - t) / (float)nb;
-            l = fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))) - t * z;
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nb) return;  // the whole wavefront
+    const int64_t p = perm[b0 + b];
+    const int ns = min(cnt[p], kmax);
+    bool t[4] = {false, false, false, false};
+    for (int i = 0; i < ns; i++) {
+        const int a = idx[p * kmax + i];
+        if (coef[p * kmax + i] != 0.f) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) t[q] |= a == lane + 64 * q;
         }
-        dZ[(int64_t)b * ko + k] = (__bf16)d;
     }
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) lpart[b] = (red[0] + red[1]) + (red[2] + red[3]);
+    float l[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int k = lane + 64 * q;
+        l[q] = 0.f;
+        if (k < ko) {
+            float d = 0.f;
+            if (k < K) {
+                const float tv = t[q] ? 1.f : 0.f;
+                const float z = Z[(int64_t)b * ko + k];
+                const float sg = 1.f / (1.f + expf(-z));
+                d = (sg - tv) / (float)nb;
+                l[q] = fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))) - tv * z;
+            }
+            dZ[(int64_t)b * ko + k] = (__bf16)d;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        for (int off = 32; off > 0; off >>= 1) l[q] += __shfl_xor(l[q], off);
+    if (lane == 0) lpart[b] = (l[0] + l[1]) + (l[2] + l[3]);
 }
 
 // AdamW over the three weight matrices (contiguous FP32 master copy):
@@ -855,7 +865,7 @@ int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, do
         return rc;
     }
     const int nblk = nb;
-    hipLaunchKernelGGL(bce_grad_kernel, dim3(nb), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr, ko,
+    hipLaunchKernelGGL(bce_grad_kernel, dim3((nb + 3) / 4), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr, ko,
                        ctx->K, nb, (const int *)dperm.ptr, 0, (const int *)didx.ptr, (const float *)dcoef.ptr,
                        (const int *)dcnt.ptr, kmx, (__bf16 *)ctx->mlp_dz.ptr, (float *)ctx->mlp_lpart.ptr);
     if ((rc = mlp_backward_batch(ctx, Xb, nb))) {
@@ -905,7 +915,7 @@ static int mlp_train_list(ksvd_ctx *ctx, const int *perm, int64_t n, int steps, 
         const __bf16 *Xb = (const __bf16 *)ctx->mlp_xb.ptr;
         if ((rc = mlp_forward_batch(ctx, Xb, B))) return rc;
         const int nblk = B;
-        hipLaunchKernelGGL(bce_grad_kernel, dim3(B), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr,
+        hipLaunchKernelGGL(bce_grad_kernel, dim3((B + 3) / 4), dim3(256), 0, ctx->stream, (const float *)ctx->mlp_zb.ptr,
                            ko, ctx->K, B, perm, (int)b0, (const int *)ctx->idx.ptr, (const float *)ctx->coef.ptr,
                            (const int *)ctx->cnt.ptr, ctx->kmax, (__bf16 *)ctx->mlp_dz.ptr,
                            (float *)ctx->mlp_lpart.ptr);
