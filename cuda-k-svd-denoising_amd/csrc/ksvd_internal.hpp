@@ -79,7 +79,8 @@ struct ksvd_ctx {
     // to one clearing pass; the pass is run only before something reads the
     // codes (they are rewritten by the next OMP otherwise)
     bool sweep_defers_zero = false, cap_zero_pending = false;
-    // rebuild: the LDS size the tiled rebuild kernel was last allowed (0: none yet)
+    // rebuild: the support stride whose tiled kernel this context has given
+    // its LDS ceiling (0: none yet)
     size_t rebuild_lds_set = 0;
     // the last proposal pass's fallback count, copied to pinned host memory
     // behind gate_ev and read at the next learned-form iteration
@@ -100,6 +101,18 @@ struct ksvd_ctx {
 
 // Set ctx->err and return code.
 int ksvd_fail(ksvd_ctx *ctx, int code, const std::string &msg);
+
+// Let kernel fn use as much dynamic LDS as a workgroup can hold beside its
+// static LDS (160 KB in all on gfx950).  The attribute belongs to the kernel,
+// which every context on the device shares, so it is never set to one
+// context's size, which another context's launch could find too small.
+inline hipError_t allow_max_lds(const void *fn)
+{
+    hipFuncAttributes fa;
+    const hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - (int)fa.sharedSizeBytes);
+}
 // Grow a device buffer to at least `bytes` (contents not preserved).
 int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes);
 

@@ -458,7 +458,7 @@ int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float 
     constexpr int LD = 4 * Q + 4;
     const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
     auto kern = omp_mfma_kernel<Q, KT, KM>;
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
     const int ntiles = (int)((ctx->P + 15) / 16);
     int grid = (ntiles + OMP_WAVES - 1) / OMP_WAVES;
     if (grid > ctx->num_cus) grid = ctx->num_cus;

@@ -571,8 +571,7 @@ int launch_bf16_q(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n)
     int grid = (ntiles + BW - 1) / BW;
     if (grid > ctx->num_cus) grid = ctx->num_cus;
     auto run = [&](auto kern) -> int {
-        KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)lds));
+        KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
         hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * BW), lds, ctx->stream, (const float *)ctx->Y.ptr, ctx->dim,
                            ctx->P, (const __bf16 *)ctx->Dbf.ptr, (const float *)ctx->Dq.ptr,
                            (const unsigned *)dmax, ctx->K, (const float *)ctx->G.ptr, ctx->kmax, ntiles, plist,

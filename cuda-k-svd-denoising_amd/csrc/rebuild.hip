@@ -308,10 +308,9 @@ int launch_rebuild(ksvd_ctx *ctx, float sigma2)
                   : km == 5 ? rebuild_tiles_kernel<5>
                   : km == 8 ? rebuild_tiles_kernel<8>
                             : rebuild_tiles_kernel<16>;
-        if (ctx->rebuild_lds_set != lds) {  // one host call per context, not per rebuild
-            KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                    (int)lds));
-            ctx->rebuild_lds_set = lds;
+        if (ctx->rebuild_lds_set != (size_t)km) {  // one host call per context, not per rebuild
+            KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
+            ctx->rebuild_lds_set = (size_t)km;
         }
         hipLaunchKernelGGL(kern, dim3(grid), dim3(RB_TA * RB_TB), lds, ctx->stream,
                            (const int *)ctx->idx.ptr, (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr,

@@ -1668,7 +1668,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
             // BU = 5: a member's columns (at most 64) in two passes of 35 with every
             // load in flight (2.33 -> 2.28 ms per update at 2048 x 2048 against BU = 4)
             PtFn pt = wide ? pt_sweep_kernel<7, 2> : pt_sweep_kernel<4, 5>;
-            KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)pt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)pt));
             unsigned *flags = (unsigned *)ctx->ticket.ptr;
             // the exchange pairs (two teams x two buffers of PT_HT x dpad) and
             // the d_k pairs (two teams x dpad) sit at the start of the
@@ -1700,7 +1700,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                       : tpw <= 3 ? ps_sweep_kernel<3, 4, 4> : ps_sweep_kernel<5, 4, 4>;
         else ps = tpw <= 1 ? ps_sweep_kernel<1, 7, 2> : tpw <= 2 ? ps_sweep_kernel<2, 7, 2>
                  : tpw <= 3 ? ps_sweep_kernel<3, 7, 2> : ps_sweep_kernel<5, 7, 2>;
-        KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)ps, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)ps));
         unsigned *flags = (unsigned *)ctx->ticket.ptr;
         // the reduced Gram follows the partials in the same buffer
         float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
@@ -1720,8 +1720,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
                                                                                              : k1_kernel<5, 4, 2>;
     else k1 = tpw <= 1 ? k1_kernel<1, 7, 2> : tpw <= 2 ? k1_kernel<2, 7, 2> : tpw <= 3 ? k1_kernel<3, 7, 2>
                                                                                        : k1_kernel<5, 7, 2>;
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
-    KSVD_HIP_CHECK(ctx, hipFuncSetAttribute((const void *)k2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)k1));
+    KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)k2_kernel));
     for (int k = 0; k <= K; k++) {
         hipLaunchKernelGGL(k1, dim3(B1), dim3(T1), lds1, ctx->stream, a, k);
         if (k < K) hipLaunchKernelGGL(k2_kernel, dim3(1), dim3(T2), lds2, ctx->stream, a, k);

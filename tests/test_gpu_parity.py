@@ -341,6 +341,47 @@ def test_rebuild_paths_bitexact(kl, orc, case):
     ks.close()
 
 
+def test_two_contexts_interleaved(kl):
+    """Two contexts of different geometry on one device, their pursuits,
+    sweeps and rebuilds interleaved, give what each gives alone (kernels whose
+    dynamic LDS differs between the contexts share one launch attribute)."""
+    def make2(m, n, patch, slide, K):
+        rng = np.random.default_rng(m + n)
+        frame = (rng.uniform(0, 255, (n, m)) + 40 * np.cos(np.arange(m) / 5.0)[None, :]).astype(np.float32)
+        ks = kl.KSvd(patch=patch, slide=slide, atoms=K, omp_iter=5)
+        ks.set_frame(frame, m, n)
+        ks.extract()
+        ks.init_dictionary()
+        return ks
+    geoms = [(128, 128, (8, 8), (2, 2), 256), (96, 90, (6, 5), (2, 3), 32)]
+
+    def steps(ks):
+        ks.omp()
+        ks.update()
+        ks.omp()
+        return ks.rebuild(400.0)
+    alone = []
+    for g in geoms:
+        ks = make2(*g)
+        alone.append((steps(ks), ks.dictionary()))
+        ks.close()
+    a, b = make2(*geoms[0]), make2(*geoms[1])
+    for ks in (a, b):
+        ks.omp()
+    for ks in (a, b):
+        ks.update()
+    for ks in (a, b):
+        ks.omp()
+    outs = [a.rebuild(400.0), b.rebuild(400.0), a.rebuild(400.0)]
+    np.testing.assert_array_equal(outs[0], alone[0][0])
+    np.testing.assert_array_equal(outs[2], alone[0][0])
+    np.testing.assert_array_equal(outs[1], alone[1][0])
+    np.testing.assert_array_equal(a.dictionary(), alone[0][1])
+    np.testing.assert_array_equal(b.dictionary(), alone[1][1])
+    a.close()
+    b.close()
+
+
 def run_gpu(kl, cfg):
     clean, noisy, ks = make(kl, cfg)
     ks.extract()
