@@ -59,9 +59,12 @@ __global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, fl
     if (q >= (int64_t)K * K) return;
     const int a = (int)(q / K), b = (int)(q - (int64_t)a * K);
     const float *da = D + (int64_t)a * dim, *db = D + (int64_t)b * dim;
-    float s = 0.f;
-    for (int e = 0; e < dim; e++) s = fmaf(da[e], db[e], s);
-    G[q] = s;
+    // accumulated in FP64 and rounded once: the pursuit's support Grams have
+    // condition numbers near 1e3 on a dictionary cut from overlapping
+    // patches, where an FP32 running sum's error would reach the codes
+    double s = 0.0;
+    for (int e = 0; e < dim; e++) s = fma((double)da[e], (double)db[e], s);
+    G[q] = (float)s;
 }
 
 __global__ void transpose_dict_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ Dt)
