@@ -191,7 +191,10 @@ void ksvd_default_params(ksvd_params *p)
 
 const char *ksvd_last_error(const ksvd_ctx *ctx)
 {
-    if (!ctx) return "null context";
+    if (!ctx) {
+        const char *m = noalloc_last_error();
+        return m ? m : "null context";
+    }
     return ctx->err.c_str();
 }
 

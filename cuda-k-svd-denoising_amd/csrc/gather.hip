@@ -115,3 +115,12 @@ int launch_dict_gram(ksvd_ctx *ctx)
     ctx->gram_valid = true;
     return KSVD_OK;
 }
+
+// G = D^T D of a dictionary given by pointer (K atoms of dim floats), on
+// `stream`; the allocation-free pursuit's Gram (omp.hip).
+hipError_t launch_dict_gram_raw(const float *D, int dim, int K, float *G, hipStream_t stream)
+{
+    const int64_t kk = (int64_t)K * K;
+    hipLaunchKernelGGL(dict_gram_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, stream, D, dim, K, G);
+    return hipGetLastError();
+}

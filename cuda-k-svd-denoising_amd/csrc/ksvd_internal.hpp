@@ -128,6 +128,7 @@ int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes);
 int launch_gather(ksvd_ctx *ctx);
 int launch_init_dict(ksvd_ctx *ctx);
 int launch_dict_gram(ksvd_ctx *ctx);
+hipError_t launch_dict_gram_raw(const float *D, int dim, int K, float *G, hipStream_t stream);
 int launch_omp(ksvd_ctx *ctx);
 int launch_omp_bf16(ksvd_ctx *ctx);
 // BF16 OMP over the patches listed in `plist` (length *plist_n on the device).
@@ -160,3 +161,5 @@ size_t sweep_gpart_bytes(int dim);
 // Synchronises the stream; fails if a persistent sweep gave up waiting.
 int check_sweep(ksvd_ctx *ctx);
 int launch_rebuild(ksvd_ctx *ctx, float sigma2);
+// The calling thread's last ksvd_omp_noalloc error, or nullptr (omp.hip).
+const char *noalloc_last_error();

@@ -20,6 +20,10 @@
 //     rule out ||r|| < 1e-3 (the retirement test), the explicit
 //     r = y - D_S x decides.
 // Semantics follow CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402).
+#include <climits>
+#include <cstdio>
+#include <mutex>
+
 #include "ksvd_internal.hpp"
 
 namespace {
@@ -535,4 +539,117 @@ int launch_omp(ksvd_ctx *ctx)
     if (rj <= 4) return launch_generic<4>(ctx);
     if (rj <= 8) return launch_generic<8>(ctx);
     return launch_generic<16>(ctx);
+}
+
+// ---- allocation-free pursuit (ksvd_omp_noalloc) ---------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops).  The only device memory beyond the caller's inputs
+// and outputs is `scratch`: the dictionary Gram (K x K floats, K <= 256) and
+// the pursuit's counters.  Each GPU holds its own copy until the process
+// ends; every call on that GPU reuses it, so calls that overlap on one GPU
+// race.  Host code never passes the array's name to a kernel: it takes the
+// device address hipGetSymbolAddress returns, once per device.
+namespace {
+
+constexpr size_t NOALLOC_GRAM_BYTES = sizeof(float) * 256 * 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_GRAM_BYTES + sizeof(unsigned long long) * CNT_SLOTS;
+static_assert(NOALLOC_SCRATCH_BYTES == KSVD_NOALLOC_SCRATCH_BYTES, "scratch size differs from the header's");
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+unsigned char *noalloc_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch` per device
+int noalloc_cus[NOALLOC_MAX_DEVICES];
+thread_local char noalloc_err[256];
+
+__global__ void noalloc_clear_kernel(unsigned long long *c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+int noalloc_fail(int code, const char *msg, hipError_t e = hipSuccess)
+{
+    if (e != hipSuccess) snprintf(noalloc_err, sizeof(noalloc_err), "ksvd_omp_noalloc: %s: %s", msg, hipGetErrorString(e));
+    else snprintf(noalloc_err, sizeof(noalloc_err), "ksvd_omp_noalloc: %s", msg);
+    return code;
+}
+
+template <int Q, int KM>
+hipError_t noalloc_launch(const float *Y, int dim, int64_t P, const float *D, int K, const float *G, int kmax,
+                          int *idx, float *coef, int *cnt, unsigned long long *counters, int ncu, hipStream_t st)
+{
+    constexpr int KT = 16;
+    constexpr int LD = 4 * Q + 4;
+    const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
+    const int ntiles = (int)((P + 15) / 16);
+    int grid = (ntiles + OMP_WAVES - 1) / OMP_WAVES;
+    if (grid > ncu) grid = ncu;
+    hipLaunchKernelGGL((omp_mfma_kernel<Q, KT, KM>), dim3(grid), dim3(64 * OMP_WAVES), lds, st, Y, dim, P, D, K, G,
+                       kmax, ntiles, (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt,
+                       (float *)nullptr, counters);
+    return hipGetLastError();
+}
+
+// Once per device, with that device current: the address of its copy of
+// `scratch`, its CU count, and the pursuit kernels' LDS ceiling.
+int noalloc_setup(int dev, unsigned char **ptr, int *ncu)
+{
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipGetSymbolAddress(scratch)", e);
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipDeviceGetAttribute", e);
+        const void *kerns[] = {(const void *)omp_mfma_kernel<16, 16, 5>, (const void *)omp_mfma_kernel<36, 16, 5>,
+                               (const void *)omp_mfma_kernel<16, 16, 8>, (const void *)omp_mfma_kernel<36, 16, 8>};
+        for (const void *k : kerns)
+            if ((e = allow_max_lds(k)) != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipFuncSetAttribute", e);
+        noalloc_cus[dev] = cus > 0 ? cus : 256;
+        noalloc_ptr[dev] = (unsigned char *)p;
+    }
+    *ptr = noalloc_ptr[dev];
+    *ncu = noalloc_cus[dev];
+    return KSVD_OK;
+}
+
+}  // namespace
+
+const char *noalloc_last_error() { return noalloc_err[0] ? noalloc_err : nullptr; }
+
+extern "C" int ksvd_omp_noalloc(const float *Y, int dim, int64_t P, const float *D, int K, int kmax, int32_t *idx,
+                                float *coef, int32_t *cnt, void *stream)
+{
+    noalloc_err[0] = 0;
+    if (!Y || !D || !idx || !coef || !cnt) return noalloc_fail(KSVD_ERR_ARG, "null pointer argument");
+    if (dim < 1 || K < 1 || kmax < 1 || P < 0) return noalloc_fail(KSVD_ERR_ARG, "dim, atoms, ompIter must be >= 1 and patches >= 0");
+    if (dim > 144 || K > 256 || kmax > 8)
+        return noalloc_fail(KSVD_ERR_UNSUPPORTED,
+                            "sizes beyond the scratch and kernel maximum (dim <= 144, atoms <= 256, ompIter <= 8)");
+    if ((P + 15) / 16 > (int64_t)INT32_MAX) return noalloc_fail(KSVD_ERR_ARG, "too many patches");
+    if (P == 0) return KSVD_OK;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipGetDevice", e);
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return noalloc_fail(KSVD_ERR_UNSUPPORTED, "device ordinal beyond 63");
+    unsigned char *ptr = nullptr;
+    int ncu = 0;
+    int rc = noalloc_setup(dev, &ptr, &ncu);
+    if (rc) return rc;
+    float *G = (float *)ptr;
+    unsigned long long *counters = (unsigned long long *)(ptr + NOALLOC_GRAM_BYTES);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(noalloc_clear_kernel, dim3(1), dim3(64), 0, st, counters, (int)CNT_SLOTS);
+    if ((e = hipGetLastError()) != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "counter clear launch", e);
+    if ((e = launch_dict_gram_raw(D, dim, K, G, st)) != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "Gram launch", e);
+    if (kmax <= 5)
+        e = dim <= 64 ? noalloc_launch<16, 5>(Y, dim, P, D, K, G, kmax, idx, coef, cnt, counters, ncu, st)
+                      : noalloc_launch<36, 5>(Y, dim, P, D, K, G, kmax, idx, coef, cnt, counters, ncu, st);
+    else
+        e = dim <= 64 ? noalloc_launch<16, 8>(Y, dim, P, D, K, G, kmax, idx, coef, cnt, counters, ncu, st)
+                      : noalloc_launch<36, 8>(Y, dim, P, D, K, G, kmax, idx, coef, cnt, counters, ncu, st);
+    if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "pursuit launch", e);
+    return KSVD_OK;
 }

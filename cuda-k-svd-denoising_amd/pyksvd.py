@@ -81,6 +81,8 @@ SIGNATURES = {
     "ksvd_extract_patches": (ctypes.c_int, [_P]),
     "ksvd_init_dictionary": (ctypes.c_int, [_P]),
     "ksvd_omp": (ctypes.c_int, [_P]),
+    "ksvd_omp_noalloc": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P,
+                                        _P]),
     "ksvd_update_dictionary": (ctypes.c_int, [_P]),
     "ksvd_iterate": (ctypes.c_int, [_P, ctypes.c_int]),
     "ksvd_run": (ctypes.c_int, [_P]),
@@ -353,6 +355,13 @@ class KSvd:
 
     def synchronize(self):
         self._check(lib().ksvd_synchronize(self._h))
+
+
+def omp_noalloc(Y, dim, P, D, K, kmax, idx, coef, cnt, stream=0):
+    """ksvd_omp_noalloc on device pointers (ints); raises KsvdError on failure."""
+    rc = lib().ksvd_omp_noalloc(_P(Y), dim, P, _P(D), K, kmax, _P(idx), _P(coef), _P(cnt), _P(stream or None))
+    if rc != KSVD_OK:
+        raise KsvdError(rc, lib().ksvd_last_error(None).decode())
 
 
 # ---- host helpers ------------------------------------------------------------

@@ -109,7 +109,8 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out);
  * finalize + SvdCudaEngine::finalize (cuda_svd_denoiser.cu:209-210). */
 int ksvd_destroy(ksvd_ctx *ctx);
 
-/* Last error message of ctx ("" if none); static text when ctx is NULL. */
+/* Last error message of ctx ("" if none).  With ctx NULL: the calling
+ * thread's last ksvd_omp_noalloc error, else static text. */
 const char *ksvd_last_error(const ksvd_ctx *ctx);
 
 /* Upload the image buffer (host memory, m*n floats, n rows of length m).
@@ -132,6 +133,23 @@ int ksvd_init_dictionary(ksvd_ctx *ctx);
  * CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402) and the pinv built by
  * CuSolverGeSvdABatch (cusolver_gesvda_batch_engine.cu:117-143). */
 int ksvd_omp(ksvd_ctx *ctx);
+
+/* Allocation-free OMP.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same pursuit as ksvd_omp
+ * (the dictionary Gram, then the FP32 MFMA pursuit), without a context, on
+ * caller-owned device memory: Y holds P patches of dim floats (patch p at
+ * Y + p*dim), D holds K atoms of dim floats; outputs are the compact codes
+ * of ksvd_get_codes (idx, coef: P x kmax; cnt: P), all device pointers.
+ * stream is a hipStream_t (NULL: the null stream); the call only enqueues.
+ * Its only scratch is a static device array of KSVD_NOALLOC_SCRATCH_BYTES
+ * per GPU (the K x K Gram and counters), held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race.
+ * Limits: dim <= 144, K <= 256, kmax <= 8 (KSVD_ERR_UNSUPPORTED beyond).
+ * Errors: ksvd_last_error(NULL) gives the calling thread's last message. */
+#define KSVD_NOALLOC_SCRATCH_BYTES 262272
+int ksvd_omp_noalloc(const float *Y, int dim, int64_t P, const float *D, int K, int kmax,
+                     int32_t *idx, float *coef, int32_t *cnt, void *stream);
 
 /* One Gauss-Seidel sweep of rank-1 atom updates over the current codes.
  * Replaces CudaKSvdDenoiser::updateDictionary (cuda_svd_denoiser.cu:285-362)
