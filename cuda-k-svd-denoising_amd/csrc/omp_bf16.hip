@@ -19,9 +19,11 @@
 //     of its exact value (hi/lo splits leave 2^-16 relative per operand,
 //     products are exact in FP32, at most N FP32 additions), so the FP32
 //     argmax is among the atoms whose score is within 2 delta of the best.
-//     Those candidates -- the best 8 of them by BF16 score when more -- are
-//     re-scored in FP32 (d_a . r with the FP32 atom, lowest index on ties)
-//     when there are two or more; a single one is the argmax already;
+//     Those candidates -- the best two keys of each of the patch's four
+//     lanes, and, on a slow path taken when a lane holds a third, every other
+//     candidate as well -- are re-scored in FP32 (d_a . r with the FP32 atom,
+//     lowest index on ties) when there are two or more; a single one is the
+//     argmax already;
 //   * least squares stay in FP32 (incremental Cholesky of the support Gram
 //     from the FP32 dictionary Gram, two triangular solves); when a new pivot
 //     falls below 1e-6 of its diagonal Gram entry the support Gram and D_S^T y
@@ -370,9 +372,11 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
                     nc++;
                 }
             }
+            // a lane with two candidates may hold a third: such a patch takes
+            // the slow path below, which re-scores every candidate beyond the
+            // lanes' best two
+            bool cut = false;
             if (__ballot(active && m2 >= tk) != 0ull) {
-                // a lane with two candidates may hold a third: count those
-                // patches (the third is not re-scored)
                 unsigned m3 = 0u;
 #pragma unroll
                 for (int tt = 0; tt < KT; tt++)
@@ -383,7 +387,8 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
                     }
                 m3 = umax(m3, (unsigned)__shfl_xor((int)m3, 16));
                 m3 = umax(m3, (unsigned)__shfl_xor((int)m3, 32));
-                if (active && g == 0 && m3 >= tk && m3 > 0u) ncut++;
+                cut = active && finite && m3 >= tk && m3 > 0u;
+                if (cut && g == 0) ncut++;
             }
             if (!finite) nc = 0;
             if (active && g == 0 && nc >= 2) nresc++;
@@ -412,6 +417,47 @@ __global__ __launch_bounds__(64 * BW, 1) void omp_bf16_kernel(
                 if (mine && (v > best32 || (v == best32 && a < j))) {
                     best32 = v;
                     j = a;
+                }
+            }
+            // slow path: every further key >= tk, lane by lane in descending
+            // order (round q takes each lane's next key below the last one it
+            // gave), re-scored in FP32 against the best so far
+            if (__ballot(cut) != 0ull) {
+                unsigned prev = m2;
+#pragma unroll 1
+                for (;;) {
+                    unsigned mn = 0u;
+#pragma unroll
+                    for (int tt = 0; tt < KT; tt++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const unsigned k = __float_as_uint(acc[tt][i]);
+                            mn = umax(mn, k < prev ? k : 0u);
+                        }
+                    const bool more = cut && mn >= tk && mn > 0u;
+                    if (__ballot(more) == 0ull) break;
+#pragma unroll 1
+                    for (int q = 0; q < 4; q++) {
+                        const unsigned kq = (unsigned)__shfl((int)mn, col + 16 * q);
+                        const bool mine = cut && kq >= tk && kq > 0u;
+                        if (__ballot(mine) == 0ull) continue;
+                        const int a = mine ? key_atom(kq) : 0;
+                        const f4 *row = (const f4 *)(Dq + (int64_t)a * 4 * Q + e0);
+                        float cr = 0.f;
+#pragma unroll
+                        for (int c4 = 0; c4 < Q / 4; c4++) {
+                            const f4 dv = row[c4];
+#pragma unroll
+                            for (int i = 0; i < 4; i++) cr = fmaf(dv[i], r[4 * c4 + i], cr);
+                        }
+                        cr = patch_sum(cr);
+                        const float v = fabsf(cr);
+                        if (mine && (v > best32 || (v == best32 && a < j))) {
+                            best32 = v;
+                            j = a;
+                        }
+                    }
+                    prev = more ? mn : 0u;
                 }
             }
             // b_j = d_j . y

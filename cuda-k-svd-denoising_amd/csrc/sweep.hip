@@ -3,7 +3,7 @@
 // of 256 short dependent steps; three implementations, chosen in launch_sweep:
 //
 //  * two-team persistent sweep (pt_sweep_kernel, the default for user caps up
-//    to 1024): two teams of 16 workgroups on XCD 0 take alternate atoms; a
+//    to 1024): two teams of 16 workgroups on one XCD take alternate atoms; a
 //    team runs its atom's power iteration as all-to-all exchanges of
 //    E_h (E_h^T u) vectors while the other team prepares the next atom;
 //  * master/helper persistent sweep (ps_sweep_kernel, user caps up to 1984):
@@ -539,8 +539,10 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 // (cap <= H * PS_CWMAX).
 //
 // The workgroups read the XCD they run on (hardware register XCC_ID); the
-// first H + 1 to arrive on XCD 0 take roles, one master and H helpers, and
-// the others leave at once.  All parties then share XCD 0's L2, so data
+// first workgroup to arrive chooses its XCD, the first H + 1 to arrive there
+// take roles, one master and H helpers, and the others leave at once.  Every
+// role that runs to its end counts itself, so the host also finds a sweep in
+// which roles never ran.  All parties then share that XCD's L2, so data
 // passes between them without an L2 write-back: the writer stores normally,
 // waits for its stores (vmcnt(0)) and sets a flag with an L1-bypassing
 // (sc1) store; the reader polls the flag and reads the data with sc1 loads.
@@ -589,8 +591,22 @@ enum {
     PSF_AFLAG = 128,
     PSF_ABORT = 129,
     PSF_STICKY = 130,
+    PSF_XSEL = 131,      // the XCD the sweep runs on, plus 1 (0: none chosen yet)
+    PSF_STARTED = 132,   // roles launched, summed over sweeps (kept across resets)
+    PSF_FINISHED = 133,  // roles that completed, summed over sweeps (kept across resets)
     PSF_WORDS = 160
 };
+
+// The first workgroup to arrive chooses its own XCD for the sweep; the others
+// take a role (slot < nroles) only on that XCD.  Returns the slot, or -1.
+__device__ __forceinline__ int ps_register(unsigned *flags, int nroles)
+{
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+    const unsigned old = atomicCAS(&flags[PSF_XSEL], 0u, xcc + 1u);
+    if ((old == 0u ? xcc + 1u : old) != xcc + 1u) return -1;
+    const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+    return slot < (unsigned)nroles ? (int)slot : -1;
+}
 
 __device__ __forceinline__ unsigned ld_flag(const unsigned *p)
 {
@@ -1027,9 +1043,7 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
     extern __shared__ __attribute__((aligned(16))) float sm[];
     __shared__ int s_role, s_bail;
     if (threadIdx.x == 0) {
-        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
-        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
-        s_role = (xcc == 0 && slot <= (unsigned)H) ? (int)slot : -1;
+        s_role = ps_register(flags, H + 1);
         s_bail = 0;
     }
     __syncthreads();
@@ -1038,11 +1052,13 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
     // Gram entries per master thread: 4 * QL >= dpad (TPW <= 3: dpad <= 144)
     if (role == 0) ps_master<TPW <= 3 ? 36 : 48>(a, flags, H, Gred, sm, &s_bail);
     else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, Gred, sm, &s_bail);
+    // a role that ran to its end (a bail makes every thread leave early)
+    if (threadIdx.x == 0 && s_bail == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
 // ===========================================================================
 // Two-team persistent sweep: no master and no Gram matrix.  2 x PT_HT
-// workgroups of PT_T threads on XCD 0 form two teams; team t takes the atoms
+// workgroups of PT_T threads on one XCD form two teams; team t takes the atoms
 // k = t, t + 2, ...  The PT_HT helpers of a team hold atom k's capped columns
 // (helper h: [h*cw, (h+1)*cw), cw = ceil(m / PT_HT)) in LDS and run the power
 // iteration together.  Exchange i: each forms w_h = E_h (E_h^T u_{i-1}) from
@@ -1577,23 +1593,24 @@ __global__ __launch_bounds__(PT_T) void pt_sweep_kernel(SweepArgs a, unsigned *f
     extern __shared__ __attribute__((aligned(16))) float sm[];
     __shared__ int s_role, s_bail;
     if (threadIdx.x == 0) {
-        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
-        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
-        s_role = (xcc == 0 && slot < 2u * PT_HT) ? (int)slot : -1;
+        s_role = ps_register(flags, 2 * PT_HT);
         s_bail = 0;
     }
     __syncthreads();
     const int role = s_role;
     if (role < 0) return;
     pt_helper<NO, BU>(a, flags, role & 1, role >> 1, Wx, sm, &s_bail);
+    if (threadIdx.x == 0 && s_bail == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
-// Clears the sweep's flags (all but the sticky abort word) and the nw
-// exchange pairs before a launch.
-__global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw)
+// Clears the sweep's flags (all but the sticky abort word and the role
+// counts) and the nw exchange pairs before a launch, and adds the launch's
+// nroles roles to the started count.
+__global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw, unsigned nroles)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < PSF_WORDS && i != PSF_STICKY) flags[i] = 0u;
+    if (i < PSF_WORDS && i != PSF_STICKY && i != PSF_STARTED && i != PSF_FINISHED) flags[i] = 0u;
+    if (i == PSF_STARTED) flags[i] += nroles;
     if (i < nw) wx[i] = 0ull;
 }
 
@@ -1676,8 +1693,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
             unsigned long long *wx = (unsigned long long *)a.Gpart;
             const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;  // and each team's d_k pairs
             hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
-                               ctx->stream, flags, wx, nw);
-            // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on XCD 0
+                               ctx->stream, flags, wx, nw, (unsigned)(2 * PT_HT));
+            // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on each XCD
             hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
             KSVD_HIP_CHECK(ctx, hipGetLastError());
             // the users beyond the cap get coefficient 0 in flush_cap_zero
@@ -1704,8 +1721,9 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
         unsigned *flags = (unsigned *)ctx->ticket.ptr;
         // the reduced Gram follows the partials in the same buffer
         float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
-        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags, (unsigned long long *)nullptr, 0);
-        // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on XCD 0
+        hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags, (unsigned long long *)nullptr, 0,
+                           (unsigned)(H + 1));
+        // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on each XCD
         hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
@@ -1730,21 +1748,30 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     return KSVD_OK;
 }
 
-// Reads and clears the persistent sweep's sticky abort word (synchronises the
-// context's stream).  Returns KSVD_OK, or KSVD_ERR_HIP if a sweep gave up.
+// Reads and clears the persistent sweep's sticky abort word and role counts
+// (synchronises the context's stream).  Returns KSVD_OK, or KSVD_ERR_HIP if a
+// sweep gave up or left roles unfinished (no workgroup took them).
 int check_sweep(ksvd_ctx *ctx)
 {
-    unsigned v = 0;
+    unsigned v[4] = {0, 0, 0, 0};  // sticky, XCD, started, finished
     unsigned *flags = (unsigned *)ctx->ticket.ptr;
     if (!flags) return KSVD_OK;
-    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(&v, flags + PSF_STICKY, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(v, flags + PSF_STICKY, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (v) {
+    if (v[0] || v[2] != v[3]) {
         KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STICKY, 0, sizeof(unsigned), ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STARTED, 0, 2 * sizeof(unsigned), ctx->stream));
         KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if (v[0])
+            return ksvd_fail(ctx, KSVD_ERR_HIP,
+                             "dictionary sweep: its workgroups did not all run on one XCD (a wait ran out); "
+                             "set KSVD_SWEEP_TWO_KERNEL=1 to use the two-kernel sweep");
         return ksvd_fail(ctx, KSVD_ERR_HIP,
-                         "dictionary sweep: its workgroups did not all run on one XCD (a wait ran out); "
-                         "set KSVD_SWEEP_TWO_KERNEL=1 to use the two-kernel sweep");
+                         "dictionary sweep: " + std::to_string(v[3]) + " of " + std::to_string(v[2]) +
+                             " workgroup roles completed; set KSVD_SWEEP_TWO_KERNEL=1 to use the two-kernel sweep");
+    }
+    if (v[2]) {
+        KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STARTED, 0, 2 * sizeof(unsigned), ctx->stream));
     }
     return KSVD_OK;
 }

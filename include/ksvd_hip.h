@@ -86,7 +86,8 @@ typedef struct ksvd_timings {
     int64_t eig_iters;      /* power iterations spent in rank-1 updates */
     int64_t omp_rescored;   /* BF16 form: patch-steps whose pick was re-scored in FP32 */
     int64_t omp_f64_refactors; /* BF16 form: support Grams re-factorised in FP64 */
-    int64_t omp_candidates_cut; /* BF16 form: patch-steps with a third candidate in one lane group */
+    int64_t omp_candidates_cut; /* BF16 form: patch-steps with a third candidate in one lane group
+                                   (re-scored on the slow path with every other candidate) */
     double train_ms;        /* learned proposer: training (forward, backward, AdamW) */
     int64_t proposed;       /* learned proposer: patches it proposed a support for */
     int64_t fallback;       /* learned proposer: of those, patches left to OMP */

@@ -124,3 +124,70 @@ def test_bf16_end_to_end_psnr(kl, orc, cfg):
         msg += f"; FP64 oracle {kl.psnr(clean, o64):.4f}"
     print(msg)
     assert abs(rb - r32) <= 0.05
+
+
+CFG3 = dict(W=2048, H=2048, sigma=20.0, patch=(12, 12), slide=(3, 3), atoms=256, iters=10, k=5, seed=3)
+
+
+def test_bf16_first_omp_2048(kl, orc):
+    """The BF16 form's first OMP at 2048x2048 (its own workload): supports
+    identical to the FP32 oracle's on >= 99.9 % of patches; coefficients
+    within 1e-4 of the FP64 oracle's (the FP32 oracle's own sums stray
+    further on this dictionary's ill-conditioned supports); every candidate
+    of the proven BF16 margin re-scored in FP32, the slow path included."""
+    from test_gpu_e2e import compare_codes
+    cfg = CFG3
+    clean, noisy, ks = make(kl, cfg, precision=kl.PREC_BF16)
+    ks.extract()
+    ks.init_dictionary()
+    ks.reset_timings()
+    ks.omp()
+    g = ks.codes()
+    t = ks.timings()
+    Y = ks.patches()
+    D = ks.dictionary()
+    ks.close()
+    o32 = orc.omp(Y, D, cfg["k"], real="f32")[:3]
+    o64 = orc.omp(Y, D.astype(np.float64), cfg["k"], real="f64")[:3]
+    frac, err = compare_codes(g, o32)
+    frac64, err64 = compare_codes(g, o64)
+    steps = int(g[2].sum())
+    print(f"\nBF16 OMP 2048^2: supports equal {frac * 100:.4f}% of the FP32 oracle's (coef err {err:.2e}), "
+          f"{frac64 * 100:.4f}% of the FP64 oracle's (coef err {err64:.2e}); re-scored {t['omp_rescored']} of "
+          f"{steps} steps, slow path {t['omp_candidates_cut']}, FP64 refactors {t['omp_f64_refactors']}")
+    assert frac >= 0.999
+    assert err64 <= 1e-4
+
+
+def test_bf16_end_to_end_2048_seeds(kl):
+    """The BF16 form against the plain form at 2048x2048, 10 iterations, on
+    eight seeds.  Ten iterations amplify any near-tie pick: the plain form
+    started from a dictionary one ulp away ends a few hundredths of a dB off.
+    Bars: the mean quality loss (plain - BF16) at most 0.05 dB, and the BF16
+    form's spread around the plain form within 0.05 dB of that one-ulp
+    spread (rms)."""
+    rows = []
+    for seed in range(3, 11):
+        cfg = dict(CFG3, seed=seed)
+        res = []
+        for prec, perturb in ((kl.PREC_FP32, False), (kl.PREC_FP32, True), (kl.PREC_BF16, False)):
+            clean, noisy, ks = make(kl, cfg, precision=prec)
+            ks.extract()
+            ks.init_dictionary()
+            if perturb:
+                D0 = ks.dictionary()
+                D0[0, 0] = np.nextafter(D0[0, 0], np.float32(10))
+                ks.set_dictionary(D0)
+            ks.run()
+            res.append(kl.psnr(clean, ks.rebuild_picture(kl.noise_variance(noisy))))
+            ks.close()
+        rp, rpu, rb = res
+        rows.append((rb - rp, rpu - rp))
+        print(f"  2048^2 seed {seed}: raw PSNR plain {rp:.4f}, plain from a one-ulp dictionary {rpu:.4f} "
+              f"({rpu - rp:+.4f}), BF16 {rb:.4f} ({rb - rp:+.4f} dB)", flush=True)
+    a = np.array(rows)
+    rms = lambda v: float(np.sqrt(np.mean(np.square(v))))
+    print(f"BF16 form 2048^2 over {len(a)} seeds: mean delta PSNR {a[:, 0].mean():+.4f} dB, rms {rms(a[:, 0]):.4f}; "
+          f"one-ulp plain rms {rms(a[:, 1]):.4f}")
+    assert a[:, 0].mean() >= -0.05
+    assert rms(a[:, 0]) <= rms(a[:, 1]) + 0.05
