@@ -93,24 +93,29 @@ extern "C" int ksvd_synth_image(uint64_t seed, int W, int H, float sigma, float 
     return KSVD_OK;
 }
 
-// CImg's variance_noise() default (variance_method = 2): the image is
+// CImg's variance_noise() default (variance_method = 2): each channel is
 // replaced by its 4-neighbour Laplacian scaled by 1/sqrt(20) (so that white
 // noise keeps its variance), borders replicated, and the variance is the
-// squared MAD estimate (1.4828 * median |L - median(L)|)^2, medians taken as
-// element siz/2 of the sorted values.
-extern "C" double ksvd_noise_variance(const float *img, int W, int H)
+// squared MAD estimate (1.4828 * median |L - median(L)|)^2 over the values
+// of every channel together, medians taken as element siz/2 of the sorted
+// values.  img holds C planes of W x H, row-major.
+extern "C" double ksvd_noise_variance_channels(const float *img, int W, int H, int C)
 {
-    if (!img || W <= 0 || H <= 0) return 0.0;
-    const size_t N = (size_t)W * H;
+    if (!img || W <= 0 || H <= 0 || C <= 0) return 0.0;
+    const size_t np = (size_t)W * H, N = np * (size_t)C;
     std::vector<double> L(N);
     const double cste = 1.0 / std::sqrt(20.0);
-    for (int y = 0; y < H; y++) {
-        const int yp = y > 0 ? y - 1 : 0, yn = y < H - 1 ? y + 1 : H - 1;
-        for (int x = 0; x < W; x++) {
-            const int xp = x > 0 ? x - 1 : 0, xn = x < W - 1 ? x + 1 : W - 1;
-            const double c = img[(size_t)y * W + x];
-            L[(size_t)y * W + x] = cste * ((double)img[(size_t)y * W + xn] + (double)img[(size_t)y * W + xp] +
-                                           (double)img[(size_t)yn * W + x] + (double)img[(size_t)yp * W + x] - 4 * c);
+    for (int c = 0; c < C; c++) {
+        const float *p = img + (size_t)c * np;
+        double *l = L.data() + (size_t)c * np;
+        for (int y = 0; y < H; y++) {
+            const int yp = y > 0 ? y - 1 : 0, yn = y < H - 1 ? y + 1 : H - 1;
+            for (int x = 0; x < W; x++) {
+                const int xp = x > 0 ? x - 1 : 0, xn = x < W - 1 ? x + 1 : W - 1;
+                const double v = p[(size_t)y * W + x];
+                l[(size_t)y * W + x] = cste * ((double)p[(size_t)y * W + xn] + (double)p[(size_t)y * W + xp] +
+                                               (double)p[(size_t)yn * W + x] + (double)p[(size_t)yp * W + x] - 4 * v);
+            }
         }
     }
     const size_t mid = N >> 1;
@@ -121,6 +126,11 @@ extern "C" double ksvd_noise_variance(const float *img, int W, int H)
     const double sig = 1.4828 * L[mid];
     const double var = sig * sig;
     return var > 0 ? var : 0;
+}
+
+extern "C" double ksvd_noise_variance(const float *img, int W, int H)
+{
+    return ksvd_noise_variance_channels(img, W, H, 1);
 }
 
 // CImg normalize(a, b): min-max stretch in float; a constant image is filled
@@ -150,6 +160,40 @@ extern "C" double ksvd_psnr(const float *ref, const float *img, int64_t count)
     const double mse = s / (double)count;
     if (mse == 0) return INFINITY;
     return 10.0 * std::log10(255.0 * 255.0 / mse);
+}
+
+// CImg RGBtoYCbCr() on 8-bit-scale planes (BT.601, studio swing), each
+// component clamped to [0, 255]; cb and cr may be NULL.
+extern "C" void ksvd_rgb_to_ycbcr(const float *r, const float *g, const float *b, int64_t count, float *y, float *cb,
+                                  float *cr)
+{
+    if (!r || !g || !b || !y || count <= 0) return;
+    auto cut = [](float v) { return std::min(255.f, std::max(0.f, v)); };
+    for (int64_t q = 0; q < count; q++) {
+        const float R = r[q], G = g[q], B = b[q];
+        y[q] = cut((66 * R + 129 * G + 25 * B + 128) / 256 + 16);
+        if (cb) cb[q] = cut((-38 * R - 74 * G + 112 * B + 128) / 256 + 128);
+        if (cr) cr[q] = cut((112 * R - 94 * G - 18 * B + 128) / 256 + 128);
+    }
+}
+
+extern "C" void ksvd_rgb_to_y(const float *r, const float *g, const float *b, int64_t count, float *y)
+{
+    ksvd_rgb_to_ycbcr(r, g, b, count, y, nullptr, nullptr);
+}
+
+// The speckle transforms of the reference's host path (myLog / myExp,
+// denoisingLib.h:124-142): log2 with 0 kept at 0, and 2^v.
+extern "C" void ksvd_speckle_log(float *buf, int64_t count)
+{
+    if (!buf) return;
+    for (int64_t q = 0; q < count; q++) buf[q] = buf[q] == 0.f ? 0.f : log2f(buf[q]);
+}
+
+extern "C" void ksvd_speckle_exp(float *buf, int64_t count)
+{
+    if (!buf) return;
+    for (int64_t q = 0; q < count; q++) buf[q] = powf(2.f, buf[q]);
 }
 
 extern "C" void ksvd_image_to_frame(const float *img, int W, int H, float *buf)

@@ -639,15 +639,19 @@ __device__ __forceinline__ f4 ld4_c(const float *base, unsigned bytes, unsigned 
 
 // Drops this CU's L1 lines, so that plain loads after it read what other
 // workgroups stored to the XCD's L2 before the flags this workgroup waited
-// for.  Wavefront 0 issues the invalidation; the barrier orders every
-// wavefront's later loads after it.  (The persistent sweep reads the atoms and
-// coefficients of the other atoms' shares this way: with L1-bypassing sc1
-// loads instead, the 2048 x 2048 sweep ran 20 % slower in about one context
-// in four, depending on where the dictionary and coefficient buffers sat in
-// memory.)
+// for.  The invalidation is agent-scope (sc1): a workgroup-scope one (sc0)
+// leaves lines that other CUs' stores have made stale in place, since a
+// workgroup's own waves share one L1, and plain loads after it then read
+// stale atoms and coefficients -- results that changed with the buffers'
+// addresses from one context to the next.  Wavefront 0 issues the
+// invalidation; the barrier orders every wavefront's later loads after it.
+// (The persistent sweep reads the atoms and coefficients of the other atoms'
+// shares this way: with L1-bypassing sc1 loads instead, the 2048 x 2048
+// sweep ran 20 % slower in about one context in four, depending on where the
+// dictionary and coefficient buffers sat in memory.)
 __device__ __forceinline__ void l1_drop()
 {
-    if ((threadIdx.x >> 6) == 0) asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+    if ((threadIdx.x >> 6) == 0) asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 }
 

@@ -44,24 +44,24 @@ Denoiser::~Denoiser()
     delete psnr;
 }
 
-// denoiser.cpp:37-62: load, transpose, keep the Y channel (colour input,
-// B&W false) or channel 0, log2 for speckle (0 stays 0), and the noise
-// variance of the image before the log.
+// denoiser.cpp:37-62: load, transpose, convert a colour picture (B&W false)
+// to YCbCr in place and keep channel 0 (Y; channel 0 of the file when B&W),
+// log2 for speckle (0 stays 0), and the noise variance of the whole
+// (transposed, converted, un-logged) image: every channel's Laplacian
+// pooled, as CImg's variance_noise does.
 bool Denoiser::loadImage()
 {
     auto t0 = std::chrono::steady_clock::now();
     img::Image in;
     if (!img::load(inputFile, in, error)) return false;
     img::Image t = img::transpose(in);
-    if (!bw) t = img::rgb_to_y(t);
+    if (!bw) t = img::rgb_to_ycbcr(t);
     inputImage = new img::Image(t);
     // Matrix(m = W, n = H, ld = W) over the transposed image's first plane
     inputMatrix = new Matrix(in.W, in.H, in.W, t.data.data());
-    if (speckle)
-        for (auto &v : *inputMatrix->hostVector) v = (v == 0.f) ? 0.f : log2f(v);
-    // variance_noise of the (transposed, un-logged) plane; the Laplacian
-    // estimator is symmetric under the transpose
-    sigma = (float)ksvd_noise_variance(t.data.data(), t.W, t.H);
+    if (speckle) ksvd_speckle_log(inputMatrix->hostVector->data(), (int64_t)inputMatrix->hostVector->size());
+    // the Laplacian estimator is symmetric under the transpose
+    sigma = (float)ksvd_noise_variance_channels(t.data.data(), t.W, t.H, t.C);
     timeElapsed->init = ms_since(t0);
     return true;
 }

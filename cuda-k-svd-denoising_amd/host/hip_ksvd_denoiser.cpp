@@ -88,8 +88,7 @@ bool HipKSvdDenoiser::internalDenoising()
     std::vector<float> out;
     if (!denoiseFrame(inputMatrix->hostVector->data(), W, H, sigma, out)) return false;
     // speckle: back from the log domain (myExp, denoisingLib.h:135-142)
-    if (speckle)
-        for (auto &v : out) v = powf(2.f, v);
+    if (speckle) ksvd_speckle_exp(out.data(), (int64_t)out.size());
     // back to the picture layout (the transpose of cuda_svd_denoiser.cu:489-490)
     std::vector<float> *pic = new std::vector<float>((size_t)W * H);
     ksvd_frame_to_image(out.data(), W, H, pic->data());

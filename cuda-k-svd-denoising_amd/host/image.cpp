@@ -1,4 +1,5 @@
 #include "image.h"
+#include "ksvd_hip.h"
 
 #include <algorithm>
 #include <cmath>
@@ -162,7 +163,7 @@ Image transpose(const Image &in)
     return o;
 }
 
-Image rgb_to_y(const Image &in)
+Image rgb_to_ycbcr(const Image &in)
 {
     Image o;
     o.W = in.W;
@@ -170,11 +171,11 @@ Image rgb_to_y(const Image &in)
     o.C = 1;
     const size_t np = in.plane();
     o.data.resize(np);
-    for (size_t q = 0; q < np; q++) {
-        const float R = in.data[q], G = in.C >= 3 ? in.data[np + q] : R, B = in.C >= 3 ? in.data[2 * np + q] : R;
-        const float Y = (66 * R + 129 * G + 25 * B + 128) / 256 + 16;
-        o.data[q] = std::min(255.f, std::max(0.f, Y));
-    }
+    const float *R = in.data.data();
+    const float *G = in.C >= 3 ? R + np : R, *B = in.C >= 3 ? R + 2 * np : R;
+    o.C = 3;
+    o.data.resize(3 * np);
+    ksvd_rgb_to_ycbcr(R, G, B, (int64_t)np, o.data.data(), o.data.data() + np, o.data.data() + 2 * np);
     return o;
 }
 

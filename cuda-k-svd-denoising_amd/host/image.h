@@ -1,5 +1,5 @@
 // Image files and the few CImg operations the reference's Denoiser uses
-// (denoiser.cpp:37-99): load, transpose, RGB -> Y, save.  Formats: PGM/PPM
+// (denoiser.cpp:37-99): load, transpose, RGB -> YCbCr, save.  Formats: PGM/PPM
 // (P2, P3, P5, P6; 8 or 16 bit) and PFM (Pf grayscale, PF colour; float).
 // Pixels are stored planar as CImg does: data[c*W*H + y*W + x].
 #pragma once
@@ -22,8 +22,9 @@ bool save(const std::string &path, const Image &im, std::string &err);
 
 // CImg::transpose(): out(x, y) = in(y, x), per channel.
 Image transpose(const Image &in);
-// CImg::RGBtoYCbCr().channel(0): BT.601 luma on the 8-bit scale,
-// Y = (66 R + 129 G + 25 B + 128) / 256 + 16, clamped to [0, 255].
-Image rgb_to_y(const Image &in);
+// CImg::RGBtoYCbCr(): planes Y, Cb, Cr (BT.601 on the 8-bit scale, each
+// clamped to [0, 255]; ksvd_rgb_to_ycbcr).  A one-channel input is taken as
+// R = G = B.
+Image rgb_to_ycbcr(const Image &in);
 
 }  // namespace img

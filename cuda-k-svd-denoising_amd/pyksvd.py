@@ -113,6 +113,11 @@ SIGNATURES = {
     "ksvd_noise_variance": (ctypes.c_double, [_FP, ctypes.c_int, ctypes.c_int]),
     "ksvd_normalize": (None, [_FP, ctypes.c_int64, ctypes.c_float, ctypes.c_float]),
     "ksvd_psnr": (ctypes.c_double, [_FP, _FP, ctypes.c_int64]),
+    "ksvd_rgb_to_y": (None, [_FP, _FP, _FP, ctypes.c_int64, _FP]),
+    "ksvd_rgb_to_ycbcr": (None, [_FP, _FP, _FP, ctypes.c_int64, _FP, _FP, _FP]),
+    "ksvd_noise_variance_channels": (ctypes.c_double, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ksvd_speckle_log": (None, [_FP, ctypes.c_int64]),
+    "ksvd_speckle_exp": (None, [_FP, ctypes.c_int64]),
     "ksvd_image_to_frame": (None, [_FP, ctypes.c_int, ctypes.c_int, _FP]),
     "ksvd_frame_to_image": (None, [_FP, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -392,6 +397,44 @@ def psnr(ref, img):
     b = np.ascontiguousarray(img, dtype=np.float32)
     assert a.size == b.size
     return lib().ksvd_psnr(_fp(a), _fp(b), a.size)
+
+
+def rgb_to_y(rgb):
+    """(H, W, 3) colour picture -> (H, W) luma as Denoiser::loadImage keeps it."""
+    rgb = np.asarray(rgb, dtype=np.float32)
+    planes = [np.ascontiguousarray(rgb[..., c]) for c in range(3)]
+    y = np.empty(rgb.shape[:2], np.float32)
+    lib().ksvd_rgb_to_y(_fp(planes[0]), _fp(planes[1]), _fp(planes[2]), y.size, _fp(y))
+    return y
+
+
+def rgb_to_ycbcr(rgb):
+    """(H, W, 3) colour picture -> (3, H, W) Y, Cb, Cr planes (CImg RGBtoYCbCr)."""
+    rgb = np.asarray(rgb, dtype=np.float32)
+    planes = [np.ascontiguousarray(rgb[..., c]) for c in range(3)]
+    out = np.empty((3,) + rgb.shape[:2], np.float32)
+    lib().ksvd_rgb_to_ycbcr(_fp(planes[0]), _fp(planes[1]), _fp(planes[2]), planes[0].size, _fp(out[0]),
+                            _fp(out[1]), _fp(out[2]))
+    return out
+
+
+def noise_variance_channels(planes):
+    """CImg variance_noise over (C, H, W) planes pooled."""
+    p = np.ascontiguousarray(planes, dtype=np.float32)
+    C, H, W = p.shape
+    return lib().ksvd_noise_variance_channels(_fp(p), W, H, C)
+
+
+def speckle_log(a):
+    out = np.array(a, dtype=np.float32, copy=True, order="C")
+    lib().ksvd_speckle_log(_fp(out), out.size)
+    return out
+
+
+def speckle_exp(a):
+    out = np.array(a, dtype=np.float32, copy=True, order="C")
+    lib().ksvd_speckle_exp(_fp(out), out.size)
+    return out
 
 
 def image_to_frame(img):

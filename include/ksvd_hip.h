@@ -250,12 +250,31 @@ int ksvd_synth_image(uint64_t seed, int W, int H, float sigma, float *clean, flo
  * then the MAD estimator), the estimate Denoiser::loadImage takes
  * (denoiser.cpp:56).  img is row-major W x H. */
 double ksvd_noise_variance(const float *img, int W, int H);
+/* The same over C planes of W x H pooled, as CImg computes it for a
+ * multi-channel image (loadImage's colour case estimates it on Y, Cb, Cr). */
+double ksvd_noise_variance_channels(const float *img, int W, int H, int C);
 
 /* CImg<float>::normalize(lo, hi) in place (denoiser.cpp:75). */
 void ksvd_normalize(float *img, int64_t count, float lo, float hi);
 
 /* CImg PSNR with the default max 255 (denoiser.cpp:94-96). */
 double ksvd_psnr(const float *ref, const float *img, int64_t count);
+
+/* CImg RGBtoYCbCr(), which Denoiser::loadImage applies to a colour picture
+ * before keeping channel 0 (denoiser.cpp:48-49): per pixel, 8-bit scale,
+ * Y = (66 R + 129 G + 25 B + 128) / 256 + 16,
+ * Cb = (-38 R - 74 G + 112 B + 128) / 256 + 128,
+ * Cr = (112 R - 94 G - 18 B + 128) / 256 + 128, each clamped to [0, 255].
+ * cb, cr may be NULL; ksvd_rgb_to_y is the luma alone. */
+void ksvd_rgb_to_ycbcr(const float *r, const float *g, const float *b, int64_t count, float *y, float *cb,
+                       float *cr);
+void ksvd_rgb_to_y(const float *r, const float *g, const float *b, int64_t count, float *y);
+
+/* The speckle transforms of Denoiser::loadImage and createImage in place
+ * (myLog / myExp, denoisingLib.h:124-142; denoiser.cpp:53-54,
+ * cuda_svd_denoiser.cu:483-484): v -> log2 v with 0 kept at 0, v -> 2^v. */
+void ksvd_speckle_log(float *buf, int64_t count);
+void ksvd_speckle_exp(float *buf, int64_t count);
 
 /* Reference frame <-> image: buf[x*H + y] = img[y*W + x] and back. */
 void ksvd_image_to_frame(const float *img, int W, int H, float *buf);

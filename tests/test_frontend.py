@@ -123,3 +123,106 @@ def test_denoising_cli_end_to_end(denoising_bin, tmp_path, kl):
     assert abs(before[0] - kl.psnr(clean, noisy)) < 1e-3  # printed with 6 significant digits
     assert all(a > 0 for a in after)
     assert os.path.exists(tmp_path / "8_8_noisy.pfm") and os.path.exists(tmp_path / "6_6_noisy.pfm")
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    parts = data.split(maxsplit=4)
+    assert parts[0] in (b"Pf", b"PF")
+    C = 3 if parts[0] == b"PF" else 1
+    w, h, scale = int(parts[1]), int(parts[2]), float(parts[3])
+    a = np.frombuffer(parts[4][: w * h * C * 4], "<f4" if scale < 0 else ">f4").reshape(h, w, C)[::-1]
+    return a[..., 0] if C == 1 else a
+
+
+def write_pfm_rgb(path, rgb):
+    rgb = np.asarray(rgb, np.float32)
+    with open(path, "wb") as f:
+        f.write(b"PF\n%d %d\n-1.0\n" % (rgb.shape[1], rgb.shape[0]))
+        f.write(rgb[::-1].astype("<f4").tobytes())
+
+
+def printed(stdout, key):
+    return [float(l.split(":")[1]) for l in stdout.splitlines() if key in l]
+
+
+@pytest.mark.gpu
+def test_cli_on_reference_pictures(denoising_bin, tmp_path, kl):
+    """`denoising config.json` with the reference's settings (8x8 patches,
+    sliding 2, 256 atoms, CUDA_K_GESVDJ) on its three test pictures at noise
+    variance 0.05 with the published iteration counts: the printed PSNRs are
+    those of the noisy file and of the min-max stretched output that the
+    C ABI gives for the same picture."""
+    import json as js
+    gold = os.path.join(ROOT, "tests", "golden")
+    meta = js.load(open(os.path.join(gold, "reference_psnr.json")))
+    arr = np.load(os.path.join(gold, "reference_images.npz"))
+    files = []
+    for c in meta["cases"]:
+        pic = c["picture"]
+        write_pgm(tmp_path / f"{pic}.pgm", arr[pic + "_noisy"])
+        write_pgm(tmp_path / f"{pic}_ref.pgm", arr[pic + "_ref"])
+        files.append({"name": f"{pic}.pgm", "ref": f"{pic}_ref.pgm", "ksvdIter": c["ksvd_iter"]})
+    cfg = {"inputFolder": str(tmp_path), "outputFolder": str(tmp_path),
+           "globalParams": dict(GLOBAL, type="CUDA_K_GESVDJ"), "files": files}
+    r = run(denoising_bin, cfg, tmp_path)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    before, after = printed(r.stdout, "PSNR before:"), printed(r.stdout, "PSNR after:")
+    for i, c in enumerate(meta["cases"]):
+        pic = c["picture"]
+        noisy = arr[pic + "_noisy"].astype(np.float32)
+        ref = arr[pic + "_ref"].astype(np.float32)
+        ks = kl.KSvd(patch=(8, 8), slide=(2, 2), atoms=256, omp_iter=5, ksvd_iter=c["ksvd_iter"])
+        ks.set_picture(noisy)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        img = ks.rebuild_picture(np.float32(kl.noise_variance(noisy)))
+        ks.close()
+        p_abi = kl.psnr(ref, kl.normalize(img))
+        print(f"{pic}: CLI PSNR before {before[i]} after {after[i]}; C ABI {p_abi:.6g}; published {c['gpu_psnr']} "
+              f"(un-normalised output)")
+        assert abs(before[i] - kl.psnr(ref, noisy)) < 1e-3
+        assert abs(after[i] - p_abi) < 1e-3
+        out = read_pgm(tmp_path / f"8_8_{pic}.pgm")
+        assert out.shape == noisy.shape and out.min() == 0 and out.max() == 255
+
+
+@pytest.mark.gpu
+def test_cli_colour_speckle_non_square(denoising_bin, tmp_path, kl):
+    """A colour picture (B&W false) with multiplicative speckle (speckle
+    true), 96 x 80: the front end keeps Y of YCbCr, estimates sigma^2 on Y,
+    Cb and Cr pooled, denoises log2 Y, takes 2^v of the result and stretches
+    it (denoiser.cpp:37-99, cuda_svd_denoiser.cu:483-484).  Its output must be
+    the bits of the same steps taken through the C ABI."""
+    W, H = 96, 80
+    rng = np.random.default_rng(5)
+    clean = np.stack([kl.synth_image(s, W, H, 0.0)[0] for s in (11, 12, 13)], axis=-1) + 10.0
+    speck = rng.gamma(4.0, 0.25, size=clean.shape).astype(np.float32)  # Gamma(L = 4), mean 1
+    noisy = (clean * speck).astype(np.float32)
+    write_pfm_rgb(tmp_path / "rgb.pfm", noisy)
+    g = dict(GLOBAL, **{"B&W": False, "speckle": True, "ksvdIter": 3, "atoms": 64, "patchWidthDim": 6,
+                        "patchHeightDim": 6})
+    cfg = {"inputFolder": str(tmp_path), "outputFolder": str(tmp_path), "globalParams": g,
+           "files": [{"name": "rgb.pfm"}]}
+    r = run(denoising_bin, cfg, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = read_pfm(tmp_path / "6_6_rgb.pfm")
+    # the same steps through the C ABI
+    ycc = kl.rgb_to_ycbcr(noisy)
+    s2 = np.float32(kl.noise_variance_channels(ycc))
+    frame = kl.speckle_log(kl.image_to_frame(ycc[0]))
+    ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5, ksvd_iter=3)
+    ks.set_frame(frame, W, H)
+    ks.extract()
+    ks.init_dictionary()
+    ks.run()
+    den = kl.speckle_exp(ks.rebuild(s2).ravel())
+    ks.close()
+    expect = kl.normalize(kl.frame_to_image(den, W, H))
+    print(f"\ncolour speckle 96x80: sigma^2 (Y, Cb, Cr pooled) {s2:.2f}, Y alone {kl.noise_variance(ycc[0]):.2f}; "
+          f"output range [{out.min():.3f}, {out.max():.3f}]")
+    assert out.shape == (H, W)
+    np.testing.assert_array_equal(out, expect)

@@ -76,6 +76,64 @@ def test_noise_variance_constant_and_ramp(kl):
     assert kl.noise_variance(ramp) == 0.0
 
 
+def test_noise_variance_stripes_exact(kl):
+    """Vertical stripes 0, c, 0, c, ... (W even): the scaled Laplacian
+    (s = 1/sqrt(20)) is +2cs on the inner 0-columns, -2cs on the inner
+    c-columns, +cs on column 0 and -cs on column W-1 (replicated borders: the
+    missing neighbour is the pixel itself).  Half the values are negative, so
+    the median (element N/2 of the sorted values) is +cs; the deviations are
+    0 (H), cs (H(W-2)/2), 2cs (H), 3cs (H(W-2)/2), whose element N/2 is 2cs.
+    Variance = (1.4828 * 2cs)^2.  Zero-padded borders would give 3cs, an
+    unscaled Laplacian 20x the value, the 1.4826 constant a 2.7e-4 change."""
+    for W, H, c in ((64, 31, 10.0), (10, 7, 3.0)):
+        img = np.zeros((H, W), np.float32)
+        img[:, 1::2] = c
+        expect = (1.4828 * 2 * c / np.sqrt(20.0)) ** 2
+        assert abs(kl.noise_variance(img) - expect) <= 1e-12 * expect, (W, H)
+        # transposed (horizontal stripes): the Laplacian is symmetric in x and y
+        assert abs(kl.noise_variance(np.ascontiguousarray(img.T)) - expect) <= 1e-12 * expect
+
+
+def test_rgb_to_y_closed_forms(kl):
+    # BT.601 luma on the 8-bit scale, Y = (66 R + 129 G + 25 B + 128) / 256 + 16
+    rgb = np.array([[[255, 255, 255], [0, 0, 0], [255, 0, 0], [0, 255, 0], [0, 0, 255], [400, 400, 400],
+                     [-100, -100, -100]]], np.float32)
+    y = kl.rgb_to_y(rgb)
+    expect = [235.640625, 16.5, 82.2421875, 144.99609375, 41.40234375, 255.0, 0.0]
+    np.testing.assert_array_equal(y[0], np.array(expect, np.float32))
+
+
+def test_rgb_to_ycbcr_closed_forms(kl):
+    # Cb = (-38 R - 74 G + 112 B + 128) / 256 + 128, Cr = (112 R - 94 G - 18 B + 128) / 256 + 128
+    rgb = np.array([[[255, 255, 255], [255, 0, 0], [0, 0, 255], [0, 0, 0]]], np.float32)
+    ycc = kl.rgb_to_ycbcr(rgb)
+    np.testing.assert_array_equal(ycc[0, 0], np.array([235.640625, 82.2421875, 41.40234375, 16.5], np.float32))
+    np.testing.assert_array_equal(ycc[1, 0], np.array([128.5, 90.6484375, 240.0625, 128.5], np.float32))
+    np.testing.assert_array_equal(ycc[2, 0], np.array([128.5, 240.0625, 110.5703125, 128.5], np.float32))
+
+
+def test_noise_variance_channels_pooled(kl):
+    """Stripes (as above) in plane 0 and a constant plane 1: the pooled
+    Laplacian values are WH/2 negatives, WH zeros and WH/2 positives, so
+    the median is 0, the deviations are 0 (WH), cs (2H) and 2cs, and element
+    WH of them is cs: variance (1.4828 cs)^2, a quarter of plane 0's alone."""
+    W, H, c = 64, 31, 10.0
+    p0 = np.zeros((H, W), np.float32)
+    p0[:, 1::2] = c
+    planes = np.stack([p0, np.full((H, W), 5.0, np.float32)])
+    expect = (1.4828 * c / np.sqrt(20.0)) ** 2
+    assert abs(kl.noise_variance_channels(planes) - expect) <= 1e-12 * expect
+    assert abs(kl.noise_variance_channels(planes[:1]) - kl.noise_variance(p0)) == 0
+
+
+def test_speckle_transforms(kl):
+    x = np.array([0.0, 1.0, 2.0, 8.0, 0.5, 1024.0], np.float32)
+    lg = kl.speckle_log(x)
+    np.testing.assert_array_equal(lg, np.array([0, 0, 1, 3, -1, 10], np.float32))
+    np.testing.assert_array_equal(kl.speckle_exp(lg[1:]), x[1:])
+    assert kl.speckle_exp(np.zeros(1, np.float32))[0] == 1.0  # a zero pixel comes back as 1
+
+
 def test_normalize_matches_minmax(kl):
     x = np.array([[-3.0, 1.0], [5.0, 2.0]], np.float32)
     y = kl.normalize(x)
