@@ -285,15 +285,24 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     if (!ctx->stream) return ksvd_fail(ctx, KSVD_ERR_STATE, "context was not created successfully");
     if (!buf || m <= 0 || n <= 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "image buffer and sizes must be valid");
     const ksvd_params &p = ctx->prm;
-    if (p.patch_w > m || p.patch_h > n)
+    // The caller's buffer holds the picture column by column (buf[x*H + y],
+    // m = W, n = H), which the reference reads as n rows of length m.  The
+    // true geometry reads it as what it holds, m rows of length n, so that a
+    // patch is a block of the picture even when W != H; the compat geometry
+    // keeps the reference's reading (the same for a square picture).
+    const bool true_geom = p.geometry == KSVD_GEOM_TRUE;
+    const int fm = true_geom ? n : m, fn = true_geom ? m : n;
+    if (p.patch_w > fm || p.patch_h > fn)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "patch larger than the image");
     int rc = set_device(ctx);
     if (rc) return rc;
     rc = resolve(ctx);
     if (rc) return rc;
     const int64_t oldP = ctx->P;
-    ctx->m = m;
-    ctx->n = n;
+    ctx->m = fm;
+    ctx->n = fn;
+    m = fm;
+    n = fn;
     ctx->Pr = (n - p.patch_h) / p.slide_h + 1;
     ctx->Pc = (m - p.patch_w) / p.slide_w + 1;
     ctx->P = (int64_t)ctx->Pr * ctx->Pc;

@@ -9,11 +9,13 @@
  *
  * Frame convention.  The image buffer is the reference's inputMatrix host
  * buffer after Denoiser::loadImage (denoiser.cpp:37-62): the CImg image
- * transposed, read as n rows of length m (m = image width W, n = image height
- * H).  Patches are cut in that frame exactly as CudaKSvdDenoiser::
- * createPatches walks it (cuda_svd_denoiser.cu:231-240), and ksvd_rebuild
- * returns the denoised buffer in the same frame (before the final transpose
- * of cuda_svd_denoiser.cu:489-490).
+ * transposed, buf[x*H + y] = picture[y][x], passed with m = image width W and
+ * n = image height H.  CudaKSvdDenoiser::createPatches reads it as n rows of
+ * length m (cuda_svd_denoiser.cu:231-240), which for W != H cuts sheared
+ * patches; KSVD_GEOM_COMPAT keeps that reading, KSVD_GEOM_TRUE (default)
+ * reads the buffer as the m columns of length n it holds, and the two agree
+ * for a square picture.  ksvd_rebuild returns the denoised buffer in the
+ * same layout (before the final transpose of cuda_svd_denoiser.cu:489-490).
  *
  * The functions below replace, one for one, the stages of the reference's
  * CUDA_K_GESVDJ pipeline; the reference interface each one stands in for is
@@ -45,8 +47,12 @@ enum {
     KSVD_PREC_BF16 = 1      /* BF16 correlation on MFMA, FP32 re-score/LS/SVD */
 };
 enum {
-    KSVD_GEOM_TRUE = 0,     /* rebuild walks the gather's patch frame */
-    KSVD_GEOM_COMPAT = 1    /* rebuild walks the reference's swapped frame */
+    KSVD_GEOM_TRUE = 0,     /* patches are blocks of the picture: the buffer is read
+                               as the m columns of length n it holds, and the
+                               rebuild walks the same grid */
+    KSVD_GEOM_COMPAT = 1    /* the reference's walks: the buffer read as n rows of
+                               length m (sheared patches when W != H), the rebuild
+                               on its transposed grid for rectangular patches */
 };
 
 /* Parameters: the per-file keys of config.json (batch_denoiser.cpp:80-94,

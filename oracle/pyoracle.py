@@ -68,10 +68,22 @@ def patch_count(m, n, pw, ph, sw, sh):
     return lib().orc_patch_count(m, n, pw, ph, sw, sh)
 
 
-def extract(buf, m, n, pw, ph, sw, sh, real="f32"):
-    """Patches of the frame buffer (n rows of length m) -> (P, pw*ph)."""
+def frame_dims(m, n, compat=False):
+    """Row length and row count the patch walk uses for a frame buffer given as
+    (m = picture width W, n = height H; buf[x*H + y] = picture[y][x]).  The
+    reference's walk (compat) reads it as n rows of length m
+    (cuda_svd_denoiser.cu:231-239); the true geometry reads it as what it
+    holds, m rows of length n, so that every patch is a block of the picture.
+    For a square picture the two agree."""
+    return (m, n) if compat else (n, m)
+
+
+def extract(buf, m, n, pw, ph, sw, sh, real="f32", compat=False):
+    """Patches of the frame buffer (m = W, n = H) -> (P, pw*ph), in the true
+    geometry or the reference's (compat) walk."""
     buf = _c(buf, np.float32).ravel()
     assert buf.size == m * n
+    m, n = frame_dims(m, n, compat)
     P = patch_count(m, n, pw, ph, sw, sh)
     Y = np.empty((P, pw * ph), _dt(real))
     getattr(lib(), "orc_extract_" + real)(_ptr(buf), m, n, pw, ph, sw, sh, _ptr(Y))
@@ -130,7 +142,11 @@ def update(Y, D, idx, coef, cnt, cap=1024, real="f32", want_gap=False):
 
 
 def rebuild(idx, coef, cnt, D, buf, m, n, pw, ph, sw, sh, sigma2, compat=False, real="f32"):
-    """-> out, sum, weight (each (n, m)), dsum (double), lambda."""
+    """-> out, sum, weight (each (n, m), the buffer's own layout), dsum
+    (double), lambda.  compat: the reference's walks (its reading of the
+    frame and its transposed rebuild grid for rectangular patches)."""
+    n0, m0 = n, m
+    m, n = frame_dims(m, n, compat)
     idx = _c(idx, np.int32)
     coef = _c(coef, _dt(real))
     cnt = _c(cnt, np.int32)
@@ -145,7 +161,7 @@ def rebuild(idx, coef, cnt, D, buf, m, n, pw, ph, sw, sh, sigma2, compat=False, 
     getattr(lib(), "orc_rebuild_" + real)(_ptr(idx), _ptr(coef), _ptr(cnt), _ptr(D), dim, P, kmax, _ptr(buf), m, n,
                                           pw, ph, sw, sh, 1 if compat else 0, ctypes.c_float(sigma2), _ptr(out),
                                           _ptr(s), _ptr(w), _ptr(dl))
-    return out.reshape(n, m), s.reshape(n, m), w.reshape(n, m), dl[0], dl[1]
+    return out.reshape(n0, m0), s.reshape(n0, m0), w.reshape(n0, m0), dl[0], dl[1]
 
 
 def ksvd(Y, D, kmax, iters, cap=1024, final_omp=True, real="f32"):

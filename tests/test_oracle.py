@@ -42,11 +42,68 @@ def test_patch_count_known_sizes(orc):
 
 @pytest.mark.parametrize("m,n,pw,ph,sw,sh", [(16, 16, 8, 8, 2, 2), (37, 23, 5, 7, 3, 2), (9, 9, 3, 3, 1, 1)])
 def test_extract_index_sets(orc, m, n, pw, ph, sw, sh):
+    # the reference's walk of its frame (compat): n rows of length m
     frame = np.arange(m * n, dtype=np.float32).reshape(n, m)
-    Y = orc.extract(frame, m, n, pw, ph, sw, sh)
+    Y = orc.extract(frame, m, n, pw, ph, sw, sh, compat=True)
     np.testing.assert_array_equal(Y, brute_patches(frame, pw, ph, sw, sh))
-    Y64 = orc.extract(frame, m, n, pw, ph, sw, sh, real="f64")
+    Y64 = orc.extract(frame, m, n, pw, ph, sw, sh, real="f64", compat=True)
     np.testing.assert_array_equal(Y64, Y.astype(np.float64))
+
+
+def picture_blocks(pic, pw, ph, sw, sh):
+    """True geometry from the picture itself (H x W): patch (a, b) is the
+    block of columns x in [a, a + ph) and rows y in [b, b + pw), a stepping by
+    sh and b by sw, elements in (x, y) order -- for a square picture the
+    reference's patch of its transposed frame (cuda_svd_denoiser.cu:231-240,
+    denoiser.cpp:46)."""
+    H, W = pic.shape
+    out = []
+    for a in range(0, W - ph + 1, sh):
+        for b in range(0, H - pw + 1, sw):
+            out.append(pic[b:b + pw, a:a + ph].T.ravel())
+    return np.array(out)
+
+
+@pytest.mark.parametrize("W,H,pw,ph,sw,sh", [(16, 16, 8, 8, 2, 2), (37, 23, 5, 7, 3, 2), (96, 64, 12, 8, 3, 2),
+                                             (10, 30, 4, 6, 5, 1), (23, 37, 7, 5, 2, 3)])
+def test_extract_true_geometry_is_picture_blocks(orc, W, H, pw, ph, sw, sh):
+    pic = np.arange(W * H, dtype=np.float32).reshape(H, W)
+    frame = np.ascontiguousarray(pic.T).ravel()  # buf[x*H + y] = pic[y][x] (denoiser.cpp:46)
+    Y = orc.extract(frame, W, H, pw, ph, sw, sh)
+    np.testing.assert_array_equal(Y, picture_blocks(pic, pw, ph, sw, sh))
+    if W == H:
+        np.testing.assert_array_equal(Y, orc.extract(frame, W, H, pw, ph, sw, sh, compat=True))
+    elif pw * ph > 1:
+        assert not np.array_equal(Y[:1], orc.extract(frame, W, H, pw, ph, sw, sh, compat=True)[:1])
+
+
+def test_rebuild_true_geometry_overlap_add(orc):
+    """Overlap-add of the true geometry from the picture: every patch's values
+    go back onto its own block; sums and weights in picture coordinates."""
+    rng = np.random.default_rng(3)
+    W, H, pw, ph, sw, sh, K, k = 29, 17, 6, 4, 3, 2, 10, 3
+    pic = rng.uniform(0, 255, (H, W)).astype(np.float32)
+    frame = np.ascontiguousarray(pic.T).ravel()
+    P = orc.patch_count(H, W, pw, ph, sw, sh)
+    D = rng.standard_normal((K, pw * ph)).astype(np.float32)
+    idx = np.stack([rng.choice(K, k, replace=False) for _ in range(P)]).astype(np.int32)
+    coef = rng.uniform(-9, 9, (P, k)).astype(np.float32)
+    cnt = np.full(P, k, np.int32)
+    out, s, w, dsum, lam = orc.rebuild(idx, coef, cnt, D.astype(np.float64), frame, W, H, pw, ph, sw, sh, 50.0,
+                                       real="f64")
+    S = np.zeros((H, W))
+    Wt = np.zeros((H, W))
+    p = 0
+    for a in range(0, W - ph + 1, sh):
+        for b in range(0, H - pw + 1, sw):
+            v = sum(coef[p, i].astype(np.float64) * D[idx[p, i]].astype(np.float64) for i in range(k))
+            S[b:b + pw, a:a + ph] += v.reshape(ph, pw).T
+            Wt[b:b + pw, a:a + ph] += 1
+            p += 1
+    assert p == P
+    # the oracle's arrays are in the buffer's layout (row x of length H)
+    np.testing.assert_array_equal(w.reshape(W, H).T, Wt)
+    np.testing.assert_allclose(s.reshape(W, H).T, S, rtol=1e-12, atol=1e-9)
 
 
 # ---- dictionary init --------------------------------------------------------
@@ -204,7 +261,7 @@ def test_rebuild_matches_loop_golden(orc):
     g = np.load(os.path.join(GOLD, "rebuild_golden.npz"))
     m, n, pw, ph, sw, sh = [int(v) for v in g["geom"]]
     out, s, w, dsum, lam = orc.rebuild(g["idx"], g["coef"], g["cnt"], g["D"], g["buf"], m, n, pw, ph, sw, sh,
-                                       float(g["sigma2"]), real="f64")
+                                       float(g["sigma2"]), compat=True, real="f64")
     np.testing.assert_array_equal(w, g["weight"])
     np.testing.assert_allclose(s, g["sum"], rtol=1e-12, atol=1e-9)
     assert abs(lam - float(g["lam"])) < 1e-6 * max(1.0, float(g["lam"]))
@@ -218,7 +275,7 @@ def test_rebuild_f32_sums_close(orc):
     g = np.load(os.path.join(GOLD, "rebuild_golden.npz"))
     m, n, pw, ph, sw, sh = [int(v) for v in g["geom"]]
     out, s, w, dsum, lam = orc.rebuild(g["idx"], g["coef"], g["cnt"], g["D"], g["buf"], m, n, pw, ph, sw, sh,
-                                       float(g["sigma2"]), real="f32")
+                                       float(g["sigma2"]), compat=True, real="f32")
     np.testing.assert_array_equal(w, g["weight"])
     np.testing.assert_allclose(s, g["sum"], rtol=1e-5, atol=1e-3)
     np.testing.assert_allclose(out, g["out"], rtol=1e-4, atol=1e-3)
