@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "baseUtilityLib.h"
+#include "ksvd_hip.h"
 
 namespace img {
 struct Image;
@@ -124,11 +125,18 @@ private:
     DenoiserType type = HIP_K_SVD;
     double patchIters = 0;
     long long nPatches = 0;
+    double iterMs = 0, patchIterCount = 0;
 
     HipKSvdDenoiser();
-    // One frame buffer (n rows of length m) through the whole K-SVD path;
-    // out receives the denoised buffer in the same frame.
-    bool denoiseFrame(const float *buf, int m, int n, float noiseVar, std::vector<float> &out);
+    // A context with this denoiser's parameters (nullptr and error on failure).
+    ksvd_ctx *makeContext();
+    // One frame buffer (m = W, n = H) through the whole K-SVD path on ctx;
+    // out receives the denoised buffer in the same layout.
+    bool denoiseFrame(ksvd_ctx *ctx, const float *buf, int m, int n, float noiseVar, std::vector<float> &out);
+    // The sub-image path (cuda_svd_denoiser.cu:130-188, 503-528): the frame cut
+    // into subImageHeightDim x subImageWidthDim tiles, each denoised on its own
+    // with its own noise variance, and put back in place.
+    bool denoiseTiles(const float *buf, int W, int H, std::vector<float> &out);
 
     friend Denoiser;
     friend BatchDenoiser;

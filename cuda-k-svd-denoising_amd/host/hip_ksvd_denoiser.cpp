@@ -29,7 +29,7 @@ bool HipKSvdDenoiser::loadImage() { return Denoiser::loadImage(); }
 
 bool HipKSvdDenoiser::saveImage() { return Denoiser::saveImage(); }
 
-bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVar, std::vector<float> &out)
+ksvd_ctx *HipKSvdDenoiser::makeContext()
 {
     ksvd_params p;
     ksvd_default_params(&p);
@@ -47,13 +47,23 @@ bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVa
     p.precision = (type == HIP_K_SVD_BF16 || type == HIP_K_SVD_BF16_LEARNED) ? KSVD_PREC_BF16 : KSVD_PREC_FP32;
     p.proposer = type == HIP_K_SVD_BF16_LEARNED ? 1 : 0;
     ksvd_ctx *ctx = nullptr;
-    int rc = ksvd_create(&p, &ctx);
+    const int rc = ksvd_create(&p, &ctx);
+    if (rc) {
+        error = std::string("HIP backend: ") + ksvd_last_error(ctx) + " (status " + std::to_string(rc) + ")";
+        ksvd_destroy(ctx);
+        return nullptr;
+    }
+    return ctx;
+}
+
+bool HipKSvdDenoiser::denoiseFrame(ksvd_ctx *ctx, const float *buf, int m, int n, float noiseVar,
+                                   std::vector<float> &out)
+{
+    int rc;
     auto fail = [&](int code) {
         error = std::string("HIP backend: ") + ksvd_last_error(ctx) + " (status " + std::to_string(code) + ")";
-        ksvd_destroy(ctx);
         return false;
     };
-    if (rc) return fail(rc);
     if ((rc = ksvd_set_image(ctx, buf, m, n))) return fail(rc);
     if ((rc = ksvd_extract_patches(ctx))) return fail(rc);
     if ((rc = ksvd_init_dictionary(ctx))) return fail(rc);
@@ -65,13 +75,61 @@ bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVa
     int dim = 0, K = 0, kmax = 0;
     int64_t P = 0;
     ksvd_get_dims(ctx, &dim, &P, &K, &kmax);
-    nPatches = P;
-    const double iter_ms = t.omp_ms + t.update_ms;
-    patchIters = iter_ms > 0 ? (double)P * iter / (iter_ms * 1e-3) : 0;
+    nPatches += P;
+    iterMs += t.omp_ms + t.update_ms;
+    patchIterCount += (double)P * iter;
     // the final sparse code: OMP, or the trained proposer with OMP fallback
     if ((rc = ksvd_run_final_code(ctx))) return fail(rc);
     out.assign((size_t)m * n, 0.f);
     if ((rc = ksvd_rebuild(ctx, noiseVar, out.data()))) return fail(rc);
+    return true;
+}
+
+bool HipKSvdDenoiser::denoiseTiles(const float *buf, int W, int H, std::vector<float> &out)
+{
+    // The reference cuts the tiles as it cuts patches (createPatches with the
+    // patch and sliding sizes set to the sub-image sizes), in the frame as
+    // the geometry reads it: rows of length fm, fn rows.
+    const bool compat = geometry == KSVD_GEOM_COMPAT;
+    const int fm = compat ? W : H, fn = compat ? H : W;
+    const int th = subImageHeightDim, tw = subImageWidthDim;
+    if (th <= 0 || tw <= 0 || th > fn || tw > fm) {
+        error = "sub-image larger than the image (or not positive)";
+        return false;
+    }
+    const int Tr = fn / th, Tc = fm / tw;
+    // Pixels no tile covers: 0 in the reference's output (its rebuilt image
+    // starts at 0, :505); the true geometry keeps them as they came in.
+    out.assign((size_t)W * H, 0.f);
+    if (!compat) {
+        out.assign(buf, buf + (size_t)W * H);
+        if (speckle) ksvd_speckle_exp(out.data(), (int64_t)out.size());
+    }
+    ksvd_ctx *ctx = makeContext();
+    if (!ctx) return false;
+    std::vector<float> tile((size_t)th * tw), den;
+    for (int ti = 0; ti < Tr; ti++)
+        for (int tj = 0; tj < Tc; tj++) {
+            for (int r = 0; r < th; r++)
+                std::copy(buf + (size_t)(ti * th + r) * fm + (size_t)tj * tw,
+                          buf + (size_t)(ti * th + r) * fm + (size_t)tj * tw + tw, tile.data() + (size_t)r * tw);
+            // Matrix(subImageHeightDim, subImageWidthDim, ...) over the tile
+            // (cuda_svd_denoiser.cu:152) and its variance_noise as a
+            // subImageHeightDim-wide CImg (:146,:158): the reference reads the
+            // tile as rows of length th; the true geometry as the rows of
+            // length tw it holds
+            const float s2 = compat ? (float)ksvd_noise_variance(tile.data(), th, tw)
+                                    : (float)ksvd_noise_variance(tile.data(), tw, th);
+            if (!denoiseFrame(ctx, tile.data(), th, tw, s2, den)) {
+                ksvd_destroy(ctx);
+                return false;
+            }
+            if (speckle) ksvd_speckle_exp(den.data(), (int64_t)den.size());  // createImage per tile (:483-484)
+            // createImageFromSubImages (:503-528): each tile back in its place
+            for (int r = 0; r < th; r++)
+                std::copy(den.data() + (size_t)r * tw, den.data() + (size_t)r * tw + tw,
+                          out.data() + (size_t)(ti * th + r) * fm + (size_t)tj * tw);
+        }
     ksvd_destroy(ctx);
     return true;
 }
@@ -79,16 +137,23 @@ bool HipKSvdDenoiser::denoiseFrame(const float *buf, int m, int n, float noiseVa
 bool HipKSvdDenoiser::internalDenoising()
 {
     auto t0 = std::chrono::steady_clock::now();
-    if (subImageWidthDim != 0 && subImageHeightDim != 0) {
-        error = "sub-image tiling (subImageWidthDim / subImageHeightDim) is not built in the HIP backend; "
-                "set both to 0 (the whole image fits in HBM)";
-        return false;
-    }
     const int W = inputMatrix->m, H = inputMatrix->n;
     std::vector<float> out;
-    if (!denoiseFrame(inputMatrix->hostVector->data(), W, H, sigma, out)) return false;
-    // speckle: back from the log domain (myExp, denoisingLib.h:135-142)
-    if (speckle) ksvd_speckle_exp(out.data(), (int64_t)out.size());
+    nPatches = 0;
+    iterMs = 0;
+    patchIterCount = 0;
+    if (subImageWidthDim != 0 && subImageHeightDim != 0) {
+        if (!denoiseTiles(inputMatrix->hostVector->data(), W, H, out)) return false;
+    } else {
+        ksvd_ctx *ctx = makeContext();
+        if (!ctx) return false;
+        const bool ok = denoiseFrame(ctx, inputMatrix->hostVector->data(), W, H, sigma, out);
+        ksvd_destroy(ctx);
+        if (!ok) return false;
+        // speckle: back from the log domain (myExp, denoisingLib.h:135-142)
+        if (speckle) ksvd_speckle_exp(out.data(), (int64_t)out.size());
+    }
+    patchIters = iterMs > 0 ? patchIterCount / (iterMs * 1e-3) : 0;
     // back to the picture layout (the transpose of cuda_svd_denoiser.cu:489-490)
     std::vector<float> *pic = new std::vector<float>((size_t)W * H);
     ksvd_frame_to_image(out.data(), W, H, pic->data());

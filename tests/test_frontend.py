@@ -96,13 +96,51 @@ def test_missing_image_reports_load_failure(denoising_bin, tmp_path):
     assert r.returncode == 2 and "Image Loading Failed" in r.stdout
 
 
-def test_subimage_tiling_reports_denoise_failure(denoising_bin, tmp_path, kl):
-    clean, noisy = kl.synth_image(1, 64, 64, 10.0)
-    write_pfm(tmp_path / "a.pfm", noisy)
-    cfg = {"inputFolder": str(tmp_path), "outputFolder": str(tmp_path),
-           "globalParams": dict(GLOBAL, subImageWidthDim=32, subImageHeightDim=32), "files": [{"name": "a.pfm"}]}
-    r = run(denoising_bin, cfg, tmp_path)
-    assert r.returncode == 2 and "Denoising Failed" in r.stdout
+def tiles_cut(frame, fm, fn, th, tw):
+    """Tiles of a frame (fn rows of length fm), th rows x tw columns each,
+    rows of tiles outer (createPatches with patch = slide = the sub-image
+    size, cuda_svd_denoiser.cu:130-140,231-240)."""
+    f = np.asarray(frame, np.float32).reshape(fn, fm)
+    return [f[ti * th:(ti + 1) * th, tj * tw:(tj + 1) * tw].copy()
+            for ti in range(fn // th) for tj in range(fm // tw)]
+
+
+def tiles_join_reference(tiles, fm, fn, th, tw):
+    """createImageFromSubImages (cuda_svd_denoiser.cu:503-528) written out as
+    the reference's loops: stride = patchIdx * tile size, stride2 = tiles
+    per row * tw, rows of the output of length stride2."""
+    img = np.zeros(fm * fn, np.float32)
+    flat = np.concatenate([t.ravel() for t in tiles])
+    per_row = fm // tw
+    tsz = th * tw
+    for patch_idx in range(0, len(tiles), per_row):
+        stride = patch_idx * tsz
+        for i in range(per_row):
+            stride2 = per_row * tw
+            for j in range(th):
+                start = tsz * (i + patch_idx) + j * tw
+                o = stride + stride2 * j + i * tw
+                img[o:o + tw] = flat[start:start + tw]
+    return img
+
+
+def test_tile_join_is_the_reference_loop():
+    """The block placement the front end uses equals the reference's loop
+    where the frame's sides are multiples of the tile's (the case its loop
+    lays out rows of the frame's own length)."""
+    rng = np.random.default_rng(2)
+    fm, fn, th, tw = 48, 36, 12, 16
+    tiles = [rng.standard_normal((th, tw)).astype(np.float32) for _ in range((fn // th) * (fm // tw))]
+    out = np.zeros((fn, fm), np.float32)
+    k = 0
+    for ti in range(fn // th):
+        for tj in range(fm // tw):
+            out[ti * th:(ti + 1) * th, tj * tw:(tj + 1) * tw] = tiles[k]
+            k += 1
+    np.testing.assert_array_equal(out.ravel(), tiles_join_reference(tiles, fm, fn, th, tw))
+    frame = rng.standard_normal(fm * fn).astype(np.float32)
+    back = tiles_join_reference(tiles_cut(frame, fm, fn, th, tw), fm, fn, th, tw)
+    np.testing.assert_array_equal(back, frame)
 
 
 @pytest.mark.gpu
@@ -226,3 +264,49 @@ def test_cli_colour_speckle_non_square(denoising_bin, tmp_path, kl):
           f"output range [{out.min():.3f}, {out.max():.3f}]")
     assert out.shape == (H, W)
     np.testing.assert_array_equal(out, expect)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H,th,tw,geom", [(96, 64, 24, 32, "true"), (64, 64, 32, 32, "compat"),
+                                            (100, 70, 30, 24, "true")],
+                         ids=["96x64_true", "64x64_compat", "100x70_true_ragged"])
+def test_cli_subimage_tiling(denoising_bin, tmp_path, kl, W, H, th, tw, geom):
+    """subImageHeightDim x subImageWidthDim tiles (cuda_svd_denoiser.cu:130-188,
+    503-528): each tile denoised on its own with its own noise variance and
+    put back in place; the front end's output must be the bits of those
+    steps taken through the C ABI tile by tile.  In the true geometry the
+    pixels no tile covers keep their input (the reference leaves 0 there)."""
+    clean, noisy = kl.synth_image(31, W, H, 18.0)
+    write_pfm(tmp_path / "n.pfm", noisy)
+    g = dict(GLOBAL, **{"subImageWidthDim": tw, "subImageHeightDim": th, "ksvdIter": 2, "atoms": 32,
+                        "geometry": geom})
+    cfg = {"inputFolder": str(tmp_path), "outputFolder": str(tmp_path), "globalParams": g,
+           "files": [{"name": "n.pfm"}]}
+    r = run(denoising_bin, cfg, tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = read_pfm(tmp_path / "8_8_n.pfm")
+    compat = geom == "compat"
+    frame = kl.image_to_frame(noisy)
+    fm, fn = (W, H) if compat else (H, W)
+    tiles = tiles_cut(frame, fm, fn, th, tw)
+    res = np.zeros((fn, fm), np.float32) if compat else frame.reshape(fn, fm).copy()
+    ks = kl.KSvd(patch=(8, 8), slide=(2, 2), atoms=32, omp_iter=5, ksvd_iter=2,
+                 geometry=kl.GEOM_COMPAT if compat else kl.GEOM_TRUE)
+    k = 0
+    for ti in range(fn // th):
+        for tj in range(fm // tw):
+            t = tiles[k]
+            k += 1
+            # the tile's noise variance: read as rows of length th (reference) or tw (true)
+            s2 = np.float32(kl.noise_variance(t.reshape(tw, th) if compat else t))
+            ks.set_frame(t.ravel(), th, tw)
+            ks.extract()
+            ks.init_dictionary()
+            ks.run()
+            res[ti * th:(ti + 1) * th, tj * tw:(tj + 1) * tw] = ks.rebuild(s2).reshape(th, tw)
+    ks.close()
+    expect = kl.normalize(kl.frame_to_image(res.ravel(), W, H))
+    gain = kl.psnr(clean, kl.frame_to_image(res.ravel(), W, H)) - kl.psnr(clean, noisy)
+    print(f"\ntiles {W}x{H} / {th}x{tw} {geom}: {k} tiles, raw PSNR gain {gain:.2f} dB")
+    np.testing.assert_array_equal(out, expect)
+    assert gain > 1.0
