@@ -118,9 +118,6 @@ int check_params(ksvd_ctx *ctx, const ksvd_params *p)
     if (p->user_cap < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "user_cap must be >= 0");
     const int dim = p->patch_w * p->patch_h;
     if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
-    // the rank-1 solve keeps the dim x dim Gram of E in LDS
-    if ((size_t)dim * dim * 4 + (size_t)(dim + 1024) * 4 > 160 * 1024)
-        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 190 not supported by the rank-1 update");
     if (p->precision != KSVD_PREC_FP32 && p->precision != KSVD_PREC_BF16)
         return ksvd_fail(ctx, KSVD_ERR_ARG, "unknown precision");
     if (p->precision == KSVD_PREC_BF16 && (dim > 144 || p->atoms > 256 || p->omp_iter > 8))

@@ -1618,6 +1618,100 @@ __global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw,
     if (i < nw) wx[i] = 0ull;
 }
 
+// ===========================================================================
+// Large patches (padded dim above 192, up to 1024): two kernels per atom with
+// E_k in global memory (pitch dim) and no Gram matrix, for the patch sizes
+// the LDS-resident sweeps cannot hold (22 x 22 patches: dim 484).
+//   kl_build_kernel(k), B1 workgroups: E_k's capped columns, e_t = y_p -
+//     sum_{j != k} d_j x_{j,p} in the column table's entry order, every
+//     coefficient read from memory (atom k-1's were written, final, by
+//     kl_power_kernel(k-1); users beyond an updated atom's cap were zeroed
+//     there too);
+//   kl_power_kernel(k), one workgroup: power iteration z = E_k (E_k^T u),
+//     u = z / |z| from d_k, the stop rule of the other sweeps; then d_k = u,
+//     x_t = e_t . u for the first m users and 0 for the others.
+constexpr int KL_T = 1024;
+
+__global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, int k)
+{
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    const int dim = a.dim, kmax = a.kmax;
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    const int cpt = KL_T / dim;  // columns per pass (dim <= 1024)
+    const int tc = threadIdx.x / dim, te = threadIdx.x - tc * dim;
+    if (tc >= cpt) return;
+    const int64_t cb = a.moff[k];
+    for (int t = blockIdx.x * cpt + tc; t < m; t += gridDim.x * cpt) {
+        const int64_t col = cb + t;
+        const int p = a.tab_p[col];
+        float v = a.Y[(int64_t)p * dim + te];
+        for (int i = 0; i < no; i++) {
+            const int4 o = a.tab_oth[col * no + i];
+            if (o.x >= 0) v = fmaf(-a.D[(int64_t)o.x * dim + te], a.coef[o.y], v);
+        }
+        E[(int64_t)t * dim + te] = v;
+    }
+}
+
+// e_t . u by one wavefront (lane-strided over dim, fixed butterfly).
+__device__ __forceinline__ float kl_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
+{
+    float s = 0.f;
+    for (int q = threadIdx.x & 63; q < dim; q += 64) s = fmaf(e[q], u[q], s);
+    return wave_sum_dpp(s);
+}
+
+__global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float *__restrict__ E, float *vbuf, int k)
+{
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float red[KL_T / 64];
+    __shared__ int s_stop;
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    const int dim = a.dim, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t cb = a.moff[k];
+    if (m > 0) {
+        if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
+        __syncthreads();
+        int it = 0;
+        for (;;) {
+            // v = E^T u
+            for (int t = wave; t < m; t += KL_T / 64) {
+                const float x = kl_dot(E + (int64_t)t * dim, u, dim);
+                if (lane == 0) vbuf[t] = x;
+            }
+            __threadfence_block();
+            __syncthreads();
+            // z = E v, ascending column order
+            float z = 0.f;
+            if (tid < dim)
+                for (int t = 0; t < m; t++) z = fmaf(E[(int64_t)t * dim + tid], vbuf[t], z);
+            const float n2 = block_sum_t2(tid < dim ? z * z : 0.f, red);
+            if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
+            const float un = z / sqrtf(n2);
+            const float du = tid < dim ? un - u[tid] : 0.f;
+            const float d2 = block_sum_t2(du * du, red);
+            it++;
+            if (tid == 0) s_stop = (it >= 2 && d2 < a.tol2) || it >= a.maxit;
+            __syncthreads();
+            if (tid < dim) u[tid] = un;
+            __syncthreads();
+            if (s_stop) break;
+        }
+        if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
+        if (tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
+        // the new coefficients of the first m users
+        for (int t = wave; t < m; t += KL_T / 64) {
+            const float x = kl_dot(E + (int64_t)t * dim, u, dim);
+            if (lane == 0) a.coef[a.tab_self[cb + t]] = x;
+        }
+    }
+    // users beyond the cap: coefficient 0
+    const int base = a.uoff[k];
+    for (int t = m + tid; t < c; t += KL_T) a.coef[(int64_t)a.users[base + t] * a.kmax + a.uslot[base + t]] = 0.f;
+}
+
 }  // namespace
 
 static bool env_flag(const char *name)
@@ -1649,7 +1743,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const int dpad = (dim + 15) / 16 * 16;
     const int nbt = dpad / 16;
     const int ntiles = nbt * (nbt + 1) / 2;
-    if (ntiles > 16 * MAX_TPW) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension too large for the sweep");
+    const bool large = ntiles > 16 * MAX_TPW;  // dpad > 192: the LDS-resident sweeps cannot hold it
     SweepArgs a;
     a.Y = (const float *)ctx->Y.ptr;
     a.D = (float *)ctx->D.ptr;
@@ -1679,6 +1773,19 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const int tpw = (ntiles + 15) / 16;
     const bool wide = kmax > 5;
 
+    // ---- large patches: E in global memory, two kernels per atom -------------
+    auto launch_large = [&]() -> int {
+        if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
+        float *E = (float *)ctx->E.ptr;   // m x dim, m <= min(users, cap): the E buffer's columns
+        float *vbuf = (float *)ctx->E2.ptr;  // E^T u, one value per column
+        for (int k = 0; k < K; k++) {
+            hipLaunchKernelGGL(kl_build_kernel, dim3(B1 * 4), dim3(KL_T), 0, ctx->stream, a, E, k);
+            hipLaunchKernelGGL(kl_power_kernel, dim3(1), dim3(KL_T), 0, ctx->stream, a, (const float *)E, vbuf, k);
+        }
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        return KSVD_OK;
+    };
+    if (large) return launch_large();
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL")) {
@@ -1736,6 +1843,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     // ---- two kernels per atom (any cap) --------------------------------------
     const size_t lds1 = sizeof(float) * (size_t)CHUNK * dpad;
     const size_t lds2 = sizeof(float) * ((size_t)dim * dim + dim) + sizeof(int) * HAND_MAX;
+    // the Gram of E must fit K2's LDS; beyond that, the Gram-free pair
+    if (lds2 > 160 * 1024) return launch_large();
     typedef void (*K1Fn)(SweepArgs, int);
     K1Fn k1 = nullptr;
     if (!wide) k1 = tpw <= 1 ? k1_kernel<1, 4, 4> : tpw <= 2 ? k1_kernel<2, 4, 3> : tpw <= 3 ? k1_kernel<3, 4, 3>

@@ -1,0 +1,99 @@
+"""Large and rectangular patches: 16 x 12 (dim 192, the LDS-resident
+two-team sweep at its widest) and 22 x 22 (dim 484: the generic pursuit and
+the Gram-free sweep with E in global memory), the patch sizes of the batched
+workload's files and per-file overrides.
+
+Bars, against the oracle on the same seeded inputs:
+  * OMP: identical supports on >= 99.9 % of patches, coefficients within
+    1e-4 of the FP64 oracle's;
+  * one sweep from the oracle's codes: sign-fixed atoms within 1e-4 where
+    lambda2/lambda1 < 0.9, and for every atom the gap-independent check of
+    test_gpu_e2e (||E_k^T d_k||^2 >= (1 - 1e-6) sigma_1^2, coefficients
+    equal to E_k^T d_k);
+  * three K-SVD iterations end to end: raw PSNR within 0.05 dB of the FP32
+    oracle (the FP64 oracle's distance printed beside it).
+"""
+import numpy as np
+import pytest
+
+from test_gpu_e2e import compare_codes, signfix, sweep_self_check
+
+pytestmark = pytest.mark.gpu
+
+CASES = [dict(W=256, H=192, patch=(16, 12), slide=(3, 3), atoms=256, k=5, seed=41, sigma=20.0),
+         dict(W=256, H=192, patch=(22, 22), slide=(3, 3), atoms=256, k=5, seed=42, sigma=25.0)]
+IDS = ["16x12_dim192", "22x22_dim484"]
+
+
+def context(kl, cfg, **kw):
+    clean, noisy = kl.synth_image(cfg["seed"], cfg["W"], cfg["H"], cfg["sigma"])
+    ks = kl.KSvd(patch=cfg["patch"], slide=cfg["slide"], atoms=cfg["atoms"], omp_iter=cfg["k"], ksvd_iter=3, **kw)
+    ks.set_picture(noisy)
+    ks.extract()
+    ks.init_dictionary()
+    return clean, noisy, ks
+
+
+@pytest.mark.parametrize("cfg", CASES, ids=IDS)
+def test_large_patch_omp(kl, orc, cfg):
+    clean, noisy, ks = context(kl, cfg)
+    ks.omp()
+    g = ks.codes()
+    Y, D = ks.patches(), ks.dictionary()
+    ks.close()
+    o32 = orc.omp(Y, D, cfg["k"], real="f32")[:3]
+    o64 = orc.omp(Y, D.astype(np.float64), cfg["k"], real="f64")[:3]
+    frac, err = compare_codes(g, o32)
+    frac64, err64 = compare_codes(g, o64)
+    print(f"\nOMP {cfg['patch']} (dim {Y.shape[1]}, {len(Y)} patches): supports equal {frac * 100:.4f}% "
+          f"(coef err {err:.2e}); FP64 oracle {frac64 * 100:.4f}% (coef err {err64:.2e})")
+    assert frac >= 0.999
+    assert err64 <= 1e-4
+
+
+@pytest.mark.parametrize("cfg", CASES, ids=IDS)
+def test_large_patch_sweep(kl, orc, cfg):
+    clean, noisy, ks = context(kl, cfg)
+    Y, D0 = ks.patches(), ks.dictionary()
+    io, co, no, _ = orc.omp(Y, D0, cfg["k"], real="f32")
+    ks.set_codes(io, co, no)
+    ks.update()
+    Dg = ks.dictionary()
+    cg = ks.codes()[1]
+    ks.close()
+    Do, coo, gap, _ = orc.update(Y, D0, io, co, no, cap=1024, real="f32", want_gap=True)
+    good = (gap >= 0) & (gap < 0.9)
+    Dgs, _ = signfix(Dg)
+    Dos, _ = signfix(Do)
+    aerr = np.max(np.abs(Dgs - Dos), axis=1)
+    chk = sweep_self_check(Y, D0, io, co, no, Dg, cg, 1024)
+    print(f"\nsweep {cfg['patch']}: {good.sum()} of {(gap >= 0).sum()} atoms with gap < 0.9, atom err "
+          f"{aerr[good].max():.2e}; all atoms: Rayleigh deficit {chk[:, 0].max():.2e}, coef vs E^T d "
+          f"{chk[:, 1].max():.2e}, |1 - norm| {chk[:, 2].max():.2e}")
+    assert aerr[good].max() <= 1e-4
+    assert chk[:, 0].max() <= 1e-6
+    assert chk[:, 1].max() <= 1e-4
+    assert chk[:, 2].max() <= 1e-5
+
+
+@pytest.mark.parametrize("cfg", CASES, ids=IDS)
+def test_large_patch_end_to_end(kl, orc, cfg):
+    clean, noisy, ks = context(kl, cfg)
+    ks.run()
+    s2 = kl.noise_variance(noisy)
+    img = ks.rebuild_picture(s2)
+    ks.close()
+    W, H = cfg["W"], cfg["H"]
+    (pw, ph), (sw, sh) = cfg["patch"], cfg["slide"]
+    fr = kl.image_to_frame(noisy)
+    po = {}
+    for real in ("f32", "f64"):
+        Y = orc.extract(fr, W, H, pw, ph, sw, sh, real=real)
+        D, idx, coef, cnt, _ = orc.ksvd(Y, orc.init_dict(Y, cfg["atoms"], real=real), cfg["k"], 3, real=real)
+        out, *_ = orc.rebuild(idx, coef, cnt, D, fr, W, H, pw, ph, sw, sh, s2, real=real)
+        po[real] = kl.psnr(clean, kl.frame_to_image(out.astype(np.float32), W, H))
+    pg = kl.psnr(clean, img)
+    print(f"\nend to end {cfg['patch']}: raw PSNR gpu {pg:.4f}, FP32 oracle {po['f32']:.4f}, FP64 oracle "
+          f"{po['f64']:.4f} (noisy {kl.psnr(clean, noisy):.4f})")
+    assert abs(pg - po["f32"]) <= 0.05
+    assert pg > kl.psnr(clean, noisy) + 2

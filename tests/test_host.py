@@ -166,7 +166,7 @@ def test_invalid_params_fail_with_message(kl):
         kl.KSvd(omp_iter=40)
     assert e.value.code == kl.KSVD_ERR_ARG
     with pytest.raises(kl.KsvdError) as e:
-        kl.KSvd(patch=(22, 22))
+        kl.KSvd(patch=(33, 32))
     assert e.value.code == kl.KSVD_ERR_UNSUPPORTED
 
 
