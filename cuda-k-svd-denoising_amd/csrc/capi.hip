@@ -268,7 +268,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
                       &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
                       &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
                       &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
-                      &ctx->mlp_fb, &ctx->mlp_cnt};
+                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt};
     for (DevBuf *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -472,12 +472,67 @@ static int update_dictionary_async(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
+// The state a persistent sweep that gives up must be undone to: the
+// dictionary and (codes = true) the compact codes, copied on the stream.
+static int take_snapshot(ksvd_ctx *ctx, bool codes)
+{
+    const size_t nd = sizeof(float) * (size_t)ctx->K * ctx->dim;
+    int rc = ksvd_reserve(ctx, ctx->snap_D, nd);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->snap_D.ptr, ctx->D.ptr, nd, hipMemcpyDeviceToDevice, ctx->stream));
+    if (codes) {
+        const size_t ni = sizeof(int) * (size_t)ctx->P * ctx->kmax, nc = sizeof(int) * (size_t)ctx->P;
+        if ((rc = ksvd_reserve(ctx, ctx->snap_idx, ni))) return rc;
+        if ((rc = ksvd_reserve(ctx, ctx->snap_coef, ni))) return rc;
+        if ((rc = ksvd_reserve(ctx, ctx->snap_cnt, nc))) return rc;
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->snap_idx.ptr, ctx->idx.ptr, ni, hipMemcpyDeviceToDevice, ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->snap_coef.ptr, ctx->coef.ptr, ni, hipMemcpyDeviceToDevice, ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->snap_cnt.ptr, ctx->cnt.ptr, nc, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return KSVD_OK;
+}
+
+static int restore_snapshot(ksvd_ctx *ctx, bool codes)
+{
+    const size_t nd = sizeof(float) * (size_t)ctx->K * ctx->dim;
+    KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->D.ptr, ctx->snap_D.ptr, nd, hipMemcpyDeviceToDevice, ctx->stream));
+    if (codes) {
+        const size_t ni = sizeof(int) * (size_t)ctx->P * ctx->kmax, nc = sizeof(int) * (size_t)ctx->P;
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->idx.ptr, ctx->snap_idx.ptr, ni, hipMemcpyDeviceToDevice, ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->coef.ptr, ctx->snap_coef.ptr, ni, hipMemcpyDeviceToDevice, ctx->stream));
+        KSVD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->cnt.ptr, ctx->snap_cnt.ptr, nc, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->gram_valid = false;
+    ctx->cap_zero_pending = false;
+    return KSVD_OK;
+}
+
 int ksvd_update_dictionary(ksvd_ctx *ctx)
 {
     if (!ctx) return KSVD_ERR_ARG;
-    int rc = update_dictionary_async(ctx);
+    if (!ctx->has_codes || !ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "codes and dictionary required");
+    int rc = set_device(ctx);
     if (rc) return rc;
-    return check_sweep(ctx);
+    // the codes as a reader sees them (the last sweep's clearing done), then
+    // the state a failed persistent sweep is undone to
+    if ((rc = flush_cap_zero(ctx))) return rc;
+    if ((rc = take_snapshot(ctx, true))) return rc;
+    if ((rc = update_dictionary_async(ctx))) return rc;
+    rc = check_sweep(ctx);
+    if (rc && ctx->sweep_failed) {
+        // a persistent sweep gave up: undo it and sweep again with two
+        // kernels per atom, which wait on nothing
+        ctx->sweep_failed = false;
+        ctx->err.clear();
+        if ((rc = restore_snapshot(ctx, true))) return rc;
+        ctx->force_two_kernel = true;
+        rc = update_dictionary_async(ctx);
+        ctx->force_two_kernel = false;
+        if (rc) return rc;
+        ctx->tm.sweep_fallbacks++;
+        return check_sweep(ctx);
+    }
+    return rc;
 }
 
 // Learned form schedule: the first iteration on a dictionary runs OMP on
@@ -553,10 +608,8 @@ static int proposer_step(ksvd_ctx *ctx)
     return span_end(ctx, &s);
 }
 
-int ksvd_iterate(ksvd_ctx *ctx, int iters)
+static int iterate_async(ksvd_ctx *ctx, int iters)
 {
-    if (!ctx) return KSVD_ERR_ARG;
-    if (iters < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "iters must be >= 0");
     for (int i = 0; i < iters; i++) {
         int rc = ctx->prm.proposer ? proposer_step(ctx) : ksvd_omp(ctx);
         if (rc) return rc;
@@ -564,7 +617,38 @@ int ksvd_iterate(ksvd_ctx *ctx, int iters)
         if (rc) return rc;
         ctx->iters_done++;
     }
-    return iters > 0 ? check_sweep(ctx) : KSVD_OK;
+    return KSVD_OK;
+}
+
+int ksvd_iterate(ksvd_ctx *ctx, int iters)
+{
+    if (!ctx) return KSVD_ERR_ARG;
+    if (iters < 0) return ksvd_fail(ctx, KSVD_ERR_ARG, "iters must be >= 0");
+    if (iters == 0) return KSVD_OK;
+    if (!ctx->has_dict) return ksvd_fail(ctx, KSVD_ERR_STATE, "dictionary required");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // the iterations' first OMP rewrites the codes: the dictionary is the
+    // state a failed persistent sweep is undone to (the sweeps are checked
+    // once, after the last iteration, so that the host never waits between
+    // them; a failure anywhere replays them all with the two-kernel sweep)
+    if ((rc = take_snapshot(ctx, false))) return rc;
+    const int64_t done0 = ctx->iters_done;
+    if ((rc = iterate_async(ctx, iters))) return rc;
+    rc = check_sweep(ctx);
+    if (rc && ctx->sweep_failed) {
+        ctx->sweep_failed = false;
+        ctx->err.clear();
+        if ((rc = restore_snapshot(ctx, false))) return rc;
+        ctx->iters_done = done0;
+        ctx->force_two_kernel = true;
+        rc = iterate_async(ctx, iters);
+        ctx->force_two_kernel = false;
+        if (rc) return rc;
+        ctx->tm.sweep_fallbacks++;
+        return check_sweep(ctx);
+    }
+    return rc;
 }
 
 int ksvd_run_final_code(ksvd_ctx *ctx)

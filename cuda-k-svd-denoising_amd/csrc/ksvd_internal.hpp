@@ -79,6 +79,11 @@ struct ksvd_ctx {
     // to one clearing pass; the pass is run only before something reads the
     // codes (they are rewritten by the next OMP otherwise)
     bool sweep_defers_zero = false, cap_zero_pending = false;
+    // A persistent sweep that gave up (check_sweep sets sweep_failed) is
+    // undone from the snapshots and run again with the two-kernel sweep
+    // (force_two_kernel, launched eagerly, not from the captured graph)
+    bool sweep_failed = false, force_two_kernel = false;
+    DevBuf snap_D, snap_idx, snap_coef, snap_cnt;
     // rebuild: the support stride whose tiled kernel this context has given
     // its LDS ceiling (0: none yet)
     size_t rebuild_lds_set = 0;

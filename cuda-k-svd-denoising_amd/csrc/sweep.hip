@@ -580,7 +580,7 @@ constexpr int PS_HMAX = 31;             // helpers at most (one master + 31 help
 constexpr int PS_CWMAX = 64;            // columns per helper slice
 constexpr int PS_CPW = PS_CWMAX / PS_W; // columns per wavefront in the dot passes
 constexpr int PS_NOMAX = 7;             // other support entries per column (ompIter <= 8)
-constexpr long PS_SPIN_MAX = 1L << 21;  // polls before a wait gives up
+constexpr long PS_SPIN_MAX = 1L << 21;  // polls before a wait gives up (default)
 // flag words: per-XCD registration; per-helper done, rdone, cdone; the
 // master's aflag; the abort word and its sticky copy
 enum {
@@ -594,6 +594,7 @@ enum {
     PSF_XSEL = 131,      // the XCD the sweep runs on, plus 1 (0: none chosen yet)
     PSF_STARTED = 132,   // roles launched, summed over sweeps (kept across resets)
     PSF_FINISHED = 133,  // roles that completed, summed over sweeps (kept across resets)
+    PSF_SPIN = 134,      // polls before a wait gives up (set by the reset kernel)
     PSF_WORDS = 160
 };
 
@@ -612,6 +613,10 @@ __device__ __forceinline__ unsigned ld_flag(const unsigned *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// The polls a wait may spend before it gives up (PS_SPIN_MAX unless a test
+// asks for fewer).
+__device__ __forceinline__ long spin_limit(const unsigned *flags) { return (long)ld_flag(flags + PSF_SPIN); }
 
 // Flag and exchange stores are workgroup-scope atomics: single stores that
 // reach the XCD's L2 (the L1 is write-through), where the persistent sweeps'
@@ -684,7 +689,7 @@ __device__ bool ps_wait3(unsigned *flags, int f0, int n0, unsigned v0, int f1, i
                     *bail = 1;
                     break;
                 }
-                if (spins > PS_SPIN_MAX) {
+                if (spins > spin_limit(flags)) {
                     st_flag(flags + PSF_STICKY, 1u);
                     st_flag(flags + PSF_ABORT, 1u);
                     *bail = 1;
@@ -1321,7 +1326,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                             *bail = 1;
                             break;
                         }
-                        if (spins > PS_SPIN_MAX / 2) {
+                        if (spins > spin_limit(flags) / 2) {
                             st_flag(flags + PSF_STICKY, 1u);
                             st_flag(flags + PSF_ABORT, 1u);
                             *bail = 1;
@@ -1445,7 +1450,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                             *bail = 1;
                             break;
                         }
-                        if (spins > PS_SPIN_MAX) {
+                        if (spins > spin_limit(flags)) {
                             st_flag(flags + PSF_STICKY, 1u);
                             st_flag(flags + PSF_ABORT, 1u);
                             *bail = 1;
@@ -1610,10 +1615,10 @@ __global__ __launch_bounds__(PT_T) void pt_sweep_kernel(SweepArgs a, unsigned *f
 // Clears the sweep's flags (all but the sticky abort word and the role
 // counts) and the nw exchange pairs before a launch, and adds the launch's
 // nroles roles to the started count.
-__global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw, unsigned nroles)
+__global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw, unsigned nroles, unsigned spin)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < PSF_WORDS && i != PSF_STICKY && i != PSF_STARTED && i != PSF_FINISHED) flags[i] = 0u;
+    if (i < PSF_WORDS && i != PSF_STICKY && i != PSF_STARTED && i != PSF_FINISHED) flags[i] = i == PSF_SPIN ? spin : 0u;
     if (i == PSF_STARTED) flags[i] += nroles;
     if (i < nw) wx[i] = 0ull;
 }
@@ -1788,7 +1793,12 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     if (large) return launch_large();
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
-    if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL")) {
+    // KSVD_SWEEP_SPIN (tests): fewer polls before a persistent sweep's wait
+    // gives up, so that the abort and the two-kernel fallback can be exercised
+    unsigned spin = (unsigned)PS_SPIN_MAX;
+    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(1, atoi(sv));
+    const bool two_kernel = ctx->force_two_kernel || env_flag("KSVD_SWEEP_TWO_KERNEL");
+    if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
                            sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
@@ -1804,7 +1814,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
             unsigned long long *wx = (unsigned long long *)a.Gpart;
             const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;  // and each team's d_k pairs
             hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
-                               ctx->stream, flags, wx, nw, (unsigned)(2 * PT_HT));
+                               ctx->stream, flags, wx, nw, (unsigned)(2 * PT_HT), spin);
             // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on each XCD
             hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
             KSVD_HIP_CHECK(ctx, hipGetLastError());
@@ -1816,7 +1826,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     int H = 24;
     if (const char *hv = getenv("KSVD_SWEEP_HELPERS")) H = std::max(1, std::min(PS_HMAX, atoi(hv)));
     if (cap > 0) H = std::max(H, (cap + PS_CWMAX - 1) / PS_CWMAX);
-    const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !env_flag("KSVD_SWEEP_TWO_KERNEL");
+    const bool persistent = cap > 0 && H <= PS_HMAX && kmax <= PS_NOMAX + 1 && !two_kernel;
     if (persistent) {
         const size_t lds_helper = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (size_t)PS_CWMAX * PS_NOMAX) +
                                   sizeof(int) * ((size_t)PS_CWMAX * PS_NOMAX + 3 * PS_CWMAX + 1);
@@ -1833,7 +1843,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
         // the reduced Gram follows the partials in the same buffer
         float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
         hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags, (unsigned long long *)nullptr, 0,
-                           (unsigned)(H + 1));
+                           (unsigned)(H + 1), spin);
         // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on each XCD
         hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
@@ -1872,6 +1882,7 @@ int check_sweep(ksvd_ctx *ctx)
     KSVD_HIP_CHECK(ctx, hipMemcpyAsync(v, flags + PSF_STICKY, sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (v[0] || v[2] != v[3]) {
+        ctx->sweep_failed = true;
         KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STICKY, 0, sizeof(unsigned), ctx->stream));
         KSVD_HIP_CHECK(ctx, hipMemsetAsync(flags + PSF_STARTED, 0, 2 * sizeof(unsigned), ctx->stream));
         KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

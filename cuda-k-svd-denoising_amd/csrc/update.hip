@@ -15,6 +15,7 @@
 // k when the sweep starts (findRelevantIndicesKernel,
 // cuda_svd_denoiser.cu:36-50, run once per sweep at :298).
 #include <cstdlib>
+#include <mutex>
 
 #include "ksvd_internal.hpp"
 
@@ -287,12 +288,30 @@ int prepare_update_graph(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
+// Persistent sweeps of different contexts on one device must not run at the
+// same time: each needs all the CUs of one XCD, and two sharing them would
+// each wait for workgroups the other holds until their waits give up.  So
+// every update on a device waits (on the GPU, not the host) for the last
+// update enqueued on that device by any context; the wait, the launch and
+// the record are ordered under one lock.
+namespace {
+constexpr int UPD_MAX_DEVICES = 64;
+std::mutex upd_mu;
+hipEvent_t upd_last[UPD_MAX_DEVICES];
+}  // namespace
+
 int launch_update(ksvd_ctx *ctx)
 {
     // the users lists read the codes: the last sweep's clearing first
     int rc = flush_cap_zero(ctx);
     if (rc) return rc;
-    if (graphs_off()) {
+    if (ctx->device < 0 || ctx->device >= UPD_MAX_DEVICES)
+        return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(upd_mu);
+    hipEvent_t &last = upd_last[ctx->device];
+    if (!last) KSVD_HIP_CHECK(ctx, hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    else KSVD_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, last, 0));
+    if (graphs_off() || ctx->force_two_kernel) {
         rc = enqueue_update(ctx);
         if (rc) return rc;
     } else {
@@ -300,6 +319,7 @@ int launch_update(ksvd_ctx *ctx)
         if (rc) return rc;
         KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
     }
+    KSVD_HIP_CHECK(ctx, hipEventRecord(last, ctx->stream));
     ctx->cap_zero_pending = ctx->sweep_defers_zero;
     ctx->gram_valid = false;
     return KSVD_OK;

@@ -64,6 +64,7 @@ class Timings(ctypes.Structure):
         ("fallback", ctypes.c_int64),
         ("train_loss", ctypes.c_double),
         ("proposer_gated", ctypes.c_int64),
+        ("sweep_fallbacks", ctypes.c_int64),
     ]
 
 

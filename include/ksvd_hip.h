@@ -100,6 +100,8 @@ typedef struct ksvd_timings {
     double train_loss;      /* learned proposer: mean loss of the last training batch */
     int64_t proposer_gated; /* learned proposer: 1 once it stood down (a pass accepted
                                under half the patches), plain OMP since */
+    int64_t sweep_fallbacks; /* persistent sweeps that gave up and were undone and run
+                               again with two kernels per atom */
 } ksvd_timings;
 
 typedef struct ksvd_ctx ksvd_ctx;

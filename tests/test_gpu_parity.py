@@ -560,3 +560,102 @@ def test_geometry_modes(kl, orc, case):
     # another shape (both reported only; the bits are the oracle's)
     if not compat or (W == H and pw == ph):
         assert gain > 1.0
+
+
+def speckle_frame(kl, W=96, H=80):
+    """Log-domain luma of a colour picture with multiplicative speckle: a
+    dictionary sweep whose results once depended on where a context's buffers
+    sat (stale L1 lines in the two-team sweep)."""
+    rng = np.random.default_rng(5)
+    clean = np.stack([kl.synth_image(s, W, H, 0.0)[0] for s in (11, 12, 13)], axis=-1) + 10.0
+    noisy = (clean * rng.gamma(4.0, 0.25, size=clean.shape)).astype(np.float32)
+    return kl.speckle_log(kl.image_to_frame(kl.rgb_to_ycbcr(noisy)[0]))
+
+
+def sweep_run(kl, frame, W, H, iters=2):
+    ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5, ksvd_iter=iters)
+    ks.set_frame(frame, W, H)
+    ks.extract()
+    ks.init_dictionary()
+    ks.run()
+    out = ks.dictionary(), ks.codes()[1], ks.timings()
+    ks.close()
+    return out
+
+
+def test_sweep_same_bits_in_every_context(kl):
+    """Fresh contexts (buffers at new addresses each time) give the same bits."""
+    frame = speckle_frame(kl)
+    ref = sweep_run(kl, frame, 96, 80)
+    for rep in range(4):
+        d, c, _ = sweep_run(kl, frame, 96, 80)
+        np.testing.assert_array_equal(d, ref[0], err_msg=f"context {rep}")
+        np.testing.assert_array_equal(c, ref[1], err_msg=f"context {rep}")
+
+
+def test_two_threads_two_contexts(kl):
+    """Two contexts driven from two host threads at once on one device (their
+    persistent sweeps are ordered on the GPU) each reproduce their bits alone."""
+    import threading
+    jobs = [(speckle_frame(kl), 96, 80), (kl.image_to_frame(kl.synth_image(8, 128, 112, 20.0)[1]), 128, 112)]
+    alone = [sweep_run(kl, f, W, H, iters=3) for f, W, H in jobs]
+    res = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            f, W, H = jobs[i]
+            for _ in range(3):
+                res[i] = sweep_run(kl, f, W, H, iters=3)
+                np.testing.assert_array_equal(res[i][0], alone[i][0])
+                np.testing.assert_array_equal(res[i][1], alone[i][1])
+        except Exception as e:  # reported by the main thread
+            errs.append((i, e))
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errs, errs
+
+
+def test_sweep_that_gives_up_is_undone_and_run_again(kl, monkeypatch):
+    """A persistent sweep whose waits give up (forced from the host: a spin
+    limit of a few polls) is undone from its snapshot and run again with the
+    two-kernel sweep: the result is the two-kernel sweep's, bit for bit, for
+    a single update and for ksvd_iterate, and the fallbacks are counted."""
+    frame = speckle_frame(kl)
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1")
+    d2, c2, _ = sweep_run(kl, frame, 96, 80)
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "0")
+    monkeypatch.setenv("KSVD_SWEEP_SPIN", "1")
+    d, c, t = sweep_run(kl, frame, 96, 80)
+    print(f"\nsweep fallbacks in ksvd_run: {t['sweep_fallbacks']}")
+    assert t["sweep_fallbacks"] == 1
+    np.testing.assert_array_equal(d, d2)
+    np.testing.assert_array_equal(c, c2)
+    # a single update from given codes
+    ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5)
+    ks.set_frame(frame, 96, 80)
+    ks.extract()
+    ks.init_dictionary()
+    ks.omp()
+    codes = ks.codes()
+    D0 = ks.dictionary()
+    ks.update()
+    da, ca = ks.dictionary(), ks.codes()[1]
+    assert ks.timings()["sweep_fallbacks"] == 1
+    ks.close()
+    monkeypatch.setenv("KSVD_SWEEP_SPIN", str(1 << 21))
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1")
+    ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5)
+    ks.set_frame(frame, 96, 80)
+    ks.extract()
+    ks.set_dictionary(D0)
+    ks.set_codes(*codes)
+    ks.update()
+    np.testing.assert_array_equal(da, ks.dictionary())
+    np.testing.assert_array_equal(ca, ks.codes()[1])
+    ks.close()
