@@ -310,3 +310,74 @@ def test_cli_subimage_tiling(denoising_bin, tmp_path, kl, W, H, th, tw, geom):
     print(f"\ntiles {W}x{H} / {th}x{tw} {geom}: {k} tiles, raw PSNR gain {gain:.2f} dB")
     np.testing.assert_array_equal(out, expect)
     assert gain > 1.0
+
+
+@pytest.mark.gpu
+def test_cli_batch_uneven(denoising_bin, tmp_path, kl, orc):
+    """A reduced batch of the uneven workload through `denoising config.json`:
+    a colour picture with speckle, a grayscale one at 16 x 12 patches, one
+    with a per-file 22 x 22 override and a non-square one, in one config
+    (batch_denoiser.cpp:77-112 overrides).  Each output must be the bits of
+    the same steps through the C ABI, and each C ABI result's raw PSNR within
+    0.05 dB of the FP32 oracle's (3 K-SVD iterations)."""
+    rng = np.random.default_rng(17)
+    specs = [dict(name="rgb_speckle.pfm", W=192, H=160, rgb=True, seed=61),
+             dict(name="gray16x12.pfm", W=256, H=256, sigma=30.0, seed=62),
+             dict(name="gray22x22.pfm", W=224, H=160, sigma=40.0, seed=63, patch=(22, 22)),
+             dict(name="wide.pfm", W=320, H=192, sigma=15.0, seed=64)]
+    files = []
+    for sp in specs:
+        W, H = sp["W"], sp["H"]
+        if sp.get("rgb"):
+            clean = np.stack([kl.synth_image(sp["seed"] + 100 * c, W, H, 0.0)[0] for c in range(3)], axis=-1) + 10.0
+            noisy = (clean * rng.gamma(4.0, 0.25, size=clean.shape)).astype(np.float32)
+            write_pfm_rgb(tmp_path / sp["name"], noisy)
+            sp["clean_y"] = kl.rgb_to_ycbcr(clean)[0]
+            sp["noisy_planes"] = kl.rgb_to_ycbcr(noisy)
+        else:
+            clean, noisy = kl.synth_image(sp["seed"], W, H, sp["sigma"])
+            write_pfm(tmp_path / sp["name"], noisy)
+            sp["clean_y"] = clean
+            sp["noisy_planes"] = noisy[None]
+        f = {"name": sp["name"]}
+        if sp.get("rgb"):
+            f.update({"B&W": False, "speckle": True})
+        if "patch" in sp:
+            f.update({"patchWidthDim": sp["patch"][0], "patchHeightDim": sp["patch"][1]})
+        files.append(f)
+    g = dict(GLOBAL, **{"patchWidthDim": 16, "patchHeightDim": 12, "slidingWidth": 3, "slidingHeight": 3,
+                        "ksvdIter": 3, "atoms": 256})
+    cfg = {"inputFolder": str(tmp_path), "outputFolder": str(tmp_path), "globalParams": g, "files": files}
+    r = run(denoising_bin, cfg, tmp_path)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for sp in specs:
+        pw, ph = sp.get("patch", (16, 12))
+        W, H = sp["W"], sp["H"]
+        planes = sp["noisy_planes"]
+        s2 = np.float32(kl.noise_variance_channels(planes))
+        frame = kl.image_to_frame(planes[0])
+        if sp.get("rgb"):
+            frame = kl.speckle_log(frame)
+        ks = kl.KSvd(patch=(pw, ph), slide=(3, 3), atoms=256, omp_iter=5, ksvd_iter=3)
+        ks.set_frame(frame, W, H)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        raw = ks.rebuild(s2).ravel()
+        ks.close()
+        den = kl.speckle_exp(raw) if sp.get("rgb") else raw
+        out = read_pfm(tmp_path / f"{pw}_{ph}_{sp['name']}")
+        np.testing.assert_array_equal(out, kl.normalize(kl.frame_to_image(den, W, H)), err_msg=sp["name"])
+        # the oracle on the same frame (the log domain for speckle)
+        Y = orc.extract(frame, W, H, pw, ph, 3, 3)
+        D, idx, coef, cnt, _ = orc.ksvd(Y, orc.init_dict(Y, 256), 5, 3)
+        oo, *_ = orc.rebuild(idx, coef, cnt, D, frame, W, H, pw, ph, 3, 3, s2)
+        oo = oo.astype(np.float32).ravel()
+        if sp.get("rgb"):
+            oo = kl.speckle_exp(oo)
+        ref = sp["clean_y"]
+        pg = kl.psnr(ref, kl.frame_to_image(den, W, H))
+        po = kl.psnr(ref, kl.frame_to_image(oo, W, H))
+        print(f"{sp['name']} {W}x{H} {pw}x{ph}: raw PSNR gpu {pg:.4f} oracle {po:.4f} ({pg - po:+.4f} dB)")
+        assert abs(pg - po) <= 0.05
