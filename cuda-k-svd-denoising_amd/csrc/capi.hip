@@ -232,8 +232,6 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
     KSVD_HIP_CHECK(ctx, hipMemsetAsync(ctx->counters.ptr, 0, ctx->counters.bytes, ctx->stream));
     rc = ksvd_reserve(ctx, ctx->D, sizeof(float) * (size_t)ctx->dim * ctx->K);
     if (rc) return rc;
-    rc = ksvd_reserve(ctx, ctx->Dt, sizeof(float) * (size_t)ctx->dim * ctx->K);
-    if (rc) return rc;
     rc = ksvd_reserve(ctx, ctx->G, sizeof(float) * (size_t)ctx->K * ctx->K);
     if (rc) return rc;
     rc = ksvd_reserve(ctx, ctx->wbuf, sizeof(float) * (size_t)ctx->dim * ctx->dim);
@@ -261,14 +259,14 @@ int ksvd_destroy(ksvd_ctx *ctx)
     drop_update_graph(ctx);
     if (ctx->gate_ev) (void)hipEventDestroy((hipEvent_t)ctx->gate_ev);
     if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
-    DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->Dt,      &ctx->G,       &ctx->idx,
+    DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
                       &ctx->users, &ctx->uslot, &ctx->E,     &ctx->wbuf, &ctx->moff, &ctx->tab_p, &ctx->tab_self, &ctx->tab_oth, &ctx->srank, &ctx->handx, &ctx->E2, &ctx->gpart, &ctx->ticket,    &ctx->rsum,    &ctx->rweight,
                       &ctx->rout, &ctx->rpart,  &ctx->rlam, &ctx->Dbf, &ctx->Dq, &ctx->bfscal, &ctx->rbound,
                       &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
                       &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
                       &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
-                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt};
+                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt, &ctx->c0};
     for (DevBuf *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

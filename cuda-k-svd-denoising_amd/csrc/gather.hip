@@ -51,8 +51,8 @@ __global__ void init_dict_kernel(const float *__restrict__ Y, int dim, int64_t o
     for (int e = 0; e < dim; e++) D[(int64_t)k * dim + e] = y[e] / norm;
 }
 
-// G = D^T D (K x K) and Dt = D^T laid out dim x K (row e holds every atom's
-// element e).  One thread per Gram entry; element order, fused multiply-adds.
+// G = D^T D (K x K).  One thread per Gram entry; element order, fused
+// multiply-adds.
 __global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ G)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,13 +67,6 @@ __global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, fl
     G[q] = (float)s;
 }
 
-__global__ void transpose_dict_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ Dt)
-{
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (int64_t)K * dim) return;
-    const int e = (int)(q / K), a = (int)(q - (int64_t)e * K);
-    Dt[q] = D[(int64_t)a * dim + e];
-}
 
 }  // namespace
 
@@ -107,10 +100,6 @@ int launch_dict_gram(ksvd_ctx *ctx)
     const int64_t kk = (int64_t)ctx->K * ctx->K;
     hipLaunchKernelGGL(dict_gram_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, ctx->stream,
                        (const float *)ctx->D.ptr, ctx->dim, ctx->K, (float *)ctx->G.ptr);
-    KSVD_HIP_CHECK(ctx, hipGetLastError());
-    const int64_t kd = (int64_t)ctx->K * ctx->dim;
-    hipLaunchKernelGGL(transpose_dict_kernel, dim3((unsigned)((kd + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const float *)ctx->D.ptr, ctx->dim, ctx->K, (float *)ctx->Dt.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
     ctx->gram_valid = true;
     return KSVD_OK;

@@ -57,7 +57,8 @@ struct ksvd_ctx {
     bool gram_valid = false;
 
     // device buffers
-    DevBuf img, Y, D, Dt, G, idx, coef, cnt, counters;
+    DevBuf img, Y, D, G, idx, coef, cnt, counters;
+    DevBuf c0;  // first-step correlations of a chunk of patches (pursuit of large patches or dictionaries)
     DevBuf blockcnt, ucount, uoff, users, uslot, E, wbuf;
     DevBuf moff, tab_p, tab_self, tab_oth;  // column table of the sweep
     DevBuf srank, handx, E2, gpart, ticket; // sweep kernels (sweep.hip); ticket: flags of the persistent sweep

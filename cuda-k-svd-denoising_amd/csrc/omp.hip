@@ -20,6 +20,7 @@
 //     rule out ||r|| < 1e-3 (the retirement test), the explicit
 //     r = y - D_S x decides.
 // Semantics follow CuBlasMatrixOmp::work (cublas_matrix_omp.cu:107-402).
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <mutex>
@@ -311,26 +312,113 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
     if (ndeg) atomicAdd(&counters[CNT_DEGEN], ndeg);
 }
 
-// Fallback for shapes the MFMA kernel does not cover (dim > 144, K > 256 or
-// ompIter > 8): one wavefront per patch, the residual spread over the lanes
-// (element lane + 64 j), correlations from the transposed dictionary so that
-// lanes read consecutive atoms.  Same semantics, no MFMA.
-template <int RJ, int KM>
-__global__ __launch_bounds__(256) void omp_generic_kernel(
-    const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D,
-    const float *__restrict__ Dt, int K, const float *__restrict__ G, int kmax,
-    int *__restrict__ idx_out, float *__restrict__ coef_out, int *__restrict__ cnt_out,
-    unsigned long long *__restrict__ counters)
+// Pursuit for the shapes the MFMA kernel does not cover (dim > 144, K > 256
+// or ompIter > 8): the first step's correlations C0 = D^T Y come from an
+// MFMA GEMM over a chunk of patches, and a pursuit kernel, one wavefront per
+// patch, takes every later step's D^T r = C0 - G[:, S] x from the dictionary
+// Gram (Batch-OMP, as the MFMA kernel does).  Same semantics as
+// omp_mfma_kernel: first index of max |c|, Cholesky rows from G, two
+// triangular solves, ||r||^2 = ||y||^2 - b.x with the explicit residual only
+// near zero, duplicate and degenerate stops.
+
+// C0[p][a] = d_a . y_p on v_mfma_f32_16x16x4_f32.  Workgroup tile: 128 atoms x
+// 64 patches, four wavefronts of 64 atoms x 32 patches (eight 16 x 16 blocks);
+// the reduction runs over the patch dimension in chunks of 16 staged in LDS
+// (zero past dim, K and P).
+constexpr int CG_TA = 128, CG_TP = 64, CG_KC = 16;
+
+__global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict__ Y, int dim, int64_t P,
+                                                        const float *__restrict__ D, int K, float *__restrict__ C0)
+{
+    __shared__ float Ds[CG_TA][CG_KC + 1];
+    __shared__ float Ys[CG_TP][CG_KC + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wa = wave & 1, wp = wave >> 1;
+    const int col = lane & 15, g = lane >> 4;
+    const int a0 = blockIdx.y * CG_TA;
+    const int64_t p0 = (int64_t)blockIdx.x * CG_TP;
+    f4 acc[4][2];
+#pragma unroll
+    for (int ab = 0; ab < 4; ab++)
+#pragma unroll
+        for (int pb = 0; pb < 2; pb++) acc[ab][pb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int e0 = 0; e0 < dim; e0 += CG_KC) {
+        for (int q = tid; q < CG_TA * CG_KC; q += 256) {
+            const int r = q / CG_KC, c = q - r * CG_KC;
+            const int at = a0 + r, e = e0 + c;
+            Ds[r][c] = (at < K && e < dim) ? D[(int64_t)at * dim + e] : 0.f;
+        }
+        for (int q = tid; q < CG_TP * CG_KC; q += 256) {
+            const int r = q / CG_KC, c = q - r * CG_KC;
+            const int64_t pp = p0 + r;
+            const int e = e0 + c;
+            Ys[r][c] = (pp < P && e < dim) ? Y[pp * dim + e] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CG_KC; kk += 4) {
+            float av[4], bv[2];
+#pragma unroll
+            for (int ab = 0; ab < 4; ab++) av[ab] = Ds[wa * 64 + ab * 16 + col][kk + g];
+#pragma unroll
+            for (int pb = 0; pb < 2; pb++) bv[pb] = Ys[wp * 32 + pb * 16 + col][kk + g];
+#pragma unroll
+            for (int ab = 0; ab < 4; ab++)
+#pragma unroll
+                for (int pb = 0; pb < 2; pb++)
+                    acc[ab][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ab], bv[pb], acc[ab][pb], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // lane (col, g) holds C[atom 4g + i][patch col] of each block
+#pragma unroll
+    for (int ab = 0; ab < 4; ab++)
+#pragma unroll
+        for (int pb = 0; pb < 2; pb++) {
+            const int at = a0 + wa * 64 + ab * 16 + 4 * g;
+            const int64_t pp = p0 + wp * 32 + pb * 16 + col;
+            if (pp < P)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (at + i < K) C0[pp * K + at + i] = acc[ab][pb][i];
+        }
+}
+
+// Fixed butterfly over the wavefront: every lane ends with the same bits.
+__device__ __forceinline__ float wave_sum_xor(float v)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One wavefront per patch: lane l holds atoms l, l + 64, ... (CPL of them) of
+// the correlations and elements l, l + 64, ... (RJ of them) of the patch.
+template <int CPL, int RJ, int KM>
+__global__ __launch_bounds__(256) void omp_batch_kernel(
+    const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
+    const float *__restrict__ G, const float *__restrict__ C0, int kmax, int *__restrict__ idx_out,
+    float *__restrict__ coef_out, int *__restrict__ cnt_out, unsigned long long *__restrict__ counters)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned long long ndup = 0, ndeg = 0;
     for (int64_t p = (int64_t)blockIdx.x * 4 + wave; p < P; p += (int64_t)gridDim.x * 4) {
-        float y[RJ], r[RJ];
+        float y[RJ];
 #pragma unroll
         for (int jj = 0; jj < RJ; jj++) {
             const int e = lane + 64 * jj;
             y[jj] = e < dim ? Y[p * dim + e] : 0.f;
-            r[jj] = y[jj];
+        }
+        float yy = 0.f;
+#pragma unroll
+        for (int jj = 0; jj < RJ; jj++) yy = fmaf(y[jj], y[jj], yy);
+        yy = wave_sum_xor(yy);
+        const float band = 1e-6f + yy * 0x1p-10f + sqrtf(yy) * 0x1p-8f;
+        float c0[CPL];
+#pragma unroll
+        for (int i = 0; i < CPL; i++) {
+            const int at = lane + 64 * i;
+            c0[i] = at < K ? C0[p * K + at] : 0.f;
         }
         int S[KM];
         float x[KM], bv[KM], L[KM * KM];
@@ -340,22 +428,29 @@ __global__ __launch_bounds__(256) void omp_generic_kernel(
         for (int i = 0; i < KM * KM; i++) L[i] = 0.f;
         int s = 0;
         for (int t = 0; t < kmax; t++) {
+            // D^T r = C0 - G[:, S] x, support entries in order
+            float c[CPL];
+#pragma unroll
+            for (int i = 0; i < CPL; i++) c[i] = c0[i];
+#pragma unroll
+            for (int q = 0; q < KM; q++)
+                if (q < s) {
+                    const float *gr = G + (int64_t)S[q] * K;
+#pragma unroll
+                    for (int i = 0; i < CPL; i++) {
+                        const int at = lane + 64 * i;
+                        if (at < K) c[i] = fmaf(-gr[at], x[q], c[i]);
+                    }
+                }
+            // first index of max |c| (cublasIsamax)
             float best = -1.f;
             int bi = 0x7fffffff;
-            for (int a0 = 0; a0 < K; a0 += 64) {
-                const int a = a0 + lane;
-                float c = 0.f;
 #pragma unroll
-                for (int jj = 0; jj < RJ; jj++)
-                    for (int l = 0; l < 64; l++) {
-                        const int e = 64 * jj + l;
-                        if (e >= dim) break;
-                        const float re = __shfl(r[jj], l);
-                        if (a < K) c = fmaf(Dt[(int64_t)e * K + a], re, c);
-                    }
-                if (a < K) {
-                    const float v = fabsf(c);
-                    if (v > best) { best = v; bi = a; }
+            for (int i = 0; i < CPL; i++) {
+                const int at = lane + 64 * i;
+                if (at < K) {
+                    const float v = fabsf(c[i]);
+                    if (v > best) { best = v; bi = at; }
                 }
             }
 #pragma unroll
@@ -365,26 +460,31 @@ __global__ __launch_bounds__(256) void omp_generic_kernel(
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
             }
             const int j = bi;
-            float bp = 0.f;
-#pragma unroll
-            for (int jj = 0; jj < RJ; jj++) {
-                const int e = lane + 64 * jj;
-                if (e < dim && j < K) bp = fmaf(D[(int64_t)j * dim + e], y[jj], bp);
-            }
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) bp += __shfl_xor(bp, off);
             bool dup = false;
 #pragma unroll
             for (int i = 0; i < KM; i++) dup |= (i < s) && (S[i] == j);
             if (dup) { if (lane == 0) ndup++; break; }
+            // a non-finite residual leaves no atom with a finite |c|: retire
             if (j >= K) { if (lane == 0) ndeg++; break; }
+            // b_j = d_j . y
+            float bp = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < RJ; jj++) {
+                const int e = lane + 64 * jj;
+                if (e < dim) bp = fmaf(D[(int64_t)j * dim + e], y[jj], bp);
+            }
+            bp = wave_sum_xor(bp);
+            // Cholesky row of the support Gram from the dictionary Gram
+            float gS[KM];
+#pragma unroll
+            for (int i = 0; i < KM; i++) gS[i] = G[(int64_t)(i < s ? S[i] : j) * K + j];
             float w[KM];
             float piv = G[(int64_t)j * K + j];
 #pragma unroll
             for (int i = 0; i < KM; i++) {
                 w[i] = 0.f;
                 if (i < s) {
-                    float gi = G[(int64_t)S[i] * K + j];
+                    float gi = gS[i];
 #pragma unroll
                     for (int q = 0; q < i; q++) gi = fmaf(-L[i * KM + q], w[q], gi);
                     w[i] = gi / L[i * KM + i];
@@ -423,33 +523,38 @@ __global__ __launch_bounds__(256) void omp_generic_kernel(
                         if (q < s) v = fmaf(-L[q * KM + i], x[q], v);
                     x[i] = v / L[i * KM + i];
                 }
-            float nr = 0.f;
+            // ||r||^2 = ||y||^2 - b.x; the explicit residual where that is
+            // too small to rule out ||r|| < 1e-3
+            float nr = yy;
 #pragma unroll
-            for (int jj = 0; jj < RJ; jj++) {
-                const int e = lane + 64 * jj;
-                float a = 0.f;
+            for (int i = 0; i < KM; i++)
+                if (i < s) nr = fmaf(-bv[i], x[i], nr);
+            if (nr < band) {
+                float ne = 0.f;
 #pragma unroll
-                for (int i = 0; i < KM; i++)
-                    if (i < s && e < dim) a = fmaf(D[(int64_t)S[i] * dim + e], x[i], a);
-                r[jj] = y[jj] - a;
-                nr = fmaf(r[jj], r[jj], nr);
+                for (int jj = 0; jj < RJ; jj++) {
+                    const int e = lane + 64 * jj;
+                    float a = 0.f;
+#pragma unroll
+                    for (int i = 0; i < KM; i++)
+                        if (i < s && e < dim) a = fmaf(D[(int64_t)S[i] * dim + e], x[i], a);
+                    const float rv = e < dim ? y[jj] - a : 0.f;
+                    ne = fmaf(rv, rv, ne);
+                }
+                nr = wave_sum_xor(ne);
             }
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) nr += __shfl_xor(nr, off);
             if (sqrtf(nr) < 1e-3f) break;
         }
-        if (lane == 0) {
-            cnt_out[p] = s;
-            for (int i = 0; i < kmax; i++) {
-                int si = -1;
-                float xi = 0.f;
+        if (lane < kmax) {
+            int si = -1;
+            float xi = 0.f;
 #pragma unroll
-                for (int q = 0; q < KM; q++)
-                    if (q == i && q < s) { si = S[q]; xi = x[q]; }
-                idx_out[p * kmax + i] = si;
-                coef_out[p * kmax + i] = xi;
-            }
+            for (int q = 0; q < KM; q++)
+                if (q == lane && q < s) { si = S[q]; xi = x[q]; }
+            idx_out[p * kmax + lane] = si;
+            coef_out[p * kmax + lane] = xi;
         }
+        if (lane == 0) cnt_out[p] = s;
     }
     if (ndup) atomicAdd(&counters[CNT_DUP], ndup);
     if (ndeg) atomicAdd(&counters[CNT_DEGEN], ndeg);
@@ -474,26 +579,59 @@ int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float 
     return KSVD_OK;
 }
 
-template <int RJ>
-int launch_generic(ksvd_ctx *ctx)
+template <int CPL, int RJ, int KM>
+int launch_batch_t(ksvd_ctx *ctx, int64_t p0, int64_t np, float *C0)
 {
-    const int64_t waves = ctx->P;
+    const int64_t waves = np;
     int64_t grid = (waves + 3) / 4;
-    if (grid > 8 * ctx->num_cus) grid = 8 * ctx->num_cus;
-    if (ctx->kmax <= 8) {
-        hipLaunchKernelGGL((omp_generic_kernel<RJ, 8>), dim3((unsigned)grid), dim3(256), 0, ctx->stream,
-                           (const float *)ctx->Y.ptr, ctx->dim, ctx->P, (const float *)ctx->D.ptr,
-                           (const float *)ctx->Dt.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
-                           (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
-                           (unsigned long long *)ctx->counters.ptr);
-    } else {
-        hipLaunchKernelGGL((omp_generic_kernel<RJ, KSVD_MAX_SUPPORT>), dim3((unsigned)grid), dim3(256), 0,
-                           ctx->stream, (const float *)ctx->Y.ptr, ctx->dim, ctx->P, (const float *)ctx->D.ptr,
-                           (const float *)ctx->Dt.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
-                           (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
-                           (unsigned long long *)ctx->counters.ptr);
-    }
+    if (grid > 8 * (int64_t)ctx->num_cus) grid = 8 * ctx->num_cus;
+    const int kmax = ctx->kmax, dim = ctx->dim;
+    hipLaunchKernelGGL((omp_batch_kernel<CPL, RJ, KM>), dim3((unsigned)grid), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->Y.ptr + p0 * dim, dim, np, (const float *)ctx->D.ptr, ctx->K,
+                       (const float *)ctx->G.ptr, (const float *)C0, kmax, (int *)ctx->idx.ptr + p0 * kmax,
+                       (float *)ctx->coef.ptr + p0 * kmax, (int *)ctx->cnt.ptr + p0,
+                       (unsigned long long *)ctx->counters.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+template <int CPL, int RJ>
+int launch_batch_k(ksvd_ctx *ctx, int64_t p0, int64_t np, float *C0)
+{
+    return ctx->kmax <= 8 ? launch_batch_t<CPL, RJ, 8>(ctx, p0, np, C0)
+                          : launch_batch_t<CPL, RJ, KSVD_MAX_SUPPORT>(ctx, p0, np, C0);
+}
+
+template <int CPL>
+int launch_batch_c(ksvd_ctx *ctx, int64_t p0, int64_t np, float *C0)
+{
+    const int dim = ctx->dim;
+    if (dim <= 256) return launch_batch_k<CPL, 4>(ctx, p0, np, C0);
+    if (dim <= 512) return launch_batch_k<CPL, 8>(ctx, p0, np, C0);
+    return launch_batch_k<CPL, 16>(ctx, p0, np, C0);
+}
+
+// The GEMM-correlation pursuit over every patch, in chunks whose C0 stays
+// under 128 MB.
+int launch_batch(ksvd_ctx *ctx)
+{
+    const int K = ctx->K;
+    const int64_t P = ctx->P;
+    int64_t chunk = ((int64_t)32 << 20) / K;
+    chunk = std::max<int64_t>(CG_TP, chunk / CG_TP * CG_TP);
+    if (chunk > P) chunk = P;
+    int rc = ksvd_reserve(ctx, ctx->c0, sizeof(float) * (size_t)chunk * K);
+    if (rc) return rc;
+    float *C0 = (float *)ctx->c0.ptr;
+    for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+        const int64_t np = std::min(chunk, P - p0);
+        hipLaunchKernelGGL(corr_gemm_kernel, dim3((unsigned)((np + CG_TP - 1) / CG_TP), (unsigned)((K + CG_TA - 1) / CG_TA)),
+                           dim3(256), 0, ctx->stream, (const float *)ctx->Y.ptr + p0 * ctx->dim, ctx->dim, np,
+                           (const float *)ctx->D.ptr, K, C0);
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        rc = K <= 256 ? launch_batch_c<4>(ctx, p0, np, C0) : launch_batch_c<16>(ctx, p0, np, C0);
+        if (rc) return rc;
+    }
     return KSVD_OK;
 }
 
@@ -533,12 +671,7 @@ int launch_omp(ksvd_ctx *ctx)
     // the learned form keeps each patch's residual norm: its proposals are
     // accepted against it
     if (mfma_ok(ctx)) return launch_mfma_any(ctx, nullptr, nullptr, ctx->prm.proposer ? (float *)ctx->rbound.ptr : nullptr);
-    const int rj = (ctx->dim + 63) / 64;
-    if (rj <= 1) return launch_generic<1>(ctx);
-    if (rj <= 2) return launch_generic<2>(ctx);
-    if (rj <= 4) return launch_generic<4>(ctx);
-    if (rj <= 8) return launch_generic<8>(ctx);
-    return launch_generic<16>(ctx);
+    return launch_batch(ctx);
 }
 
 // ---- allocation-free pursuit (ksvd_omp_noalloc) ---------------------------
