@@ -1624,18 +1624,22 @@ __global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw,
 }
 
 // ===========================================================================
-// Large patches (padded dim above 192, up to 1024): two kernels per atom with
-// E_k in global memory (pitch dim) and no Gram matrix, for the patch sizes
-// the LDS-resident sweeps cannot hold (22 x 22 patches: dim 484).
-//   kl_build_kernel(k), B1 workgroups: E_k's capped columns, e_t = y_p -
+// Large patches (padded dim above 192, up to 1024): four launches per atom,
+// E_k and its Gram in global memory, for the patch sizes the LDS-resident
+// sweeps cannot hold (22 x 22 patches: dim 484).
+//   kl_build_kernel(k), many workgroups: E_k's capped columns, e_t = y_p -
 //     sum_{j != k} d_j x_{j,p} in the column table's entry order, every
 //     coefficient read from memory (atom k-1's were written, final, by
-//     kl_power_kernel(k-1); users beyond an updated atom's cap were zeroed
+//     kl_coef_kernel(k-1); users beyond an updated atom's cap were zeroed
 //     there too);
-//   kl_power_kernel(k), one workgroup: power iteration z = E_k (E_k^T u),
-//     u = z / |z| from d_k, the stop rule of the other sweeps; then d_k = u,
-//     x_t = e_t . u for the first m users and 0 for the others.
+//   kl_gram_kernel(k): G = E_k E_k^T on MFMA, one workgroup per 64 x 64 tile
+//     of the upper triangle (mirrored), summed over the columns in order;
+//   kl_power_kernel(k), one workgroup: power iteration on G from d_k with the
+//     stop rule of the other sweeps; d_k = u;
+//   kl_coef_kernel(k), many workgroups: x_t = e_t . d_k for the first m
+//     users, 0 for the others.
 constexpr int KL_T = 1024;
+constexpr int KG_TB = 64, KG_KC = 64;
 
 __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, int k)
 {
@@ -1659,6 +1663,137 @@ __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, i
     }
 }
 
+// Tile (bi, bj), bi <= bj, of G = E E^T (dim x dim, pitch dim), four
+// wavefronts of 32 x 32; the columns of E in chunks of KG_KC staged in LDS,
+// the next chunk's loads in flight (registers) while the current one is on
+// MFMA.
+constexpr int KG_LD = (KG_KC * KG_TB) / 256;  // values of each operand a thread stages per chunk
+
+__global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *__restrict__ E, float *__restrict__ G,
+                                                      int k)
+{
+    __shared__ float Ea[KG_KC][KG_TB + 4];
+    __shared__ float Eb[KG_KC][KG_TB + 4];
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    const int dim = a.dim;
+    const int nb = (dim + KG_TB - 1) / KG_TB;
+    // tile index -> (bi, bj) of the upper triangle, row by row
+    int ti = blockIdx.x, bi = 0;
+    while (ti >= nb - bi) { ti -= nb - bi; bi++; }
+    const int bj = bi + ti;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int e0 = bi * KG_TB, f0 = bj * KG_TB;
+    f4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+    // thread tid stages entries q = tid + 256 i: row q / KG_TB, column q % KG_TB
+    float ra[KG_LD], rb[KG_LD];
+    auto fetch = [&](int t0) {
+#pragma unroll
+        for (int i = 0; i < KG_LD; i++) {
+            const int q = tid + 256 * i, r = q / KG_TB, cc = q - r * KG_TB, t = t0 + r;
+            ra[i] = (t < m && e0 + cc < dim) ? E[(int64_t)t * dim + e0 + cc] : 0.f;
+            rb[i] = (t < m && f0 + cc < dim) ? E[(int64_t)t * dim + f0 + cc] : 0.f;
+        }
+    };
+    if (m > 0) fetch(0);
+    for (int t0 = 0; t0 < m; t0 += KG_KC) {
+#pragma unroll
+        for (int i = 0; i < KG_LD; i++) {
+            const int q = tid + 256 * i, r = q / KG_TB, cc = q - r * KG_TB;
+            Ea[r][cc] = ra[i];
+            Eb[r][cc] = rb[i];
+        }
+        __syncthreads();
+        if (t0 + KG_KC < m) fetch(t0 + KG_KC);
+#pragma unroll
+        for (int kk = 0; kk < KG_KC; kk += 4) {
+            float av[2], bv[2];
+#pragma unroll
+            for (int x = 0; x < 2; x++) av[x] = Ea[kk + g][wr * 32 + x * 16 + col];
+#pragma unroll
+            for (int y = 0; y < 2; y++) bv[y] = Eb[kk + g][wc * 32 + y * 16 + col];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x], bv[y], acc[x][y], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // lane (col, g) holds G[row 4g + i][column col] of each 16 x 16 block
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int e = e0 + wr * 32 + x * 16 + 4 * g + i, f = f0 + wc * 32 + y * 16 + col;
+                if (e < dim && f < dim) {
+                    G[(int64_t)e * dim + f] = acc[x][y][i];
+                    G[(int64_t)f * dim + e] = acc[x][y][i];
+                }
+            }
+}
+
+// Power iteration on G (global, symmetric): z = G u with thread (sl, e)
+// summing column e over rows f of slice sl (coalesced across e, loads
+// unrolled so that many are in flight), the slices added in order.
+constexpr int KP_U = 8;
+
+__global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float *__restrict__ G, int k)
+{
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float zp[KL_T];
+    __shared__ float red[KL_T / 64];
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    if (m == 0) return;
+    const int dim = a.dim, tid = threadIdx.x;
+    const int ns = KL_T / dim;  // slices (dim <= 1024)
+    const int sl = tid / dim, e = tid - sl * dim;
+    const int fl = (dim + ns - 1) / ns;
+    const int fb = sl * fl, fe = min(dim, fb + fl);
+    if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
+    __syncthreads();
+    int it = 0;
+    for (;;) {
+        float s = 0.f;
+        if (sl < ns) {
+            int f = fb;
+            for (; f + KP_U <= fe; f += KP_U) {
+                float gv[KP_U];
+#pragma unroll
+                for (int q = 0; q < KP_U; q++) gv[q] = G[(int64_t)(f + q) * dim + e];
+#pragma unroll
+                for (int q = 0; q < KP_U; q++) s = fmaf(gv[q], u[f + q], s);
+            }
+            for (; f < fe; f++) s = fmaf(G[(int64_t)f * dim + e], u[f], s);
+        }
+        zp[tid] = s;
+        __syncthreads();
+        float zv = 0.f;
+        if (tid < dim)
+            for (int q = 0; q < ns; q++) zv += zp[q * dim + tid];
+        const float n2 = block_sum_t2(zv * zv, red);
+        if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
+        const float un = zv / sqrtf(n2);
+        const float du = tid < dim ? un - u[tid] : 0.f;
+        const float d2 = block_sum_t2(du * du, red);
+        it++;
+        if (tid < dim) u[tid] = un;
+        __syncthreads();
+        if ((it >= 2 && d2 < a.tol2) || it >= a.maxit) break;
+    }
+    if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
+    if (tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
+}
+
 // e_t . u by one wavefront (lane-strided over dim, fixed butterfly).
 __device__ __forceinline__ float kl_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
 {
@@ -1667,54 +1802,23 @@ __device__ __forceinline__ float kl_dot(const float *__restrict__ e, const float
     return wave_sum_dpp(s);
 }
 
-__global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float *__restrict__ E, float *vbuf, int k)
+__global__ __launch_bounds__(256) void kl_coef_kernel(SweepArgs a, const float *__restrict__ E, int k)
 {
     __shared__ float u[KSVD_MAX_DIM];
-    __shared__ float red[KL_T / 64];
-    __shared__ int s_stop;
     int c;
     const int m = atom_m(a.uoff, k, a.cap, &c);
-    const int dim = a.dim, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int dim = a.dim, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < dim; e += 256) u[e] = a.D[(int64_t)k * dim + e];
+    __syncthreads();
     const int64_t cb = a.moff[k];
-    if (m > 0) {
-        if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
-        __syncthreads();
-        int it = 0;
-        for (;;) {
-            // v = E^T u
-            for (int t = wave; t < m; t += KL_T / 64) {
-                const float x = kl_dot(E + (int64_t)t * dim, u, dim);
-                if (lane == 0) vbuf[t] = x;
-            }
-            __threadfence_block();
-            __syncthreads();
-            // z = E v, ascending column order
-            float z = 0.f;
-            if (tid < dim)
-                for (int t = 0; t < m; t++) z = fmaf(E[(int64_t)t * dim + tid], vbuf[t], z);
-            const float n2 = block_sum_t2(tid < dim ? z * z : 0.f, red);
-            if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
-            const float un = z / sqrtf(n2);
-            const float du = tid < dim ? un - u[tid] : 0.f;
-            const float d2 = block_sum_t2(du * du, red);
-            it++;
-            if (tid == 0) s_stop = (it >= 2 && d2 < a.tol2) || it >= a.maxit;
-            __syncthreads();
-            if (tid < dim) u[tid] = un;
-            __syncthreads();
-            if (s_stop) break;
-        }
-        if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
-        if (tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
-        // the new coefficients of the first m users
-        for (int t = wave; t < m; t += KL_T / 64) {
-            const float x = kl_dot(E + (int64_t)t * dim, u, dim);
-            if (lane == 0) a.coef[a.tab_self[cb + t]] = x;
-        }
+    for (int t = blockIdx.x * 4 + wave; t < m; t += gridDim.x * 4) {
+        const float x = kl_dot(E + (int64_t)t * dim, u, dim);
+        if (lane == 0) a.coef[a.tab_self[cb + t]] = x;
     }
     // users beyond the cap: coefficient 0
     const int base = a.uoff[k];
-    for (int t = m + tid; t < c; t += KL_T) a.coef[(int64_t)a.users[base + t] * a.kmax + a.uslot[base + t]] = 0.f;
+    for (int t = m + blockIdx.x * 256 + threadIdx.x; t < c; t += gridDim.x * 256)
+        a.coef[(int64_t)a.users[base + t] * a.kmax + a.uslot[base + t]] = 0.f;
 }
 
 }  // namespace
@@ -1781,11 +1885,17 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     // ---- large patches: E in global memory, two kernels per atom -------------
     auto launch_large = [&]() -> int {
         if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
-        float *E = (float *)ctx->E.ptr;   // m x dim, m <= min(users, cap): the E buffer's columns
-        float *vbuf = (float *)ctx->E2.ptr;  // E^T u, one value per column
+        float *E = (float *)ctx->E.ptr;  // m x dim, m <= min(users, cap): the E buffer's columns
+        float *G = a.Gpart;              // dim x dim
+        if (sizeof(float) * (size_t)dim * dim > ctx->gpart.bytes)
+            return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: Gram buffer too small");
+        const int nb = (dim + KG_TB - 1) / KG_TB;
+        const int ntg = nb * (nb + 1) / 2;
         for (int k = 0; k < K; k++) {
             hipLaunchKernelGGL(kl_build_kernel, dim3(B1 * 4), dim3(KL_T), 0, ctx->stream, a, E, k);
-            hipLaunchKernelGGL(kl_power_kernel, dim3(1), dim3(KL_T), 0, ctx->stream, a, (const float *)E, vbuf, k);
+            hipLaunchKernelGGL(kl_gram_kernel, dim3(ntg), dim3(256), 0, ctx->stream, a, (const float *)E, G, k);
+            hipLaunchKernelGGL(kl_power_kernel, dim3(1), dim3(KL_T), 0, ctx->stream, a, (const float *)G, k);
+            hipLaunchKernelGGL(kl_coef_kernel, dim3(B1 * 4), dim3(256), 0, ctx->stream, a, (const float *)E, k);
         }
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
