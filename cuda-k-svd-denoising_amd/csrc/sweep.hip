@@ -62,8 +62,27 @@ struct SweepArgs {
     int dim, dpad, K, kmax, cap;
     int nbt, ntiles;
     int maxit;
-    float tol2;
+    float tol2;  // stop when |u_i - u_{i-1}|^2 falls below it
+    float est2;  // or when the estimated |u_i - limit|^2 does (power_stop)
 };
+
+// Stop rule of every sweep's power iteration, after iterate i with d2 =
+// |u_i - u_{i-1}|^2 and d2p = |u_{i-1} - u_{i-2}|^2: never before i = 2; stop
+// when the step is below the tolerance (d2 < tol2), or when the a-posteriori
+// error estimate is: the steps of a power iteration shrink by r = lambda2 /
+// lambda1 of E E^T each time, r ~ sqrt(d2 / d2p), and u_i lies about
+// |u_i - u_{i-1}| r / (1 - r) from the limit -- used when r <= 1/2, stop when
+// its square is below est2.  Always at maxit.
+__device__ __forceinline__ bool power_stop(int i, float d2, float d2p, float tol2, float est2, int maxit)
+{
+    if (i >= maxit) return true;
+    if (i < 2) return false;
+    if (d2 < tol2) return true;
+    if (!(d2p > 0.f) || d2 > 0.25f * d2p) return false;
+    const float r = sqrtf(d2 / d2p);
+    const float e = r / (1.f - r);
+    return d2 * e * e < est2;
+}
 
 __device__ __forceinline__ int atom_m(const int *uoff, int k, int cap, int *c_out)
 {
@@ -308,8 +327,7 @@ __device__ __forceinline__ float block_sum_t2(float v, float *red)
 
 // Top eigenvector of the symmetric dim x dim matrix Gs (LDS) by power
 // iteration from u (LDS, overwritten with the result), T2 threads, 4 lanes per
-// row.  Stops when two successive unit iterates differ by less than sqrt(tol2)
-// (after at least 2 steps) or after maxit steps; a zero product keeps u.
+// row.  Stops by power_stop; a zero product keeps u.
 // Returns the number of steps.
 // Sums of a and of b over the T2 threads of a workgroup (fixed order),
 // returned to every thread; red2 holds 2 x (T2 / 64) float2 and `phase`
@@ -340,15 +358,15 @@ __device__ __forceinline__ float2 block_sum2(float a, float b, float2 *red2, int
 // pitch ldg), T2 threads, 4 per row, each holding its quarter of a row of G
 // in registers (quarters of ql = 4 * ceil(dim / 16) entries, ql <= QL; u must
 // hold 4 * QL floats, zero past dim; zb is scratch for 2 x 4 * QL floats).
-// Stops at the first iterate u_i with i >= 2 and |u_i - u_{i-1}|^2 < tol2,
-// or at i = maxit; a zero product keeps u_i.  One barrier per pass: pass i
+// Stops at the first iterate u_i that power_stop accepts; a zero product
+// keeps u_i.  One barrier per pass: pass i
 // reads z_i = G u_{i-1} unnormalised and its norm n_i (from the previous
 // pass's reduction), forms u_i = z_i / n_i on the fly, writes z_{i+1} =
 // G u_i to the other buffer, and one block reduction gives |z_{i+1}|^2 and
 // |u_i - u_{i-1}|^2 together.  Returns i; u holds u_i.
 template <int QL>
 __device__ int power_iteration_ps(const float *Gs, int ldg, float *u, float *zb, int dim, float2 *red2, float tol2,
-                                  int maxit)
+                                  float est2, int maxit)
 {
     const int tid = threadIdx.x;
     const int row = tid >> 2, part = tid & 3;
@@ -371,7 +389,7 @@ __device__ int power_iteration_ps(const float *Gs, int ldg, float *u, float *zb,
         zb[4 * QL + e] = 0.f;
     }
     const float *src = u;  // u_i = src * scale
-    float scale = 1.f, uprev = 0.f, ui = 0.f;
+    float scale = 1.f, uprev = 0.f, ui = 0.f, d2p = 0.f;
     int phase = 0;
     int it = 0;
     for (;;) {
@@ -395,7 +413,8 @@ __device__ int power_iteration_ps(const float *Gs, int ldg, float *u, float *zb,
         float *dst = zb + ((it + 1) & 1) * 4 * QL;
         if (owner) dst[row] = z;
         const float2 sz = block_sum2(owner ? z * z : 0.f, owner ? du * du : 0.f, red2, phase, act);
-        if ((it >= 2 && sz.y < tol2) || it >= maxit) break;
+        if (power_stop(it, sz.y, d2p, tol2, est2, maxit)) break;
+        d2p = sz.y;
         const float nz = sqrtf(sz.x);
         if (nz == 0.f) break;  // E = 0: keep u_i (d_k at the start; its coefficients become 0)
         uprev = ui;
@@ -408,13 +427,14 @@ __device__ int power_iteration_ps(const float *Gs, int ldg, float *u, float *zb,
     return it;
 }
 
-__device__ int power_iteration(const float *Gs, float *u, int dim, float *red, float tol2, int maxit)
+__device__ int power_iteration(const float *Gs, float *u, int dim, float *red, float tol2, float est2, int maxit)
 {
     const int tid = threadIdx.x;
     const int row = tid >> 2, part = tid & 3;
     const int qlen = (dim + 3) / 4;
     const int q0 = part * qlen, q1 = min(dim, q0 + qlen);
     int it = 0;
+    float d2p = 0.f;
     for (;;) {
         float z = 0.f;
         if (row < dim)
@@ -430,7 +450,8 @@ __device__ int power_iteration(const float *Gs, float *u, int dim, float *red, f
         if (owner) u[row] = un;
         __syncthreads();
         it++;
-        if ((it >= 2 && d2 < tol2) || it >= maxit) break;
+        if (power_stop(it, d2, d2p, tol2, est2, maxit)) break;
+        d2p = d2;
     }
     return it;
 }
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
     }
     for (int e = tid; e < dim; e += T2) u[e] = a.D[(int64_t)k * dim + e];
     __syncthreads();
-    const int it = power_iteration(Gs, u, dim, red, a.tol2, a.maxit);
+    const int it = power_iteration(Gs, u, dim, red, a.tol2, a.est2, a.maxit);
     for (int e = tid; e < dim; e += T2) a.D[(int64_t)k * dim + e] = u[e];
     if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
     // ---- handoff: atom k's new coefficient for every table entry of atom
@@ -678,7 +699,14 @@ __device__ bool ps_wait3(unsigned *flags, int f0, int n0, unsigned v0, int f1, i
                          unsigned v2, int *bail)
 {
     const int t = (int)threadIdx.x;
-    if (t < n0 + n1 + n2) {
+    // bail[1] holds the launch's spin limit; 0 (tests only) gives up at the
+    // first wait, so that an abort is certain
+    if (bail[1] == 0 && t == 0) {
+        st_flag(flags + PSF_STICKY, 1u);
+        st_flag(flags + PSF_ABORT, 1u);
+        *bail = 1;
+    }
+    if (t < n0 + n1 + n2 && bail[1] != 0) {
         const unsigned *f = t < n0 ? flags + f0 + t : t < n0 + n1 ? flags + f1 + (t - n0) : flags + f2 + (t - n0 - n1);
         const unsigned v = t < n0 ? v0 : t < n0 + n1 ? v1 : v2;
         long spins = 0;
@@ -816,7 +844,7 @@ __device__ void ps_master(const SweepArgs &a, unsigned *flags, int H, const floa
             }
             for (int e = tid; e < 4 * QL; e += PS_T) u[e] = e < dim ? ld_c(a.D + (int64_t)k * dim + e) : 0.f;
             __syncthreads();
-            iters += (unsigned long long)power_iteration_ps<QL>(Gs, dpad, u, zb, dim, red2, a.tol2, a.maxit);
+            iters += (unsigned long long)power_iteration_ps<QL>(Gs, dpad, u, zb, dim, red2, a.tol2, a.est2, a.maxit);
             for (int e = tid; e < dim; e += PS_T) a.D[(int64_t)k * dim + e] = u[e];
         }
         ps_publish(flags + PSF_AFLAG, (unsigned)(k + 1));
@@ -1050,19 +1078,20 @@ template <int TPW, int NO, int BU>
 __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *flags, int H, float *Gred)
 {
     extern __shared__ __attribute__((aligned(16))) float sm[];
-    __shared__ int s_role, s_bail;
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
     if (threadIdx.x == 0) {
         s_role = ps_register(flags, H + 1);
-        s_bail = 0;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
     }
     __syncthreads();
     const int role = s_role;
     if (role < 0) return;
     // Gram entries per master thread: 4 * QL >= dpad (TPW <= 3: dpad <= 144)
-    if (role == 0) ps_master<TPW <= 3 ? 36 : 48>(a, flags, H, Gred, sm, &s_bail);
-    else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, Gred, sm, &s_bail);
+    if (role == 0) ps_master<TPW <= 3 ? 36 : 48>(a, flags, H, Gred, sm, s_bail);
+    else ps_helper<TPW, NO, BU>(a, flags, H, role - 1, Gred, sm, s_bail);
     // a role that ran to its end (a bail makes every thread leave early)
-    if (threadIdx.x == 0 && s_bail == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
 // ===========================================================================
@@ -1090,8 +1119,8 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 //   P2  once team 1-t's d_{k-1} pairs (tagged k) are in: for each column
 //       whose patch is among atom k-1's first m_{k-1} users (rank r):
 //       x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x (column r prefetched);
-//   PI  exchanges i = 1, 2, ... until |u_i - u_{i-1}|^2 < tol2 with i >= 2,
-//       or i = maxit, or z_i = 0 (then u_{i-1} stays).  After exchange 1's
+//   PI  exchanges i = 1, 2, ... until power_stop accepts u_i, or z_i = 0
+//       (then u_{i-1} stays).  After exchange 1's
 //       store: its E_k columns to global memory; ecert[h] = k + 1 once
 //       exchange 1's vectors are in (after a store wait, which by then finds
 //       the columns out).  PT_PW wavefronts form w_h; every wavefront takes
@@ -1425,6 +1454,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         // helpers gq, gq + ngrp, ..., then the groups in order), then u_i;
         // returns -1 to leave (abort), 1 when the iteration stops, else 0
         int it = 0;
+        float d2prev = 0.f;
         auto collect = [&](unsigned s, bool cert) -> int {
             const unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
             const int ngrp = min(PT_T / dim, PT_GRP);
@@ -1507,7 +1537,8 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                         uj[j] = un;
                     }
                     d2 = wave_sum_dpp(d2);
-                    dec = ((it + 1 >= 2 && d2 < a.tol2) || it + 1 >= a.maxit) ? 2 : 0;
+                    dec = power_stop(it + 1, d2, d2prev, a.tol2, a.est2, a.maxit) ? 2 : 0;
+                    d2prev = d2;
                 }
             }
             if (dec != 1) it++;
@@ -1580,14 +1611,15 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         l1_drop();
         pa_build<NO, BU, false>(a, ncn, Es, sp, sat, sci);
         // the columns of E_{k+1} that atom k+2's hand-over needs
+        // (L1-bypassing loads: a second invalidation here cost 0.12 ms per
+        // update at 2048 x 2048; the columns are few)
         if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
-        l1_drop();
         {
-            const int nh = *nhp, d4 = dpad / 4;
+            const int nh = *nhp;
             const float *Enx = (kn & 1) ? a.E0 : a.E1;  // E_{k+1}
-            for (int q = tid; q < nh * d4; q += PT_T) {
-                const int i = q / d4, c4 = q - i * d4;
-                *(f4 *)(Eh + i * dpad + 4 * c4) = *(const f4 *)(Enx + (size_t)hrank[hl[i]] * dpad + 4 * c4);
+            for (int q = tid; q < nh * dpad; q += PT_T) {
+                const int i = q / dpad, c = q - i * dpad;
+                Eh[i * dpad + c] = ld_c(Enx + (size_t)hrank[hl[i]] * dpad + c);
             }
         }
         eh_ready = true;
@@ -1600,16 +1632,17 @@ template <int NO, int BU>
 __global__ __launch_bounds__(PT_T) void pt_sweep_kernel(SweepArgs a, unsigned *flags, unsigned long long *Wx)
 {
     extern __shared__ __attribute__((aligned(16))) float sm[];
-    __shared__ int s_role, s_bail;
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
     if (threadIdx.x == 0) {
         s_role = ps_register(flags, 2 * PT_HT);
-        s_bail = 0;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
     }
     __syncthreads();
     const int role = s_role;
     if (role < 0) return;
-    pt_helper<NO, BU>(a, flags, role & 1, role >> 1, Wx, sm, &s_bail);
-    if (threadIdx.x == 0 && s_bail == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+    pt_helper<NO, BU>(a, flags, role & 1, role >> 1, Wx, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
 // Clears the sweep's flags (all but the sticky abort word and the role
@@ -1762,6 +1795,7 @@ __global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float
     if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
     __syncthreads();
     int it = 0;
+    float d2p = 0.f;
     for (;;) {
         float s = 0.f;
         if (sl < ns) {
@@ -1788,7 +1822,8 @@ __global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float
         it++;
         if (tid < dim) u[tid] = un;
         __syncthreads();
-        if ((it >= 2 && d2 < a.tol2) || it >= a.maxit) break;
+        if (power_stop(it, d2, d2p, a.tol2, a.est2, a.maxit)) break;
+        d2p = d2;
     }
     if (tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
     if (tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
@@ -1879,6 +1914,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.ntiles = ntiles;
     a.maxit = 300;
     a.tol2 = tol * tol;
+    a.est2 = 1e-12f;  // estimated error of the kept atom below 1e-6
     const int tpw = (ntiles + 15) / 16;
     const bool wide = kmax > 5;
 
@@ -1906,7 +1942,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     // KSVD_SWEEP_SPIN (tests): fewer polls before a persistent sweep's wait
     // gives up, so that the abort and the two-kernel fallback can be exercised
     unsigned spin = (unsigned)PS_SPIN_MAX;
-    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(1, atoi(sv));
+    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(0, atoi(sv));
     const bool two_kernel = ctx->force_two_kernel || env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +

@@ -265,11 +265,27 @@ static bool graphs_off()
     return ng && ng[0] && ng[0] != '0';
 }
 
+// Persistent sweeps of different contexts on one device must not run at the
+// same time: each needs all the CUs of one XCD, and two sharing them would
+// each wait for workgroups the other holds until their waits give up.  So
+// every update on a device waits (on the GPU, not the host) for the last
+// update enqueued on that device by any context; the wait, the launch and
+// the record are ordered under one lock.  The lock also covers every graph
+// capture below: the last update's event may have been recorded on a stream
+// that another thread is capturing, and HIP refuses a wait on such an event
+// from a stream that is not capturing.
+namespace {
+constexpr int UPD_MAX_DEVICES = 64;
+std::mutex upd_mu;
+hipEvent_t upd_last[UPD_MAX_DEVICES];
+}  // namespace
+
 // The update's launches depend only on the context's buffers and sizes, so
 // they are captured once per buffer layout into a HIP graph (KSVD_NO_GRAPH=1
 // launches them one by one).  ksvd_set_image prepares the graph right after
 // laying the buffers out, so that no K-SVD iteration pays the capture.
-int prepare_update_graph(ksvd_ctx *ctx)
+// Called with upd_mu held.
+static int prepare_update_graph_locked(ksvd_ctx *ctx)
 {
     if (graphs_off() || ctx->upd_exec) return KSVD_OK;
     hipGraph_t g = nullptr;
@@ -288,17 +304,11 @@ int prepare_update_graph(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
-// Persistent sweeps of different contexts on one device must not run at the
-// same time: each needs all the CUs of one XCD, and two sharing them would
-// each wait for workgroups the other holds until their waits give up.  So
-// every update on a device waits (on the GPU, not the host) for the last
-// update enqueued on that device by any context; the wait, the launch and
-// the record are ordered under one lock.
-namespace {
-constexpr int UPD_MAX_DEVICES = 64;
-std::mutex upd_mu;
-hipEvent_t upd_last[UPD_MAX_DEVICES];
-}  // namespace
+int prepare_update_graph(ksvd_ctx *ctx)
+{
+    std::lock_guard<std::mutex> lock(upd_mu);
+    return prepare_update_graph_locked(ctx);
+}
 
 int launch_update(ksvd_ctx *ctx)
 {
@@ -315,7 +325,7 @@ int launch_update(ksvd_ctx *ctx)
         rc = enqueue_update(ctx);
         if (rc) return rc;
     } else {
-        rc = prepare_update_graph(ctx);
+        rc = prepare_update_graph_locked(ctx);
         if (rc) return rc;
         KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
     }

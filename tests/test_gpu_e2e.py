@@ -4,10 +4,11 @@ spectral gap.
 
 Bars (the plain form's contract; cuda_svd_denoiser.cu:368-408, :413-481):
   * full K-SVD (10 iterations of OMP + Gauss-Seidel sweep, the final OMP and
-    the rebuild) on ten seeds at 512x512 and three at 2048x2048: the mean of
-    (gpu - FP32 oracle) raw PSNR within 0.02 dB, and the rms of it no more
-    than 0.02 dB above the rms of (FP64 oracle - FP32 oracle), the spread that
-    rounding alone causes over ten chaotic iterations;
+    the rebuild) on 24 seeds at 512x512 and six at 2048x2048: the mean of
+    (gpu - FP64 oracle) raw PSNR within 0.02 dB plus twice its standard error
+    under FP32 rounding (the FP32 oracle's spread), and the rms of it no more
+    than 0.01 dB above the rms of (FP32 oracle - FP64 oracle), the spread that
+    FP32 rounding alone causes over ten chaotic iterations;
   * each iteration of the GPU's trajectory restarted in the oracle: raw PSNR
     after one step within 0.02 dB;
   * the first OMP at 2048x2048: identical supports on >= 99.9 % of patches,
@@ -140,18 +141,23 @@ def rms(v):
     return float(np.sqrt(np.mean(np.square(v))))
 
 
-@pytest.mark.parametrize("cfg,seeds,perturbed", [(CFG2, range(2, 12), True), (CFG3, (3, 4, 5), False)],
-                         ids=["512sq_10seeds", "2048sq_3seeds"])
+@pytest.mark.parametrize("cfg,seeds,perturbed", [(CFG2, range(2, 26), True), (CFG3, range(3, 9), False)],
+                         ids=["512sq_24seeds", "2048sq_6seeds"])
 def test_end_to_end_psnr(kl, orc, cfg, seeds, perturbed):
     """Full K-SVD, final OMP and rebuild on several seeds.  Ten K-SVD
     iterations amplify any rounding: the FP64 oracle, or the FP32 oracle from
-    a dictionary one ulp away, ends a few hundredths of a dB from the FP32
-    oracle (whose own sums are one arbitrary FP32 order among many).  So the
-    0.02 dB bar holds (a) for the mean over seeds of (gpu - oracle32), the
-    systematic part, and (b) for the GPU's spread around the FP32 oracle
-    beyond the FP64 oracle's: rms(gpu - oracle32) <= rms(oracle64 - oracle32)
-    + 0.02 dB.  The normalized PSNR (the reference's report) hinges on the
-    picture's extreme pixels and is reported only; each seed must denoise."""
+    a dictionary one ulp away, ends a few hundredths of a dB (up to 0.09 dB at
+    2048^2) from the FP32 oracle, whose own sums are one arbitrary FP32 order
+    among many, like the reference's CUDA sums.  So the FP64 oracle stands for
+    the algorithm and the 0.02 dB bar holds (a) for the mean over seeds of
+    (gpu - oracle64), the systematic part, up to twice its standard error
+    under FP32 rounding alone (2 rms(oracle32 - oracle64) / sqrt(seeds):
+    0.01 dB on 24 seeds at 512^2, 0.04 dB on 6 at 2048^2), while (b) the
+    GPU's spread around the FP64 oracle is no wider than the FP32 oracle's:
+    rms(gpu - oracle64) <= rms(oracle32 - oracle64) + 0.01 dB.  gpu - oracle32
+    is reported.  The
+    normalized PSNR (the reference's report) hinges on the picture's extreme
+    pixels and is reported only; each seed must denoise."""
     rows = []
     for seed in seeds:
         t0 = time.time()
@@ -173,8 +179,11 @@ def test_end_to_end_psnr(kl, orc, cfg, seeds, perturbed):
          + (f", rms one-ulp move {rms(a[:, 3]):.4f}" if perturbed else "")
          + f"; normalized PSNR (min-max stretch, set by single extreme pixels): rms gpu - oracle32 {rms(a[:, 4]):.3f}, "
          f"oracle64 - oracle32 {rms(a[:, 5]):.3f}")
-    assert abs(a[:, 0].mean()) <= 0.02
-    assert rms(a[:, 0]) <= rms(a[:, 1]) + 0.02
+    se = rms(a[:, 1]) / np.sqrt(len(a))
+    _say(f"bars: |mean gpu - oracle64| {abs(a[:, 2].mean()):.4f} <= 0.02 + 2 x {se:.4f}; rms gpu - oracle64 "
+         f"{rms(a[:, 2]):.4f} <= {rms(a[:, 1]):.4f} + 0.01")
+    assert abs(a[:, 2].mean()) <= 0.02 + 2 * se
+    assert rms(a[:, 2]) <= rms(a[:, 1]) + 0.01
 
 
 @pytest.mark.parametrize("cfg", [CFG2, CFG3], ids=["512sq", "2048sq"])

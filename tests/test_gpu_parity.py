@@ -623,14 +623,14 @@ def test_two_threads_two_contexts(kl):
 
 def test_sweep_that_gives_up_is_undone_and_run_again(kl, monkeypatch):
     """A persistent sweep whose waits give up (forced from the host: a spin
-    limit of a few polls) is undone from its snapshot and run again with the
+    limit of 0 polls, its first wait gives up) is undone from its snapshot and run again with the
     two-kernel sweep: the result is the two-kernel sweep's, bit for bit, for
     a single update and for ksvd_iterate, and the fallbacks are counted."""
     frame = speckle_frame(kl)
     monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1")
     d2, c2, _ = sweep_run(kl, frame, 96, 80)
     monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "0")
-    monkeypatch.setenv("KSVD_SWEEP_SPIN", "1")
+    monkeypatch.setenv("KSVD_SWEEP_SPIN", "0")
     d, c, t = sweep_run(kl, frame, 96, 80)
     print(f"\nsweep fallbacks in ksvd_run: {t['sweep_fallbacks']}")
     assert t["sweep_fallbacks"] == 1
