@@ -266,7 +266,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
                       &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
                       &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
                       &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
-                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt, &ctx->c0};
+                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->mlp_rej, &ctx->mlp_bcnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt, &ctx->c0};
     for (DevBuf *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -551,6 +551,14 @@ int ksvd_update_dictionary(ksvd_ctx *ctx)
 
 static int proposer_pass(ksvd_ctx *ctx, int64_t n, bool probe);
 
+// KSVD_PROPOSER_MIN_ACCEPT (tests): the acceptance below which the proposer
+// stands down; 0 keeps it proposing whatever it accepts
+static double min_accept()
+{
+    const char *v = getenv("KSVD_PROPOSER_MIN_ACCEPT");
+    return v && v[0] ? atof(v) : (double)PROP_MIN_ACCEPT;
+}
+
 // Queues the copy of the last pass's fallback count (n patches) to the host.
 static int post_gate(ksvd_ctx *ctx, int64_t n, bool probe)
 {
@@ -577,7 +585,7 @@ static int proposer_step(ksvd_ctx *ctx)
         KSVD_HIP_CHECK(ctx, hipEventSynchronize((hipEvent_t)ctx->gate_ev));
         ctx->gate_pending = false;
         const int64_t n = ctx->gate_n;
-        if ((double)(n - (int64_t)*ctx->gate_host) < PROP_MIN_ACCEPT * (double)n) {
+        if ((double)(n - (int64_t)*ctx->gate_host) < min_accept() * (double)n) {
             ctx->mlp_gated = true;
         } else if (ctx->gate_probe) {
             if ((rc = flush_cap_zero(ctx))) return rc;

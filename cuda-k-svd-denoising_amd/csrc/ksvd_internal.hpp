@@ -74,6 +74,7 @@ struct ksvd_ctx {
     uint64_t mlp_perm_seed = 0;   // mlp_perm holds the seeded permutation of mlp_perm_n patches
     int64_t mlp_perm_n = -1;
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
+    DevBuf mlp_rej, mlp_bcnt;                                // per-patch fallback flags, per-block counts
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
     // The two-team sweep leaves the coefficients of the users beyond the cap

@@ -383,15 +383,15 @@ __global__ void f32_to_bf16_kernel(const float *__restrict__ s, __bf16 *__restri
 // patch with the FP32 atoms (Cholesky of the support Gram from the FP32
 // dictionary Gram, two triangular solves); accepted when the residual is at
 // most accept x the patch's last OMP residual and every pivot exceeds
-// 1e-6 of its Gram diagonal, else the patch joins the fallback list.
+// 1e-6 of its Gram diagonal, else the patch is flagged for the fallback
+// list (rej[p] = 1), which fallback_compact builds in ascending patch order.
 template <int KM, int RJ>
 __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__ Z, int ko, int K,
                                                       const float *__restrict__ Y, int dim, int64_t p0, int64_t np,
                                                       const float *__restrict__ D, const float *__restrict__ G,
                                                       int kmax, const float *__restrict__ rbound, float accept,
                                                       int write, int *__restrict__ idx_out, float *__restrict__ coef_out,
-                                                      int *__restrict__ cnt_out, int *__restrict__ fb_list,
-                                                      unsigned *__restrict__ fb_count)
+                                                      int *__restrict__ cnt_out, unsigned char *__restrict__ rej)
 {
     const int lane = threadIdx.x & 63;
     const int sub = lane & 15;
@@ -515,21 +515,78 @@ __global__ __launch_bounds__(256) void propose_kernel(const __bf16 *__restrict__
     for (int off = 1; off < 16; off <<= 1) nr += __shfl_xor(nr, off);
     const bool accept_it = ok && sqrtf(nr) <= accept * rbound[p];
     if (valid && sub == 0) {
-        if (accept_it) {
-            if (write) {
-                cnt_out[p] = kmax;
+        rej[p] = accept_it ? 0 : 1;
+        if (accept_it && write) {
+            cnt_out[p] = kmax;
 #pragma unroll
-                for (int i = 0; i < KM; i++)
-                    if (i < kmax) {
-                        idx_out[p * kmax + i] = S[i];
-                        coef_out[p * kmax + i] = x[i];
-                    }
-            }
-        } else {
-            const unsigned slot = atomicAdd(fb_count, 1u);
-            if (write) fb_list[slot] = (int)p;
+            for (int i = 0; i < KM; i++)
+                if (i < kmax) {
+                    idx_out[p * kmax + i] = S[i];
+                    coef_out[p * kmax + i] = x[i];
+                }
         }
     }
+}
+
+// ---- fallback list: the flagged patches in ascending order ----------------
+// Three passes over n flags: counts per block of 1024, an exclusive scan of
+// the counts (one workgroup, carried over tiles of 1024; the total goes to
+// *total), and the stores at each block's offset by wavefront ballots.  The
+// list is the same whatever order the proposals ran in, so the training
+// batches cut from it are too.
+constexpr int FB_B = 1024;
+
+__global__ __launch_bounds__(FB_B) void fb_count_kernel(const unsigned char *__restrict__ f, int64_t n,
+                                                        int *__restrict__ bcnt)
+{
+    __shared__ int ws[FB_B / 64];
+    const int64_t i = (int64_t)blockIdx.x * FB_B + threadIdx.x;
+    const unsigned long long b = __ballot(i < n && f[i] != 0);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < FB_B / 64; w++) t += ws[w];
+        bcnt[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(FB_B) void fb_scan_kernel(int *__restrict__ bcnt, int nb, unsigned *__restrict__ total)
+{
+    __shared__ int sh[FB_B];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += FB_B) {
+        const int i = b0 + (int)threadIdx.x;
+        const int v = i < nb ? bcnt[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        // Hillis-Steele inclusive scan in LDS
+        for (int off = 1; off < FB_B; off <<= 1) {
+            const int a = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nb) bcnt[i] = carry + sh[threadIdx.x] - v;
+        carry += sh[FB_B - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = (unsigned)carry;
+}
+
+__global__ __launch_bounds__(FB_B) void fb_store_kernel(const unsigned char *__restrict__ f, int64_t n,
+                                                        const int *__restrict__ boff, int *__restrict__ list)
+{
+    __shared__ int ws[FB_B / 64];
+    const int64_t i = (int64_t)blockIdx.x * FB_B + threadIdx.x;
+    const bool on = i < n && f[i] != 0;
+    const unsigned long long b = __ballot(on);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = __popcll(b);
+    __syncthreads();
+    int off = boff[blockIdx.x];
+    for (int v = 0; v < w; v++) off += ws[v];
+    if (on) list[off + __popcll(b & ((1ull << lane) - 1ull))] = (int)i;
 }
 
 // Host-side helpers ----------------------------------------------------------
@@ -979,6 +1036,8 @@ int mlp_prepare(ksvd_ctx *ctx, uint64_t init_seed, uint64_t perm_seed)
     if ((rc = ksvd_reserve(ctx, ctx->mlp_h2c, sizeof(__bf16) * CH * MLP_HP))) return rc;
     if ((rc = ksvd_reserve(ctx, ctx->mlp_zc, sizeof(__bf16) * CH * ko))) return rc;
     if ((rc = ksvd_reserve(ctx, ctx->mlp_fb, sizeof(int) * (size_t)ctx->P))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_rej, (size_t)ctx->P))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_bcnt, sizeof(int) * (size_t)((ctx->P + FB_B - 1) / FB_B)))) return rc;
     return mlp_permutation(ctx, perm_seed);
 }
 
@@ -1010,6 +1069,9 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write)
     if ((rc = ksvd_reserve(ctx, ctx->mlp_h2c, sizeof(__bf16) * CH * MLP_HP))) return rc;
     if ((rc = ksvd_reserve(ctx, ctx->mlp_zc, sizeof(__bf16) * CH * ko))) return rc;
     if ((rc = ksvd_reserve(ctx, ctx->mlp_fb, sizeof(int) * (size_t)P))) return rc;
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_rej, (size_t)P))) return rc;
+    const int nfb = (int)((P + FB_B - 1) / FB_B);
+    if ((rc = ksvd_reserve(ctx, ctx->mlp_bcnt, sizeof(int) * (size_t)nfb))) return rc;
     if (ctx->mlp_chunk_rows < CH) {
         hipLaunchKernelGGL(ones_column_kernel, dim3((unsigned)((CH + 255) / 256)), dim3(256), 0, ctx->stream,
                            (__bf16 *)ctx->mlp_h1c.ptr, MLP_HP, MLP_H, CH);
@@ -1017,7 +1079,6 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write)
                            (__bf16 *)ctx->mlp_h2c.ptr, MLP_HP, MLP_H, CH);
         ctx->mlp_chunk_rows = CH;
     }
-    KSVD_HIP_CHECK(ctx, hipMemsetAsync(ctx->mlp_cnt.ptr, 0, sizeof(unsigned), ctx->stream));
     const __bf16 *W1 = (const __bf16 *)ctx->mlp_wb.ptr, *W2 = W1 + (size_t)MLP_H * kin,
                  *W3 = W2 + (size_t)MLP_H * MLP_HP;
     __bf16 *H1 = (__bf16 *)ctx->mlp_h1c.ptr, *H2 = (__bf16 *)ctx->mlp_h2c.ptr, *Z = (__bf16 *)ctx->mlp_zc.ptr;
@@ -1038,8 +1099,8 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write)
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, (const __bf16 *)Z, ko, ctx->K,
                                (const float *)ctx->Y.ptr, ctx->dim, p0, (int64_t)nb, (const float *)ctx->D.ptr,
                                (const float *)ctx->G.ptr, ctx->kmax, (const float *)ctx->rbound.ptr, accept,
-                               write ? 1 : 0, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr, (int *)ctx->mlp_fb.ptr,
-                               (unsigned *)ctx->mlp_cnt.ptr);
+                               write ? 1 : 0, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
+                               (unsigned char *)ctx->mlp_rej.ptr);
         };
         const int rj = (ctx->dim + 15) / 16;
         if (ctx->kmax <= 5) {
@@ -1051,5 +1112,15 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write)
         }
         KSVD_HIP_CHECK(ctx, hipGetLastError());
     }
+    // the fallback count (and, when the codes are kept, the list)
+    const unsigned char *rej = (const unsigned char *)ctx->mlp_rej.ptr;
+    int *bcnt = (int *)ctx->mlp_bcnt.ptr;
+    if (nfb > 0)
+        hipLaunchKernelGGL(fb_count_kernel, dim3((unsigned)nfb), dim3(FB_B), 0, ctx->stream, rej, P, bcnt);
+    hipLaunchKernelGGL(fb_scan_kernel, dim3(1), dim3(FB_B), 0, ctx->stream, bcnt, nfb, (unsigned *)ctx->mlp_cnt.ptr);
+    if (write && nfb > 0)
+        hipLaunchKernelGGL(fb_store_kernel, dim3((unsigned)nfb), dim3(FB_B), 0, ctx->stream, rej, P,
+                           (const int *)bcnt, (int *)ctx->mlp_fb.ptr);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
     return KSVD_OK;
 }

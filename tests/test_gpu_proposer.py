@@ -12,7 +12,9 @@
   * the fallback list through the FP32 pursuit gives the full pass's codes
     bit for bit;
   * end to end, BF16 + learned: PSNR no more than 0.05 dB below the plain FP32
-    form, fallback rate reported.
+    form, fallback rate reported;
+  * determinism: two runs give the same bits, on the stand-down path and on
+    the trained-proposal path (forced by a zero acceptance gate).
 """
 import numpy as np
 import pytest
@@ -160,6 +162,40 @@ def test_learned_run_deterministic(kl):
         np.testing.assert_array_equal(x, y)
     np.testing.assert_array_equal(a[2], b[2])
     assert a[3:] == b[3:]
+
+
+def test_learned_trained_path_deterministic(kl, monkeypatch):
+    """The trained-proposal path (the proposer kept whatever it accepts, so
+    every iteration after the probe proposes, codes its fallback list with
+    OMP and trains on that list) twice on the same image gives the same bits:
+    the fallback list is built in ascending patch order, whatever order the
+    proposals ran in, so the training batches cut from it are the same."""
+    monkeypatch.setenv("KSVD_PROPOSER_MIN_ACCEPT", "0")
+    cfg = CFG2
+    res = []
+    for _ in range(2):
+        clean, noisy, ks = make(kl, cfg, precision=kl.PREC_BF16, proposer=1)
+        ks.extract()
+        ks.init_dictionary()
+        ks.run()
+        t = ks.timings()
+        res.append((ks.dictionary(), ks.codes(), ks.rebuild_picture(kl.noise_variance(noisy)), t["proposed"],
+                    t["fallback"], t["train_loss"], t["proposer_gated"]))
+        ks.close()
+    a, b = res
+    P = a[1][2].shape[0]
+    print(f"\ntrained path twice: proposed {a[3]} / {b[3]} ({P} patches), fallback {a[4]} / {b[4]}, "
+          f"loss {a[5]:.5f} / {b[5]:.5f}, gated {a[6]} / {b[6]}; PSNR raw {kl.psnr(clean, a[2]):.4f}")
+    assert a[6] == 0
+    # the probe, then a full pass per later iteration and the final code
+    assert a[3] > 2 * P
+    assert 0 < a[4] < a[3]
+    np.testing.assert_array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[2], b[2])
+    assert a[3:] == b[3:]
+    assert kl.psnr(clean, a[2]) > kl.psnr(clean, noisy) + 3
 
 
 def test_bf16_learned_end_to_end_psnr(kl):
