@@ -1181,11 +1181,14 @@ __device__ __forceinline__ int pa_m(const int *suo, int k, int cap)
 // order).  An entry of an atom j < kb-1 (updated already) whose patch is not
 // among atom j's first m_j users has coefficient 0 from atom j's update on;
 // its coefficient word is cleared only after the sweep, so it is dropped here.
+// sx: 1 for a column whose patch is also among atom kb+1's first m_{kb+1}
+// users (the only columns of E_kb that atom kb+1's hand-over reads).
 template <int NO>
 __device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int kb, int tb, int nc, int mx, int *sp,
-                         int *ss, int *sat, int *sci, int *hrank, int *hl, int *nhp)
+                         int *ss, int *sat, int *sci, int *hrank, int *hl, int *nhp, int *sx)
 {
     const int no = a.kmax > 1 ? a.kmax - 1 : 1;
+    const int mnext = kb + 1 < a.K ? pa_m(suo, kb + 1, a.cap) : 0;
     const int64_t cb = (int64_t)smo[kb] + tb;
     for (int tl = threadIdx.x; tl < nc; tl += PT_T) {
         const int64_t col = cb + tl;
@@ -1196,11 +1199,12 @@ __device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int
         int4 o[NO];
 #pragma unroll
         for (int i = 0; i < NO; i++) o[i] = a.tab_oth[col * no + min(i, no - 1)];
-        int hr = -1;
+        int hr = -1, ex = 0;
 #pragma unroll
         for (int i = 0; i < NO; i++) {
             int at = i < no ? o[i].x : -1;
             const int rk = o[i].z;
+            if (at == kb + 1 && rk >= 0 && rk < mnext) ex = 1;
             if (at >= 0 && at == kb - 1) {
                 if (rk >= 0 && rk < mx) hr = rk;
                 at = -1;
@@ -1211,6 +1215,7 @@ __device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int
             sci[tl * PS_NOMAX + i] = at >= 0 ? o[i].y : -1;
         }
         hrank[tl] = hr;
+        sx[tl] = ex;
         if (hr >= 0) hl[atomicAdd(nhp, 1)] = tl;
     }
 }
@@ -1291,7 +1296,8 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     int *hl = hrank + PS_CWMAX;               // PS_CWMAX: columns with a hand-over
     int *nhp = hl + PS_CWMAX;                 // their number
     int *ss = nhp + 1;                        // PS_CWMAX: own coefficient slots of the columns
-    int *suo = ss + PS_CWMAX;                 // K + 1: uoff
+    int *sx = ss + PS_CWMAX;                  // PS_CWMAX: 1 for a column atom k+1's hand-over reads
+    int *suo = sx + PS_CWMAX;                 // K + 1: uoff
     int *smo = suo + K + 1;                   // K + 1: moff
     unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
     const int me = team * PT_HT + h, other = (1 - team) * PT_HT;
@@ -1313,7 +1319,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     if (k < K) {
         int tb0;
         const int nc0 = cols_of(pa_m(suo, k, cap), H, h, &tb0);
-        pa_table<NO>(a, suo, smo, k, tb0, nc0, k >= 1 ? pa_m(suo, k - 1, cap) : 0, sp, ss, sat, sci, hrank, hl, nhp);
+        pa_table<NO>(a, suo, smo, k, tb0, nc0, k >= 1 ? pa_m(suo, k - 1, cap) : 0, sp, ss, sat, sci, hrank, hl, nhp, sx);
         if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)k * dim + tid);
         __syncthreads();
         pa_build<NO, BU>(a, nc0, Es, sp, sat, sci);
@@ -1395,12 +1401,13 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             const int e = lane + 64 * j;
             uj[j] = e < dim ? d0[e] : 0.f;
         }
-        // E_k's columns out for team 1-t's hand-over of atom k+1, and ecert
+        // E_k's columns that team 1-t's hand-over of atom k+1 reads out, and ecert
         auto store_e = [&](bool publish) {
             const int d4 = dpad / 4;
             for (int q = tid; q < nc * d4; q += PT_T) {
                 const int tl = q / d4, c4 = q - tl * d4;
-                *(f4 *)(Ecur + (int64_t)(tb + tl) * dpad + 4 * c4) = *(const f4 *)(Es + tl * dpad + 4 * c4);
+                if (sx[tl])
+                    *(f4 *)(Ecur + (int64_t)(tb + tl) * dpad + 4 * c4) = *(const f4 *)(Es + tl * dpad + 4 * c4);
             }
             if (publish) ps_publish(ecert + me, (unsigned)(k + 1));
         };
@@ -1602,7 +1609,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         __syncthreads();
         int tbn;
         const int ncn = cols_of(pa_m(suo, kn, cap), H, h, &tbn);
-        pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp);
+        pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp, sx);
         if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
         __syncthreads();
         if (!ps_wait3(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), PTF_CCERT + other, k >= 1 ? H : 0,
@@ -1946,7 +1953,7 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const bool two_kernel = ctx->force_two_kernel || env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
-                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 4 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
+                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 5 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
             // BU = 5: a member's columns (at most 64) in two passes of 35 with every
