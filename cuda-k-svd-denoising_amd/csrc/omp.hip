@@ -33,13 +33,27 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int OMP_WAVES = 4;
 
+// v + (v at lane ^ 16) and v + (v at lane ^ 32) on the VALU: the permlane
+// swaps of a value with itself return, in their two results, every lane's
+// value and its partner's (rows 0 and 2 against rows 1 and 3; the lower half
+// against the upper), so the sum is the same in both lanes of a pair
+__device__ __forceinline__ float add_xor16(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float add_xor32(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 // Cross-lane sum over the four lanes that hold one patch (lanes col, col+16,
 // col+32, col+48).  Every lane ends with the same bits.
 __device__ __forceinline__ float patch_sum(float v)
 {
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+    return add_xor32(add_xor16(v));
 }
 
 template <int Q, int KT, int KM>

@@ -737,6 +737,22 @@ __device__ __forceinline__ bool ps_wait(unsigned *flags, int first, int count, u
     return ps_wait3(flags, first, count, v, 0, 0, 0u, 0, 0, 0u, bail);
 }
 
+// v + (v at lane ^ 16) and v + (v at lane ^ 32) on the VALU: the permlane
+// swaps of a value with itself return, in their two results, every lane's
+// value and its partner's (rows 0 and 2 against rows 1 and 3; the lower half
+// against the upper), so the sum is the same in both lanes of a pair
+__device__ __forceinline__ float add_xor16(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float add_xor32(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 // Sums over the 64 lanes of four values a[0..3] at once, returned to every
 // lane in t[0..3] (fixed order): exchanges over lane bits 0 and 1 leave each
 // lane one column's quad sum (column 2 (lane & 1) + ((lane >> 1) & 1)), then
@@ -751,8 +767,7 @@ __device__ __forceinline__ void reduce4_cols(const float *a, float *t)
     float v = (hi ? y : x) + dpp_f<0x4E, 0xF>(hi ? x : y);
     v += dpp_f<0x128, 0xF>(v);  // row_ror:8
     v += dpp_f<0x124, 0xF>(v);  // row_ror:4
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
+    v = add_xor32(add_xor16(v));
     t[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
     t[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 2));
     t[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 1));
