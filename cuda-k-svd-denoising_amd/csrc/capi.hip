@@ -134,7 +134,7 @@ int sync_copy_d2h(ksvd_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     int rc = resolve(ctx);
     if (rc) return rc;
-    KSVD_HIP_CHECK(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost));
     return KSVD_OK;
 }
 
@@ -152,11 +152,20 @@ int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes)
     if (b.bytes >= bytes) return KSVD_OK;
     if (b.ptr) {
         KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        KSVD_HIP_CHECK(ctx, hipFree(b.ptr));
+        hipError_t ef;
+        {
+            std::lock_guard<std::mutex> lock(device_op_mutex());
+            ef = hipFree(b.ptr);
+        }
+        KSVD_HIP_CHECK(ctx, ef);
         b.ptr = nullptr;
         b.bytes = 0;
     }
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> lock(device_op_mutex());
+        e = hipMalloc(&b.ptr, bytes);
+    }
     if (e != hipSuccess) {
         b.ptr = nullptr;
         return ksvd_fail(ctx, KSVD_ERR_ALLOC, std::string("hipMalloc of ") + std::to_string(bytes) +
@@ -223,6 +232,9 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
     hipDeviceProp_t prop;
     KSVD_HIP_CHECK(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    int nx = 0;
+    if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, ctx->device) == hipSuccess && nx > 0)
+        ctx->n_xcd = nx;
     KSVD_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->dim = p->patch_w * p->patch_h;
     ctx->K = p->atoms;
@@ -257,6 +269,8 @@ int ksvd_destroy(ksvd_ctx *ctx)
     }
     drop_timeline(ctx);
     drop_update_graph(ctx);
+    update_release(ctx);
+    std::lock_guard<std::mutex> lock(device_op_mutex());
     if (ctx->gate_ev) (void)hipEventDestroy((hipEvent_t)ctx->gate_ev);
     if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->G,       &ctx->idx,
@@ -354,8 +368,8 @@ static int set_image_common(ksvd_ctx *ctx, const float *buf, int m, int n, hipMe
     if (!same) drop_update_graph(ctx);
     // flags of the persistent sweep (its reset kernel keeps the sticky abort word)
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipMemset(ctx->ticket.ptr, 0, ctx->ticket.bytes));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
+    KSVD_HIP_CHECK(ctx, ksvd_set(ctx, ctx->ticket.ptr, 0, ctx->ticket.bytes));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->img.ptr, buf, sizeof(float) * (size_t)N, kind));
     if ((rc = prepare_update_graph(ctx))) return rc;
     ctx->has_image = true;
     ctx->has_patches = ctx->has_codes = ctx->has_rebuild = false;
@@ -383,7 +397,10 @@ constexpr uint64_t PROP_SEED = 0x5EEDull;
 // count reaches the host (created once per context).
 static int gate_setup(ksvd_ctx *ctx)
 {
-    if (!ctx->gate_host) KSVD_HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->gate_host, sizeof(unsigned), hipHostMallocDefault));
+    if (!ctx->gate_host) {
+        std::lock_guard<std::mutex> lock(device_op_mutex());
+        KSVD_HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->gate_host, sizeof(unsigned), hipHostMallocDefault));
+    }
     if (!ctx->gate_ev) {
         hipEvent_t e;
         KSVD_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -745,7 +762,7 @@ int ksvd_set_dictionary(ksvd_ctx *ctx, const float *D)
     int rc = set_device(ctx);
     if (rc) return rc;
     if ((rc = resolve(ctx))) return rc;
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->D.ptr, D, sizeof(float) * (size_t)ctx->dim * ctx->K, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->D.ptr, D, sizeof(float) * (size_t)ctx->dim * ctx->K, hipMemcpyHostToDevice));
     ctx->has_dict = true;
     ctx->gram_valid = false;
     ctx->has_rebuild = false;
@@ -789,9 +806,9 @@ int ksvd_set_codes(ksvd_ctx *ctx, const int32_t *idx, const float *coef, const i
     if (rc) return rc;
     if ((rc = resolve(ctx))) return rc;
     const size_t pk = (size_t)P * kmax;
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->idx.ptr, idx, sizeof(int) * pk, hipMemcpyHostToDevice));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->coef.ptr, coef, sizeof(float) * pk, hipMemcpyHostToDevice));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->cnt.ptr, cnt, sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->idx.ptr, idx, sizeof(int) * pk, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->coef.ptr, coef, sizeof(float) * pk, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->cnt.ptr, cnt, sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
     ctx->has_codes = true;
     ctx->has_rebuild = false;
     ctx->cap_zero_pending = false;
@@ -936,7 +953,7 @@ int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
     int rc = resolve(ctx);
     if (rc) return rc;
     unsigned long long c[CNT_SLOTS];
-    KSVD_HIP_CHECK(ctx, hipMemcpy(c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
     ctx->tm.omp_dup_stops = (int64_t)c[CNT_DUP];
     ctx->tm.omp_degenerate = (int64_t)c[CNT_DEGEN];
     ctx->tm.eig_iters = (int64_t)c[CNT_EIG_IT];
@@ -948,7 +965,7 @@ int ksvd_get_timings(ksvd_ctx *ctx, ksvd_timings *t)
     ctx->tm.fallback = (int64_t)c[CNT_FALLBACK];
     if (ctx->mlp_ready) {
         float l = 0.f;
-        KSVD_HIP_CHECK(ctx, hipMemcpy(&l, (const float *)ctx->mlp_cnt.ptr + 1, sizeof(float), hipMemcpyDeviceToHost));
+        KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, &l, (const float *)ctx->mlp_cnt.ptr + 1, sizeof(float), hipMemcpyDeviceToHost));
         ctx->tm.train_loss = l;
     }
     *t = ctx->tm;
@@ -961,7 +978,7 @@ int ksvd_reset_timings(ksvd_ctx *ctx)
     int rc = resolve(ctx);
     if (rc) return rc;
     ctx->tm = ksvd_timings{};
-    KSVD_HIP_CHECK(ctx, hipMemset(ctx->counters.ptr, 0, ctx->counters.bytes));
+    KSVD_HIP_CHECK(ctx, ksvd_set(ctx, ctx->counters.ptr, 0, ctx->counters.bytes));
     return KSVD_OK;
 }
 

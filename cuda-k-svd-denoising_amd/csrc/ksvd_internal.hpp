@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "ksvd_hip.h"
@@ -41,6 +42,7 @@ struct ksvd_ctx {
     ksvd_params prm{};
     int device = 0;
     int num_cus = 256;
+    int n_xcd = 8;  // XCDs of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
 
@@ -123,6 +125,21 @@ inline hipError_t allow_max_lds(const void *fn)
 // Grow a device buffer to at least `bytes` (contents not preserved).
 int ksvd_reserve(ksvd_ctx *ctx, DevBuf &b, size_t bytes);
 
+// Blocking copies and fills on the context's own stream: the legacy null
+// stream that hipMemcpy / hipMemset use may not wait on a stream that another
+// thread is capturing, and HIP refuses such a call.
+inline hipError_t ksvd_copy(ksvd_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx->stream);
+}
+
+inline hipError_t ksvd_set(ksvd_ctx *ctx, void *dst, int value, size_t bytes)
+{
+    const hipError_t e = hipMemsetAsync(dst, value, bytes, ctx->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx->stream);
+}
+
 #define KSVD_HIP_CHECK(ctx, call)                                                                    \
     do {                                                                                             \
         hipError_t e_ = (call);                                                                      \
@@ -162,6 +179,8 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write);
 size_t bf16_dict_bytes(int dim);
 size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);
+void update_release(ksvd_ctx *ctx);  // at destroy: the device's last-update event, if this context's
+std::mutex &device_op_mutex();       // held over allocations, frees and graph captures
 int prepare_update_graph(ksvd_ctx *ctx);  // capture the update's launches for the current buffers
 void drop_update_graph(ksvd_ctx *ctx);
 size_t sweep_gpart_bytes(int dim);

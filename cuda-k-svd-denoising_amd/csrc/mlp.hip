@@ -675,9 +675,9 @@ static int upload_theta(ksvd_ctx *ctx, const std::vector<float> &th)
 {
     const int64_t np = (int64_t)th.size();
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->mlp_theta.ptr, th.data(), sizeof(float) * np, hipMemcpyHostToDevice));
-    KSVD_HIP_CHECK(ctx, hipMemset(ctx->mlp_m.ptr, 0, sizeof(float) * np));
-    KSVD_HIP_CHECK(ctx, hipMemset(ctx->mlp_v.ptr, 0, sizeof(float) * np));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->mlp_theta.ptr, th.data(), sizeof(float) * np, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_set(ctx, ctx->mlp_m.ptr, 0, sizeof(float) * np));
+    KSVD_HIP_CHECK(ctx, ksvd_set(ctx, ctx->mlp_v.ptr, 0, sizeof(float) * np));
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream,
                        (const float *)ctx->mlp_theta.ptr, (__bf16 *)ctx->mlp_wb.ptr, np);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
@@ -722,7 +722,7 @@ int mlp_get_params(ksvd_ctx *ctx, float *theta)
 {
     if (!ctx->mlp_ready) return ksvd_fail(ctx, KSVD_ERR_STATE, "proposer not initialised");
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(theta, ctx->mlp_theta.ptr, sizeof(float) * mlp_nparams(ctx), hipMemcpyDeviceToHost));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, theta, ctx->mlp_theta.ptr, sizeof(float) * mlp_nparams(ctx), hipMemcpyDeviceToHost));
     return KSVD_OK;
 }
 
@@ -897,6 +897,8 @@ int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, do
     }
     DevBuf didx, dcoef, dcnt, dperm;
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);
+        std::lock_guard<std::mutex> lock(device_op_mutex());
         for (DevBuf *b : {&didx, &dcoef, &dcnt, &dperm})
             if (b->ptr) (void)hipFree(b->ptr);
     };
@@ -907,11 +909,11 @@ int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, do
         return rc;
     }
     hipError_t e = hipSuccess;
-    e = e ? e : hipMemcpy(ctx->mlp_xb.ptr, xb.data(), sizeof(__bf16) * xb.size(), hipMemcpyHostToDevice);
-    e = e ? e : hipMemcpy(didx.ptr, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
-    e = e ? e : hipMemcpy(dcoef.ptr, coef.data(), sizeof(float) * coef.size(), hipMemcpyHostToDevice);
-    e = e ? e : hipMemcpy(dcnt.ptr, cnt.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
-    e = e ? e : hipMemcpy(dperm.ptr, perm.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+    e = e ? e : ksvd_copy(ctx, ctx->mlp_xb.ptr, xb.data(), sizeof(__bf16) * xb.size(), hipMemcpyHostToDevice);
+    e = e ? e : ksvd_copy(ctx, didx.ptr, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+    e = e ? e : ksvd_copy(ctx, dcoef.ptr, coef.data(), sizeof(float) * coef.size(), hipMemcpyHostToDevice);
+    e = e ? e : ksvd_copy(ctx, dcnt.ptr, cnt.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+    e = e ? e : ksvd_copy(ctx, dperm.ptr, perm.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         cleanup();
         return ksvd_fail(ctx, KSVD_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
@@ -933,8 +935,8 @@ int mlp_loss_grad_host(ksvd_ctx *ctx, const float *X, const float *T, int nb, do
     std::vector<float> lp(nblk);
     const int64_t n1 = (int64_t)MLP_H * kin, n2 = (int64_t)MLP_H * MLP_HP, n3 = (int64_t)ko * MLP_HP;
     std::vector<float> gp((size_t)MLP_SPLIT * (n1 + n2 + n3));
-    e = e ? e : hipMemcpy(lp.data(), ctx->mlp_lpart.ptr, sizeof(float) * nblk, hipMemcpyDeviceToHost);
-    e = e ? e : hipMemcpy(gp.data(), ctx->mlp_grad.ptr, sizeof(float) * gp.size(), hipMemcpyDeviceToHost);
+    e = e ? e : ksvd_copy(ctx, lp.data(), ctx->mlp_lpart.ptr, sizeof(float) * nblk, hipMemcpyDeviceToHost);
+    e = e ? e : ksvd_copy(ctx, gp.data(), ctx->mlp_grad.ptr, sizeof(float) * gp.size(), hipMemcpyDeviceToHost);
     cleanup();
     if (e != hipSuccess) return ksvd_fail(ctx, KSVD_ERR_HIP, std::string("gradients: ") + hipGetErrorString(e));
     double l = 0.0;
@@ -1004,7 +1006,7 @@ static int mlp_permutation(ksvd_ctx *ctx, uint64_t seed)
     int rc = ksvd_reserve(ctx, ctx->mlp_perm, sizeof(int) * (size_t)P);
     if (rc) return rc;
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(ctx->mlp_perm.ptr, perm.data(), sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, ctx->mlp_perm.ptr, perm.data(), sizeof(int) * (size_t)P, hipMemcpyHostToDevice));
     ctx->mlp_perm_n = P;
     ctx->mlp_perm_seed = seed;
     return KSVD_OK;
@@ -1048,7 +1050,7 @@ int mlp_train_fallback(ksvd_ctx *ctx, int steps, float lr)
 {
     unsigned n = 0;
     KSVD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    KSVD_HIP_CHECK(ctx, hipMemcpy(&n, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost));
+    KSVD_HIP_CHECK(ctx, ksvd_copy(ctx, &n, ctx->mlp_cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost));
     const int usable = (int)std::min<int64_t>(steps, (int64_t)n / MLP_BATCH);
     if (usable <= 0) return KSVD_OK;
     return mlp_train_list(ctx, (const int *)ctx->mlp_fb.ptr, n, usable, lr);

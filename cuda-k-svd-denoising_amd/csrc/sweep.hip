@@ -1983,8 +1983,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
             const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;  // and each team's d_k pairs
             hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
                                ctx->stream, flags, wx, nw, (unsigned)(2 * PT_HT), spin);
-            // 8 XCDs take workgroups round robin: 2 PT_HT of the 16 PT_HT land on each XCD
-            hipLaunchKernelGGL(pt, dim3(8 * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
+            // the XCDs take workgroups round robin: 2 PT_HT of the n_xcd x 2 PT_HT land on each
+            hipLaunchKernelGGL(pt, dim3(ctx->n_xcd * 2 * PT_HT), dim3(PT_T), lds, ctx->stream, a, flags, wx);
             KSVD_HIP_CHECK(ctx, hipGetLastError());
             // the users beyond the cap get coefficient 0 in flush_cap_zero
             ctx->sweep_defers_zero = true;
@@ -2012,8 +2012,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
         float *Gred = a.Gpart + (size_t)std::max(B1, PS_HMAX) * ntiles * 256;
         hipLaunchKernelGGL(ps_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, flags, (unsigned long long *)nullptr, 0,
                            (unsigned)(H + 1), spin);
-        // 8 XCDs take workgroups round robin: H + 1 of the 8 (H + 1) land on each XCD
-        hipLaunchKernelGGL(ps, dim3(8 * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
+        // the XCDs take workgroups round robin: H + 1 of the n_xcd (H + 1) land on each
+        hipLaunchKernelGGL(ps, dim3(ctx->n_xcd * (H + 1)), dim3(PS_T), lds, ctx->stream, a, flags, H, Gred);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
     }

@@ -273,12 +273,35 @@ static bool graphs_off()
 // the record are ordered under one lock.  The lock also covers every graph
 // capture below: the last update's event may have been recorded on a stream
 // that another thread is capturing, and HIP refuses a wait on such an event
-// from a stream that is not capturing.
+// from a stream that is not capturing.  A context that goes away takes the
+// device's event with it if it recorded it last, so that no wait names an
+// event of a destroyed stream.
 namespace {
 constexpr int UPD_MAX_DEVICES = 64;
 std::mutex upd_mu;
 hipEvent_t upd_last[UPD_MAX_DEVICES];
+const ksvd_ctx *upd_owner[UPD_MAX_DEVICES];
 }  // namespace
+
+// HIP frees (and may allocate) device memory only after synchronising every
+// stream of the device, which breaks a capture in progress on another
+// thread's stream; so allocations, frees and captures take turns under the
+// same lock.
+std::mutex &device_op_mutex() { return upd_mu; }
+
+void update_release(ksvd_ctx *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= UPD_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(upd_mu);
+    const int d = ctx->device;
+    if (upd_owner[d] == ctx) {
+        // the context's stream is synchronised and about to go: nothing
+        // enqueued later waits on this event any more
+        (void)hipEventDestroy(upd_last[d]);
+        upd_last[d] = nullptr;
+        upd_owner[d] = nullptr;
+    }
+}
 
 // The update's launches depend only on the context's buffers and sizes, so
 // they are captured once per buffer layout into a HIP graph (KSVD_NO_GRAPH=1
@@ -330,6 +353,7 @@ int launch_update(ksvd_ctx *ctx)
         KSVD_HIP_CHECK(ctx, hipGraphLaunch((hipGraphExec_t)ctx->upd_exec, ctx->stream));
     }
     KSVD_HIP_CHECK(ctx, hipEventRecord(last, ctx->stream));
+    upd_owner[ctx->device] = ctx;
     ctx->cap_zero_pending = ctx->sweep_defers_zero;
     ctx->gram_valid = false;
     return KSVD_OK;

@@ -593,13 +593,19 @@ def test_sweep_same_bits_in_every_context(kl):
         np.testing.assert_array_equal(c, ref[1], err_msg=f"context {rep}")
 
 
-def test_two_threads_two_contexts(kl):
-    """Two contexts driven from two host threads at once on one device (their
-    persistent sweeps are ordered on the GPU) each reproduce their bits alone."""
+@pytest.mark.parametrize("nthreads", [2, 8])
+def test_threads_and_contexts(kl, nthreads):
+    """Contexts driven from several host threads at once on one device each
+    reproduce their bits alone (their persistent sweeps are ordered on the
+    GPU; allocations, frees and graph captures take turns), and no sweep gives
+    up (no fallback)."""
     import threading
-    jobs = [(speckle_frame(kl), 96, 80), (kl.image_to_frame(kl.synth_image(8, 128, 112, 20.0)[1]), 128, 112)]
+    jobs = [(speckle_frame(kl), 96, 80)]
+    for i in range(1, nthreads):
+        W, H = 96 + 16 * (i % 3), 80 + 16 * (i % 4)
+        jobs.append((kl.image_to_frame(kl.synth_image(7 + i, W, H, 20.0)[1]), W, H))
     alone = [sweep_run(kl, f, W, H, iters=3) for f, W, H in jobs]
-    res = [None, None]
+    res = [None] * nthreads
     errs = []
 
     def work(i):
@@ -609,16 +615,18 @@ def test_two_threads_two_contexts(kl):
                 res[i] = sweep_run(kl, f, W, H, iters=3)
                 np.testing.assert_array_equal(res[i][0], alone[i][0])
                 np.testing.assert_array_equal(res[i][1], alone[i][1])
+                assert res[i][2]["sweep_fallbacks"] == 0
         except Exception as e:  # reported by the main thread
             errs.append((i, e))
 
-    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    th = [threading.Thread(target=work, args=(i,)) for i in range(nthreads)]
     for t in th:
         t.start()
     for t in th:
         t.join(timeout=120)
     assert not any(t.is_alive() for t in th)
     assert not errs, errs
+    assert all(a[2]["sweep_fallbacks"] == 0 for a in alone)
 
 
 def test_sweep_that_gives_up_is_undone_and_run_again(kl, monkeypatch):
