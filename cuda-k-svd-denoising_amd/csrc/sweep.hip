@@ -597,7 +597,7 @@ __global__ __launch_bounds__(T2) void k2_kernel(SweepArgs a, int k)
 // leave; the host then reports the sweep as failed.
 constexpr int PS_T = T2;                // threads per workgroup (the power iteration's block size)
 constexpr int PS_W = PS_T / 64;         // wavefronts per workgroup
-constexpr int PS_HMAX = 31;             // helpers at most (one master + 31 helpers fill XCD 0's 32 CUs)
+constexpr int PS_HMAX = 31;             // helpers at most (one master + 31 helpers fill one XCD's 32 CUs)
 constexpr int PS_CWMAX = 64;            // columns per helper slice
 constexpr int PS_CPW = PS_CWMAX / PS_W; // columns per wavefront in the dot passes
 constexpr int PS_NOMAX = 7;             // other support entries per column (ompIter <= 8)
@@ -640,11 +640,14 @@ __device__ __forceinline__ unsigned ld_flag(const unsigned *p)
 __device__ __forceinline__ long spin_limit(const unsigned *flags) { return (long)ld_flag(flags + PSF_SPIN); }
 
 // Flag and exchange stores are workgroup-scope atomics: single stores that
-// reach the XCD's L2 (the L1 is write-through), where the persistent sweeps'
-// workgroups, all on XCD 0, read them with L1-bypassing loads.  Agent scope
-// would also write them through to memory for the other XCDs, which none of
-// the readers is on; that made each pair store's round trip twice as long
-// and the 2048 x 2048 update 10 % slower (2.90 -> 2.62 ms).
+// reach the XCD's L2 because gfx950's L1 is write-through, where the
+// persistent sweeps' workgroups, all on the sweep's XCD, read them with
+// L1-bypassing loads.  Readers in other workgroups see them through that
+// write-through L1, which the HIP memory model does not promise for
+// workgroup scope: this relies on the hardware.  Agent scope would also write
+// them through to memory for the other XCDs, which none of the readers is on;
+// that made each pair store's round trip twice as long and the 2048 x 2048
+// update 10 % slower (2.90 -> 2.62 ms).
 __device__ __forceinline__ void st_flag(unsigned *p, unsigned v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1156,7 +1159,7 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 constexpr int PT_T = 1024;               // threads per helper workgroup
 constexpr int PT_W = PT_T / 64;          // wavefronts per helper
 constexpr int PT_PW = 8;                 // wavefronts that run the power iteration's arithmetic
-constexpr int PT_HT = 16;                // helpers per team (the two teams fill XCD 0's 32 CUs)
+constexpr int PT_HT = 16;                // helpers per team (the two teams fill one XCD's 32 CUs)
 constexpr int PT_GRP = 4;                // groups reading the exchange, PT_HT / PT_GRP pairs each (4 groups: 3.0 ms per
                                          // update at 2048x2048; 7 groups: 3.3 ms, 1 group: 5.1 ms)
 constexpr int PT_XH = PT_HT / PT_GRP;    // exchange pairs read per thread
