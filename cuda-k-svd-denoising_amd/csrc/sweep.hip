@@ -1682,22 +1682,27 @@ __global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw,
 }
 
 // ===========================================================================
-// Large patches (padded dim above 192, up to 1024): four launches per atom,
+// Large patches (padded dim above 192, up to 1024): three launches per atom,
 // E_k and its Gram in global memory, for the patch sizes the LDS-resident
 // sweeps cannot hold (22 x 22 patches: dim 484).
-//   kl_build_kernel(k), many workgroups: E_k's capped columns, e_t = y_p -
-//     sum_{j != k} d_j x_{j,p} in the column table's entry order, every
-//     coefficient read from memory (atom k-1's were written, final, by
-//     kl_coef_kernel(k-1); users beyond an updated atom's cap were zeroed
-//     there too);
+//   kl_build_kernel(k), one pass of many workgroups: E_k's capped columns,
+//     e_t = y_p - sum_{j != k} d_j x_{j,p} in the column table's entry order,
+//     every coefficient read from memory (atom k-1's were written, final, by
+//     the previous atom's last launch; users beyond an updated atom's cap were
+//     zeroed there too);
 //   kl_gram_kernel(k): G = E_k E_k^T on MFMA, one workgroup per 64 x 64 tile
-//     of the upper triangle (mirrored), summed over the columns in order;
-//   kl_power_kernel(k), one workgroup: power iteration on G from d_k with the
-//     stop rule of the other sweeps; d_k = u;
-//   kl_coef_kernel(k), many workgroups: x_t = e_t . d_k for the first m
-//     users, 0 for the others.
+//     of the upper triangle (mirrored) and column range (KG_SPLIT partial
+//     Grams, each summed over its columns in order);
+//   kq_power_kernel(k), the workgroups of one XCD: the partial Grams added in
+//     order into each role's rows in LDS, the power iteration from d_k with
+//     the stop rule of the other sweeps by tagged-pair exchanges, d_k = u,
+//     then x_t = e_t . d_k for the first m users and 0 for the others.
+// The fallback after a persistent sweep gave up runs the power iteration in
+// one workgroup (kl_power_kernel, the single Gram) and the coefficients in
+// kl_coef_kernel, which wait on nothing.
 constexpr int KL_T = 1024;
 constexpr int KG_TB = 64, KG_KC = 64;
+constexpr int KG_SPLIT = 4;  // column ranges of the large-patch Gram when the persistent power iteration reads it
 
 __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, int k)
 {
@@ -1721,20 +1726,27 @@ __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, i
     }
 }
 
-// Tile (bi, bj), bi <= bj, of G = E E^T (dim x dim, pitch dim), four
+// Tile (bi, bj), bi <= bj, of G = E E^T (dim x dim, pitch gp), four
 // wavefronts of 32 x 32; the columns of E in chunks of KG_KC staged in LDS,
 // the next chunk's loads in flight (registers) while the current one is on
-// MFMA.
+// MFMA.  With gridDim.y = S > 1, workgroup (tile, s) sums the s-th of S
+// column ranges into partial Gram s (at G + s dim gp); the reader adds the
+// partials in order.
 constexpr int KG_LD = (KG_KC * KG_TB) / 256;  // values of each operand a thread stages per chunk
 
 __global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *__restrict__ E, float *__restrict__ G,
-                                                      int k)
+                                                      int gp, int k)
 {
     __shared__ float Ea[KG_KC][KG_TB + 4];
     __shared__ float Eb[KG_KC][KG_TB + 4];
     int c;
     const int m = atom_m(a.uoff, k, a.cap, &c);
     const int dim = a.dim;
+    // this workgroup's column range [tb, te)
+    const int S = (int)gridDim.y, sp = (int)blockIdx.y;
+    const int ck = ((m + S - 1) / S + KG_KC - 1) / KG_KC * KG_KC;
+    const int tb = min(m, sp * ck), te = min(m, tb + ck);
+    G += (size_t)sp * dim * gp;
     const int nb = (dim + KG_TB - 1) / KG_TB;
     // tile index -> (bi, bj) of the upper triangle, row by row
     int ti = blockIdx.x, bi = 0;
@@ -1755,12 +1767,12 @@ __global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *
 #pragma unroll
         for (int i = 0; i < KG_LD; i++) {
             const int q = tid + 256 * i, r = q / KG_TB, cc = q - r * KG_TB, t = t0 + r;
-            ra[i] = (t < m && e0 + cc < dim) ? E[(int64_t)t * dim + e0 + cc] : 0.f;
-            rb[i] = (t < m && f0 + cc < dim) ? E[(int64_t)t * dim + f0 + cc] : 0.f;
+            ra[i] = (t < te && e0 + cc < dim) ? E[(int64_t)t * dim + e0 + cc] : 0.f;
+            rb[i] = (t < te && f0 + cc < dim) ? E[(int64_t)t * dim + f0 + cc] : 0.f;
         }
     };
-    if (m > 0) fetch(0);
-    for (int t0 = 0; t0 < m; t0 += KG_KC) {
+    if (te > tb) fetch(tb);
+    for (int t0 = tb; t0 < te; t0 += KG_KC) {
 #pragma unroll
         for (int i = 0; i < KG_LD; i++) {
             const int q = tid + 256 * i, r = q / KG_TB, cc = q - r * KG_TB;
@@ -1768,7 +1780,7 @@ __global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *
             Eb[r][cc] = rb[i];
         }
         __syncthreads();
-        if (t0 + KG_KC < m) fetch(t0 + KG_KC);
+        if (t0 + KG_KC < te) fetch(t0 + KG_KC);
 #pragma unroll
         for (int kk = 0; kk < KG_KC; kk += 4) {
             float av[2], bv[2];
@@ -1793,28 +1805,30 @@ __global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *
             for (int i = 0; i < 4; i++) {
                 const int e = e0 + wr * 32 + x * 16 + 4 * g + i, f = f0 + wc * 32 + y * 16 + col;
                 if (e < dim && f < dim) {
-                    G[(int64_t)e * dim + f] = acc[x][y][i];
-                    G[(int64_t)f * dim + e] = acc[x][y][i];
+                    G[(int64_t)e * gp + f] = acc[x][y][i];
+                    G[(int64_t)f * gp + e] = acc[x][y][i];
                 }
             }
 }
 
-// Power iteration on G (global, symmetric): z = G u with thread (sl, e)
-// summing column e over rows f of slice sl (coalesced across e, loads
-// unrolled so that many are in flight), the slices added in order.
+// Power iteration on G (global, symmetric, row pitch gp = dim rounded up to
+// 4): z = G u with thread (sl, c) summing four columns 4c .. 4c+3 over the
+// rows f of slice sl by 16-byte loads (coalesced across c, KP_U rows in
+// flight), the slices' partials added in order.
 constexpr int KP_U = 8;
 
-__global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float *__restrict__ G, int k)
+__global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float *__restrict__ G, int gp, int k)
 {
     __shared__ float u[KSVD_MAX_DIM];
-    __shared__ float zp[KL_T];
+    __shared__ __attribute__((aligned(16))) float zp[4 * KL_T];  // slices x gp <= 4 KL_T
     __shared__ float red[KL_T / 64];
     int c;
     const int m = atom_m(a.uoff, k, a.cap, &c);
     if (m == 0) return;
     const int dim = a.dim, tid = threadIdx.x;
-    const int ns = KL_T / dim;  // slices (dim <= 1024)
-    const int sl = tid / dim, e = tid - sl * dim;
+    const int ncg = gp / 4;
+    const int ns = KL_T / ncg;  // slices (gp <= 1024)
+    const int sl = tid / ncg, cg = tid - sl * ncg;
     const int fl = (dim + ns - 1) / ns;
     const int fb = sl * fl, fe = min(dim, fb + fl);
     if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
@@ -1822,23 +1836,37 @@ __global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float
     int it = 0;
     float d2p = 0.f;
     for (;;) {
-        float s = 0.f;
         if (sl < ns) {
+            f4 s4 = f4{0.f, 0.f, 0.f, 0.f};
+            const float *gc = G + 4 * cg;
             int f = fb;
             for (; f + KP_U <= fe; f += KP_U) {
-                float gv[KP_U];
+                f4 gv[KP_U];
 #pragma unroll
-                for (int q = 0; q < KP_U; q++) gv[q] = G[(int64_t)(f + q) * dim + e];
+                for (int q = 0; q < KP_U; q++) gv[q] = *(const f4 *)(gc + (int64_t)(f + q) * gp);
 #pragma unroll
-                for (int q = 0; q < KP_U; q++) s = fmaf(gv[q], u[f + q], s);
+                for (int q = 0; q < KP_U; q++) {
+                    const float uf = u[f + q];
+                    s4.x = fmaf(gv[q].x, uf, s4.x);
+                    s4.y = fmaf(gv[q].y, uf, s4.y);
+                    s4.z = fmaf(gv[q].z, uf, s4.z);
+                    s4.w = fmaf(gv[q].w, uf, s4.w);
+                }
             }
-            for (; f < fe; f++) s = fmaf(G[(int64_t)f * dim + e], u[f], s);
+            for (; f < fe; f++) {
+                const f4 g4 = *(const f4 *)(gc + (int64_t)f * gp);
+                const float uf = u[f];
+                s4.x = fmaf(g4.x, uf, s4.x);
+                s4.y = fmaf(g4.y, uf, s4.y);
+                s4.z = fmaf(g4.z, uf, s4.z);
+                s4.w = fmaf(g4.w, uf, s4.w);
+            }
+            *(f4 *)(zp + sl * gp + 4 * cg) = s4;
         }
-        zp[tid] = s;
         __syncthreads();
         float zv = 0.f;
         if (tid < dim)
-            for (int q = 0; q < ns; q++) zv += zp[q * dim + tid];
+            for (int q = 0; q < ns; q++) zv += zp[q * gp + tid];
         const float n2 = block_sum_t2(zv * zv, red);
         if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
         const float un = zv / sqrtf(n2);
@@ -1854,12 +1882,163 @@ __global__ __launch_bounds__(KL_T) void kl_power_kernel(SweepArgs a, const float
     if (tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
 }
 
-// e_t . u by one wavefront (lane-strided over dim, fixed butterfly).
+// e_t . u by one wavefront (lane-strided over dim, fixed butterfly); every
+// load in flight before the first product (dim <= KSVD_MAX_DIM).
 __device__ __forceinline__ float kl_dot(const float *__restrict__ e, const float *__restrict__ u, int dim)
 {
+    constexpr int J = KSVD_MAX_DIM / 64;
+    const int lane = threadIdx.x & 63;
+    float ev[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) ev[j] = lane + 64 * j < dim ? e[lane + 64 * j] : 0.f;
     float s = 0.f;
-    for (int q = threadIdx.x & 63; q < dim; q += 64) s = fmaf(e[q], u[q], s);
+#pragma unroll
+    for (int j = 0; j < J; j++)
+        if (lane + 64 * j < dim) s = fmaf(ev[j], u[lane + 64 * j], s);
     return wave_sum_dpp(s);
+}
+
+// Power iteration on G by the KQ_R workgroups of one XCD (one launch per
+// atom): the first workgroup to arrive chooses its XCD for this atom (word
+// kw[k]), the others take roles only there (counter kw[K + k]) and the rest
+// leave.  Role r holds rows [r rpr, (r + 1) rpr) of G in LDS.  Each iteration
+// every role forms its rows of z = G u (a wavefront per row, lane-strided,
+// fixed butterfly) and stores them as (tag, value) pairs, tag = (k << 12) |
+// iteration, into one of two buffers by the iteration's parity; every role then
+// reads all dim pairs, rereading one until it carries the tag, and forms |z|,
+// u and the stop decision from the same values in the same order, so all roles
+// hold the same u bit for bit.  A role reads iteration i's pairs before it
+// stores those of i + 1, so nobody stores i + 2 while i may still be read.  The
+// pairs and the per-atom words are cleared before each sweep.  The waits give
+// up as the persistent sweeps' do (sticky abort; started and finished counts).
+constexpr int KQ_R = 32, KQ_T = 1024;
+
+// Sum over the workgroup (fixed order: DPP within each wavefront, then the
+// wavefronts in order), returned to every thread.
+__device__ __forceinline__ float kq_block_sum(float v, float *red)
+{
+    v = wave_sum_dpp(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < KQ_T / 64; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(KQ_T) void kq_power_kernel(SweepArgs a, const float *__restrict__ G, int gp, int nsplit,
+                                                        const float *__restrict__ E, int k, unsigned *flags,
+                                                        unsigned long long *wx, unsigned *kw)
+{
+    extern __shared__ __attribute__((aligned(16))) float gs[];  // rpr x gp
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float red[KQ_T / 64];
+    __shared__ int s_role, s_bail[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+        const unsigned old = atomicCAS(&kw[k], 0u, xcc + 1u);
+        int role = -1;
+        if ((old == 0u ? xcc + 1u : old) == xcc + 1u) {
+            const unsigned r = atomicAdd(&kw[a.K + k], 1u);
+            if (r < (unsigned)KQ_R) role = (int)r;
+        }
+        s_role = role;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    int c;
+    const int m = atom_m(a.uoff, k, a.cap, &c);
+    const int dim = a.dim;
+    const int rpr = (dim + KQ_R - 1) / KQ_R;
+    const int r0 = min(dim, role * rpr), r1 = min(dim, r0 + rpr), nr = r1 - r0;
+    int it = 0;
+    if (m > 0) {
+        // this role's rows, the partial Grams added in order
+        const int g4 = gp / 4;
+        const size_t pstride = (size_t)dim * gp;
+        for (int q = tid; q < nr * g4; q += KQ_T) {
+            const int r = q / g4, c4 = q - r * g4;
+            const float *src = G + (int64_t)(r0 + r) * gp + 4 * c4;
+            f4 v = *(const f4 *)src;
+            for (int sp = 1; sp < nsplit; sp++) {
+                const f4 w = *(const f4 *)(src + sp * pstride);
+                v.x += w.x;
+                v.y += w.y;
+                v.z += w.z;
+                v.w += w.w;
+            }
+            *(f4 *)(gs + r * gp + 4 * c4) = v;
+        }
+        if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
+        __syncthreads();
+        float d2p = 0.f;
+        for (;;) {
+            const unsigned tag = ((unsigned)k << 12) | (unsigned)(it + 1);
+            unsigned long long *W = wx + (size_t)(it & 1) * gp;
+            // this role's rows of z = G u
+            for (int r = wave; r < nr; r += KQ_T / 64) {
+                float z = 0.f;
+                for (int f = lane; f < dim; f += 64) z = fmaf(gs[r * gp + f], u[f], z);
+                z = wave_sum_dpp(z);
+                if (lane == 0) st64_c(W + r0 + r, ((unsigned long long)tag << 32) | __float_as_uint(z));
+            }
+            // every role's rows
+            float zv = 0.f;
+            if (tid < dim) {
+                unsigned long long pv = ld64_c(W + tid);
+                long spins = 0;
+                while ((unsigned)(pv >> 32) != tag) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 31) == 0) {
+                        if (ld_flag(flags + PSF_ABORT)) {
+                            s_bail[0] = 1;
+                            break;
+                        }
+                        if (spins > (long)s_bail[1]) {
+                            st_flag(flags + PSF_STICKY, 1u);
+                            st_flag(flags + PSF_ABORT, 1u);
+                            s_bail[0] = 1;
+                            break;
+                        }
+                    }
+                    pv = ld64_c(W + tid);
+                }
+                zv = __uint_as_float((unsigned)pv);
+            }
+            const float n2 = kq_block_sum(zv * zv, red);
+            if (s_bail[0]) return;
+            if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
+            const float un = zv / sqrtf(n2);
+            const float du = tid < dim ? un - u[tid] : 0.f;
+            const float d2 = kq_block_sum(du * du, red);
+            it++;
+            if (tid < dim) u[tid] = un;
+            __syncthreads();
+            if (power_stop(it, d2, d2p, a.tol2, a.est2, a.maxit)) break;
+            d2p = d2;
+        }
+        if (role == 0 && tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
+        // the new coefficients x_t = e_t . d_k of the first m users, a
+        // wavefront per column over all roles
+        const int64_t cb = a.moff[k];
+        for (int t = role * (KQ_T / 64) + wave; t < m; t += KQ_R * (KQ_T / 64)) {
+            const float x = kl_dot(E + (int64_t)t * dim, u, dim);
+            if (lane == 0) a.coef[a.tab_self[cb + t]] = x;
+        }
+    }
+    // users beyond the cap: coefficient 0
+    const int base = a.uoff[k];
+    for (int t = m + role * KQ_T + tid; t < c; t += KQ_R * KQ_T)
+        a.coef[(int64_t)a.users[base + t] * a.kmax + a.uslot[base + t]] = 0.f;
+    if (tid == 0) {
+        if (role == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
+        atomicAdd(&flags[PSF_FINISHED], 1u);
+    }
 }
 
 __global__ __launch_bounds__(256) void kl_coef_kernel(SweepArgs a, const float *__restrict__ E, int k)
@@ -1943,20 +2122,59 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     const int tpw = (ntiles + 15) / 16;
     const bool wide = kmax > 5;
 
-    // ---- large patches: E in global memory, two kernels per atom -------------
+    // KSVD_SWEEP_SPIN (tests): fewer polls before a persistent sweep's wait
+    // gives up, so that the abort and the two-kernel fallback can be exercised
+    unsigned spin = (unsigned)PS_SPIN_MAX;
+    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(0, atoi(sv));
+    const bool two_kernel = ctx->force_two_kernel || env_flag("KSVD_SWEEP_TWO_KERNEL");
+
+    // ---- large patches: E in global memory, four kernels per atom ------------
     auto launch_large = [&]() -> int {
         if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
         float *E = (float *)ctx->E.ptr;  // m x dim, m <= min(users, cap): the E buffer's columns
-        float *G = a.Gpart;              // dim x dim
-        if (sizeof(float) * (size_t)dim * dim > ctx->gpart.bytes)
+        float *G = a.Gpart;              // dim x gp
+        const int gp = (dim + 3) / 4 * 4;
+        if (sizeof(float) * (size_t)dim * gp > ctx->gpart.bytes)
             return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: Gram buffer too small");
         const int nb = (dim + KG_TB - 1) / KG_TB;
         const int ntg = nb * (nb + 1) / 2;
+        // enough workgroups for one pass over the most columns an atom can
+        // have (its capped users; every patch without a cap)
+        const int64_t ecols = a.cap > 0 ? a.cap : ctx->P;
+        const int cpt = KL_T / dim;
+        const unsigned gb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (ecols + cpt - 1) / cpt));
+        const unsigned gc = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (ecols + 3) / 4));
+        // the power iteration on one XCD's workgroups (their pairs and per-atom
+        // words after G), or, for the fallback, on one workgroup
+        const bool persistent = !two_kernel;
+        // the persistent power iteration reads the Gram once per atom, so the
+        // Gram is split over KG_SPLIT column ranges (more workgroups on MFMA)
+        const int nsplit = persistent ? KG_SPLIT : 1;
+        unsigned *flags = (unsigned *)ctx->ticket.ptr;
+        unsigned long long *wx = (unsigned long long *)(G + (size_t)nsplit * dim * gp + 4);
+        wx = (unsigned long long *)(((uintptr_t)wx + 15) & ~(uintptr_t)15);
+        unsigned *kw = (unsigned *)(wx + 2 * (size_t)gp);
+        const int nw = 2 * gp + K;  // 64-bit words cleared: the pairs, then K + K 32-bit per-atom words
+        const size_t lds_q = sizeof(float) * (size_t)((dim + KQ_R - 1) / KQ_R) * gp;
+        if (persistent) {
+            if ((char *)(wx + nw) > (char *)a.Gpart + ctx->gpart.bytes || lds_q > 150 * 1024)
+                return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: large-patch power iteration does not fit");
+            KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kq_power_kernel));
+            hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
+                               ctx->stream, flags, wx, nw, (unsigned)(K * KQ_R), spin);
+        }
         for (int k = 0; k < K; k++) {
-            hipLaunchKernelGGL(kl_build_kernel, dim3(B1 * 4), dim3(KL_T), 0, ctx->stream, a, E, k);
-            hipLaunchKernelGGL(kl_gram_kernel, dim3(ntg), dim3(256), 0, ctx->stream, a, (const float *)E, G, k);
-            hipLaunchKernelGGL(kl_power_kernel, dim3(1), dim3(KL_T), 0, ctx->stream, a, (const float *)G, k);
-            hipLaunchKernelGGL(kl_coef_kernel, dim3(B1 * 4), dim3(256), 0, ctx->stream, a, (const float *)E, k);
+            hipLaunchKernelGGL(kl_build_kernel, dim3(gb), dim3(KL_T), 0, ctx->stream, a, E, k);
+            hipLaunchKernelGGL(kl_gram_kernel, dim3(ntg, nsplit), dim3(256), 0, ctx->stream, a, (const float *)E, G, gp,
+                               k);
+            if (persistent) {
+                // power iteration and the new coefficients
+                hipLaunchKernelGGL(kq_power_kernel, dim3(ctx->n_xcd * KQ_R), dim3(KQ_T), lds_q, ctx->stream, a,
+                                   (const float *)G, gp, nsplit, (const float *)E, k, flags, wx, kw);
+            } else {
+                hipLaunchKernelGGL(kl_power_kernel, dim3(1), dim3(KL_T), 0, ctx->stream, a, (const float *)G, gp, k);
+                hipLaunchKernelGGL(kl_coef_kernel, dim3(gc), dim3(256), 0, ctx->stream, a, (const float *)E, k);
+            }
         }
         KSVD_HIP_CHECK(ctx, hipGetLastError());
         return KSVD_OK;
@@ -1964,11 +2182,6 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     if (large) return launch_large();
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
-    // KSVD_SWEEP_SPIN (tests): fewer polls before a persistent sweep's wait
-    // gives up, so that the abort and the two-kernel fallback can be exercised
-    unsigned spin = (unsigned)PS_SPIN_MAX;
-    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(0, atoi(sv));
-    const bool two_kernel = ctx->force_two_kernel || env_flag("KSVD_SWEEP_TWO_KERNEL");
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
         const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
                            sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 5 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
