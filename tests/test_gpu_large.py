@@ -97,3 +97,28 @@ def test_large_patch_end_to_end(kl, orc, cfg):
           f"{po['f64']:.4f} (noisy {kl.psnr(clean, noisy):.4f})")
     assert abs(pg - po["f32"]) <= 0.05
     assert pg > kl.psnr(clean, noisy) + 2
+
+
+def test_large_patch_sweep_that_gives_up_is_redone(kl, monkeypatch):
+    """22 x 22: a persistent power iteration that gives up (a spin limit of 0)
+    is undone and the iterations run again with the one-workgroup power
+    iteration: the result equals that path's, bit for bit, and the fallback is
+    counted."""
+    cfg = dict(CASES[1], W=128, H=96)
+
+    def run():
+        clean, noisy, ks = context(kl, cfg)
+        ks.iterate(2)
+        out = ks.dictionary(), ks.codes()[1], ks.timings()["sweep_fallbacks"]
+        ks.close()
+        return out
+
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1")
+    d2, c2, f2 = run()
+    monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "0")
+    monkeypatch.setenv("KSVD_SWEEP_SPIN", "0")
+    d, c, f = run()
+    print(f"\nlarge-patch sweep fallbacks: {f} (two-kernel run: {f2})")
+    assert f2 == 0 and f == 1
+    np.testing.assert_array_equal(d, d2)
+    np.testing.assert_array_equal(c, c2)
