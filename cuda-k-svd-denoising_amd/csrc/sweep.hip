@@ -1201,14 +1201,15 @@ __device__ __forceinline__ int pa_m(const int *suo, int k, int cap)
 // its coefficient word is cleared only after the sweep, so it is dropped here.
 // sx: 1 for a column whose patch is also among atom kb+1's first m_{kb+1}
 // users (the only columns of E_kb that atom kb+1's hand-over reads).
+// Threads t0 .. t0 + tn - 1 of the workgroup build it.
 template <int NO>
 __device__ void pa_table(const SweepArgs &a, const int *suo, const int *smo, int kb, int tb, int nc, int mx, int *sp,
-                         int *ss, int *sat, int *sci, int *hrank, int *hl, int *nhp, int *sx)
+                         int *ss, int *sat, int *sci, int *hrank, int *hl, int *nhp, int *sx, int t0, int tn)
 {
     const int no = a.kmax > 1 ? a.kmax - 1 : 1;
     const int mnext = kb + 1 < a.K ? pa_m(suo, kb + 1, a.cap) : 0;
     const int64_t cb = (int64_t)smo[kb] + tb;
-    for (int tl = threadIdx.x; tl < nc; tl += PT_T) {
+    for (int tl = (int)threadIdx.x - t0; tl >= 0 && tl < nc; tl += tn) {
         const int64_t col = cb + tl;
         sp[tl] = a.tab_p[col];
         ss[tl] = a.tab_self[col];
@@ -1293,6 +1294,17 @@ __global__ void pa_zero_kernel(SweepArgs a)
     }
 }
 
+// A team member's table of one atom's columns (pa_table) and the atom's
+// starting vector; two sets, so that atom k+2's is loaded while atom k runs.
+constexpr int PT_TABW = 2 * PS_CWMAX * PS_NOMAX + 5 * PS_CWMAX + 1;  // ints of one set
+constexpr int PT_EH0 = 192;  // threads [PT_EH0, PT_TB0): the hand-over columns at an atom's start
+constexpr int PT_TB0 = 512;  // threads [PT_TB0, PT_T): the next atom's table at an atom's start
+
+struct PtTab {
+    int *sat, *sci, *sp, *hrank, *hl, *ss, *sx, *nhp;
+    float *d0;
+};
+
 template <int NO, int BU>
 __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, unsigned long long *Wx, float *sm,
                           int *bail)
@@ -1304,19 +1316,25 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
     float *Es = sm;                           // PS_CWMAX x dpad: this helper's columns of the team's atom
     float *Eh = Es + PS_CWMAX * dpad;         // PS_CWMAX x dpad: columns of E_{k-1} for the hand-over (hl order)
     float *dprev = Eh + PS_CWMAX * dpad;      // dpad: d_{k-1}, then d_k for P4
-    float *d0 = dprev + dpad;                 // dpad: atom k as it stands (the starting vector)
-    float *wpw = d0 + dpad;                   // PT_W x dpad: per-wavefront partial sums of E v
+    float *wpw = dprev + dpad;                // PT_W x dpad: per-wavefront partial sums of E v
     float *wx = wpw + PT_W * dpad;            // PT_GRP x dpad: partial sums of the exchange
-    int *sat = (int *)(wx + PT_GRP * dpad);   // PS_CWMAX x PS_NOMAX atoms (-1: none)
-    int *sci = sat + PS_CWMAX * PS_NOMAX;     // PS_CWMAX x PS_NOMAX coefficient indices (-1: none)
-    int *sp = sci + PS_CWMAX * PS_NOMAX;      // PS_CWMAX patches
-    int *hrank = sp + PS_CWMAX;               // PS_CWMAX: rank among atom k-1's users, -1: none
-    int *hl = hrank + PS_CWMAX;               // PS_CWMAX: columns with a hand-over
-    int *nhp = hl + PS_CWMAX;                 // their number
-    int *ss = nhp + 1;                        // PS_CWMAX: own coefficient slots of the columns
-    int *sx = ss + PS_CWMAX;                  // PS_CWMAX: 1 for a column atom k+1's hand-over reads
-    int *suo = sx + PS_CWMAX;                 // K + 1: uoff
+    float *d0s = wx + PT_GRP * dpad;          // 2 x dpad: the starting vectors of the two table sets
+    int *tabs = (int *)(d0s + 2 * dpad);      // 2 x PT_TABW: the two table sets
+    int *suo = tabs + 2 * PT_TABW;            // K + 1: uoff
     int *smo = suo + K + 1;                   // K + 1: moff
+    auto tab = [&](int set) {
+        PtTab t;
+        t.sat = tabs + set * PT_TABW;         // PS_CWMAX x PS_NOMAX atoms (-1: none)
+        t.sci = t.sat + PS_CWMAX * PS_NOMAX;  // PS_CWMAX x PS_NOMAX coefficient indices (-1: none)
+        t.sp = t.sci + PS_CWMAX * PS_NOMAX;   // PS_CWMAX patches
+        t.hrank = t.sp + PS_CWMAX;            // PS_CWMAX: rank among atom k-1's users, -1: none
+        t.hl = t.hrank + PS_CWMAX;            // PS_CWMAX: columns with a hand-over
+        t.ss = t.hl + PS_CWMAX;               // PS_CWMAX: own coefficient slots of the columns
+        t.sx = t.ss + PS_CWMAX;               // PS_CWMAX: 1 for a column atom k+1's hand-over reads
+        t.nhp = t.sx + PS_CWMAX;              // the number of hand-over columns
+        t.d0 = d0s + set * dpad;              // dpad: the atom as it stands (the starting vector)
+        return t;
+    };
     unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
     const int me = team * PT_HT + h, other = (1 - team) * PT_HT;
     unsigned long long *Wt = Wx + (size_t)team * 2 * PT_HT * dpad;  // this team's two pair buffers
@@ -1329,69 +1347,110 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         suo[i] = a.uoff[i];
         smo[i] = a.moff[i];
     }
-    if (tid == 0) *nhp = 0;
+    if (tid == 0) {
+        *tab(0).nhp = 0;
+        *tab(1).nhp = 0;
+    }
     __syncthreads();
     // the team's first atom k = team: its columns without atom k-1 (nothing
     // is final yet) and atom k's vector
     int k = team;
     if (k < K) {
+        const PtTab T0 = tab(0);
         int tb0;
         const int nc0 = cols_of(pa_m(suo, k, cap), H, h, &tb0);
-        pa_table<NO>(a, suo, smo, k, tb0, nc0, k >= 1 ? pa_m(suo, k - 1, cap) : 0, sp, ss, sat, sci, hrank, hl, nhp, sx);
-        if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)k * dim + tid);
+        pa_table<NO>(a, suo, smo, k, tb0, nc0, k >= 1 ? pa_m(suo, k - 1, cap) : 0, T0.sp, T0.ss, T0.sat, T0.sci,
+                     T0.hrank, T0.hl, T0.nhp, T0.sx, 0, PT_T);
+        if (tid < dim) T0.d0[tid] = ld_c(a.D + (int64_t)k * dim + tid);
         __syncthreads();
-        pa_build<NO, BU>(a, nc0, Es, sp, sat, sci);
+        pa_build<NO, BU>(a, nc0, Es, T0.sp, T0.sat, T0.sci);
         __syncthreads();
     }
-    bool eh_ready = false;  // Eh holds the hand-over columns for atom k
     for (; k < K; k += 2) {
         // dim and dpad pass through an opaque copy each atom, so that offsets
         // derived from them are recomputed where they are used instead of
         // being held in registers across the loop
         asm volatile("" : "+s"(dim), "+s"(dpad));
+        const int cs = (k >> 1) & 1;  // this atom's table set; atom k+2's is the other
+        const PtTab T = tab(cs), Tn = tab(cs ^ 1);
         int tb;
         const int m = pa_m(suo, k, cap);
         const int nc = cols_of(m, H, h, &tb);
         float *Ecur = (k & 1) ? a.E1 : a.E0;
         const float *Eprev = (k & 1) ? a.E0 : a.E1;
-        // ---- P2: atom k-1's share --------------------------------------------
-        if (k >= 1) {
-            const int nh = *nhp;
-            if (!eh_ready) {
-                // (the first atom of team 1) the columns of E_{k-1}
-                if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)k, bail)) return;
-                l1_drop();
-                const int d4 = dpad / 4;
-                for (int q = tid; q < nh * d4; q += PT_T) {
-                    const int i = q / d4, c4 = q - i * d4;
-                    *(f4 *)(Eh + i * dpad + 4 * c4) = *(const f4 *)(Eprev + (size_t)hrank[hl[i]] * dpad + 4 * c4);
+        const int kn = k + 2;
+        // ---- atom start: three thread groups at once --------------------------
+        //   threads < dim: team 1-t's d_{k-1} pairs (tagged k);
+        //   [PT_EH0, PT_TB0): the hand-over columns of E_{k-1}, each once its
+        //     owner (the member of team 1-t that holds that rank) has certified
+        //     its columns (ecert >= k), read with L1-bypassing loads;
+        //   [PT_TB0, PT_T): atom k+2's table and starting vector (static until
+        //     that atom's own update).
+        if (k >= 1 && tid < dim) {
+            const unsigned long long *pp = Dp + (size_t)(1 - team) * dpad + tid;
+            unsigned long long pv = ld64_c(pp);
+            long spins = 0;
+            while ((unsigned)(pv >> 32) != (unsigned)k) {
+                __builtin_amdgcn_s_sleep(2);
+                if ((++spins & 15) == 0) {
+                    if (ld_flag(flags + PSF_ABORT)) {
+                        *bail = 1;
+                        break;
+                    }
+                    if (spins > spin_limit(flags) / 2) {
+                        st_flag(flags + PSF_STICKY, 1u);
+                        st_flag(flags + PSF_ABORT, 1u);
+                        *bail = 1;
+                        break;
+                    }
                 }
+                pv = ld64_c(pp);
             }
-            // d_{k-1}: team 1-t's pairs tagged k
-            if (tid < dim) {
-                const unsigned long long *pp = Dp + (size_t)(1 - team) * dpad + tid;
-                unsigned long long pv = ld64_c(pp);
-                long spins = 0;
-                while ((unsigned)(pv >> 32) != (unsigned)k) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if ((++spins & 15) == 0) {
-                        if (ld_flag(flags + PSF_ABORT)) {
-                            *bail = 1;
-                            break;
-                        }
-                        if (spins > spin_limit(flags) / 2) {
-                            st_flag(flags + PSF_STICKY, 1u);
-                            st_flag(flags + PSF_ABORT, 1u);
-                            *bail = 1;
-                            break;
+            dprev[tid] = __uint_as_float((unsigned)pv);
+        } else if (k >= 1 && tid >= PT_EH0 && tid < PT_TB0) {
+            const int nh = *T.nhp;
+            const int cwp = (pa_m(suo, k - 1, cap) + H - 1) / H;  // team 1-t's columns per member for atom k-1
+            int polled = -1;
+            bool gaveup = false;
+            for (int q = tid - PT_EH0; q < nh * dpad && !gaveup; q += PT_TB0 - PT_EH0) {
+                const int i = q / dpad, c = q - i * dpad;
+                const int r = T.hrank[T.hl[i]];
+                const int owner = r / cwp;
+                if (owner != polled) {
+                    long spins = 0;
+                    while (ld_flag(ecert + other + owner) < (unsigned)k) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if ((++spins & 15) == 0) {
+                            if (ld_flag(flags + PSF_ABORT)) {
+                                *bail = 1;
+                                gaveup = true;
+                                break;
+                            }
+                            if (spins > spin_limit(flags)) {
+                                st_flag(flags + PSF_STICKY, 1u);
+                                st_flag(flags + PSF_ABORT, 1u);
+                                *bail = 1;
+                                gaveup = true;
+                                break;
+                            }
                         }
                     }
-                    pv = ld64_c(pp);
+                    polled = owner;
                 }
-                dprev[tid] = __uint_as_float((unsigned)pv);
+                if (!gaveup) Eh[i * dpad + c] = ld_c(Eprev + (size_t)r * dpad + c);
             }
-            __syncthreads();
-            if (*bail) return;
+        } else if (kn < K && tid >= PT_TB0) {
+            int tbn;
+            const int ncn = cols_of(pa_m(suo, kn, cap), H, h, &tbn);
+            pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), Tn.sp, Tn.ss, Tn.sat, Tn.sci, Tn.hrank,
+                         Tn.hl, Tn.nhp, Tn.sx, PT_TB0, PT_T - PT_TB0);
+            if (tid - PT_TB0 < dim) Tn.d0[tid - PT_TB0] = ld_c(a.D + (int64_t)kn * dim + (tid - PT_TB0));
+        }
+        __syncthreads();
+        if (*bail) return;
+        // ---- P2: atom k-1's share --------------------------------------------
+        if (k >= 1) {
+            const int nh = *T.nhp;
             float dv[3];
 #pragma unroll
             for (int j = 0; j < 3; j++) {
@@ -1399,7 +1458,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                 dv[j] = e < dim ? dprev[e] : 0.f;
             }
             for (int i = wave; i < nh; i += PT_W) {
-                const int tl = hl[i];
+                const int tl = T.hl[i];
                 float ev[3];
 #pragma unroll
                 for (int j = 0; j < 3; j++) ev[j] = Eh[i * dpad + min(lane + 64 * j, dim - 1)];
@@ -1417,14 +1476,14 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int e = lane + 64 * j;
-            uj[j] = e < dim ? d0[e] : 0.f;
+            uj[j] = e < dim ? T.d0[e] : 0.f;
         }
         // E_k's columns that team 1-t's hand-over of atom k+1 reads out, and ecert
         auto store_e = [&](bool publish) {
             const int d4 = dpad / 4;
             for (int q = tid; q < nc * d4; q += PT_T) {
                 const int tl = q / d4, c4 = q - tl * d4;
-                if (sx[tl])
+                if (T.sx[tl])
                     *(f4 *)(Ecur + (int64_t)(tb + tl) * dpad + 4 * c4) = *(const f4 *)(Es + tl * dpad + 4 * c4);
             }
             if (publish) ps_publish(ecert + me, (unsigned)(k + 1));
@@ -1613,41 +1672,25 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
 #pragma unroll
                 for (int j = 0; j < 3; j++) ev[j] = Es[tl * dpad + min(lane + 64 * j, dim - 1)];
                 const float x = ps_dot(ev, dv, dim);
-                if (lane == 0) a.coef[ss[tl]] = x;
+                if (lane == 0) a.coef[T.ss[tl]] = x;
             }
         }
         ps_publish(ccert + me, (unsigned)(k + 1));
         // ---- P1: atom k+2's columns without atom k+1 ------------------------------
-        // (the table and the atom's current vector first, before the waits:
-        // neither changes during the sweep before atom k+2's own update; then
-        // the patches and the other atoms' shares in one pass)
-        const int kn = k + 2;
+        // (its table and vector were loaded at this atom's start; once the
+        // other atoms' coefficients are certified, the patches and the other
+        // atoms' shares in one pass)
         if (kn >= K) break;
-        if (tid == 0) *nhp = 0;
-        __syncthreads();
+        // this atom's set is refilled at atom k+2's start, for atom k+4 (the
+        // barriers of the wait below order this before those threads' use)
+        if (tid == 0) *T.nhp = 0;
         int tbn;
         const int ncn = cols_of(pa_m(suo, kn, cap), H, h, &tbn);
-        pa_table<NO>(a, suo, smo, kn, tbn, ncn, pa_m(suo, k + 1, cap), sp, ss, sat, sci, hrank, hl, nhp, sx);
-        if (tid < dim) d0[tid] = ld_c(a.D + (int64_t)kn * dim + tid);
-        __syncthreads();
         if (!ps_wait3(flags, PTF_CCERT + team * PT_HT, H, (unsigned)(k + 1), PTF_CCERT + other, k >= 1 ? H : 0,
                       (unsigned)k, PTF_DONE + team, 1, (unsigned)(k + 1), bail))
             return;
         l1_drop();
-        pa_build<NO, BU, false>(a, ncn, Es, sp, sat, sci);
-        // the columns of E_{k+1} that atom k+2's hand-over needs
-        // (L1-bypassing loads: a second invalidation here cost 0.12 ms per
-        // update at 2048 x 2048; the columns are few)
-        if (!ps_wait(flags, PTF_ECERT + other, H, (unsigned)(k + 2), bail)) return;
-        {
-            const int nh = *nhp;
-            const float *Enx = (kn & 1) ? a.E0 : a.E1;  // E_{k+1}
-            for (int q = tid; q < nh * dpad; q += PT_T) {
-                const int i = q / dpad, c = q - i * dpad;
-                Eh[i * dpad + c] = ld_c(Enx + (size_t)hrank[hl[i]] * dpad + c);
-            }
-        }
-        eh_ready = true;
+        pa_build<NO, BU, false>(a, ncn, Es, Tn.sp, Tn.sat, Tn.sci);
         __syncthreads();
     }
     if (h == 0 && tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], iters);
@@ -2188,8 +2231,8 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
     if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
-        const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (2 + PT_W + PT_GRP) * (size_t)dpad) +
-                           sizeof(int) * (2 * (size_t)PS_CWMAX * PS_NOMAX + 5 * PS_CWMAX + 1 + 2 * ((size_t)K + 1));
+        const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (3 + PT_W + PT_GRP) * (size_t)dpad) +
+                           sizeof(int) * (2 * (size_t)PT_TABW + 2 * ((size_t)K + 1));
         if (lds <= 160 * 1024) {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
             // BU = 5: a member's columns (at most 64) in two passes of 35 with every
