@@ -1132,17 +1132,22 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 //
 // While one team iterates on atom k, the other prepares its next atom k+1:
 // the final coefficients of its atom k-1, then atom k+1's columns without
-// atom k (atom k's share is handed over when d_k is out), and the columns of
-// E_k that this hand-over needs.  Atom k, team t, helper h:
-//   P2  once team 1-t's d_{k-1} pairs (tagged k) are in: for each column
-//       whose patch is among atom k-1's first m_{k-1} users (rank r):
-//       x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x (column r prefetched);
+// atom k (atom k's share is handed over when d_k is out).  Atom k, team t,
+// helper h:
+//   start  three thread groups at once: team 1-t's d_{k-1} pairs (tagged
+//       k); the columns of E_{k-1} that the hand-over needs, each read once
+//       the team 1-t member that holds it has certified it (ecert >= k);
+//       atom k+2's column table and starting vector (static until that
+//       atom's own update) into the other of two table sets;
+//   P2  for each column whose patch is among atom k-1's first m_{k-1} users
+//       (rank r): x = e_{k-1,r} . d_{k-1} and e -= d_{k-1} x;
 //   PI  exchanges i = 1, 2, ... until power_stop accepts u_i, or z_i = 0
-//       (then u_{i-1} stays).  After exchange 1's
-//       store: its E_k columns to global memory; ecert[h] = k + 1 once
-//       exchange 1's vectors are in (after a store wait, which by then finds
-//       the columns out).  PT_PW wavefronts form w_h; every wavefront takes
-//       part in the exchange's reads and forms u_i and the stop decision;
+//       (then u_{i-1} stays).  After exchange 1's store: the columns of E_k
+//       that atom k+1's hand-over reads, to global memory; ecert[h] = k + 1 at
+//       exchange 2's barrier (after a store wait, which by then finds the
+//       columns out), or after the iteration when it took one exchange.
+//       PT_PW wavefronts form w_h; every wavefront takes part in the
+//       exchange's reads and forms u_i and the stop decision;
 //   out helper 0 stores d_k as pairs tagged k + 1 for team 1-t (no store
 //       wait on this path), then into the dictionary, and sets done[t] =
 //       k + 1 (after a store wait);
@@ -1150,10 +1155,8 @@ __global__ __launch_bounds__(PS_T) void ps_sweep_kernel(SweepArgs a, unsigned *f
 //       arithmetic on the same values as team 1-t's P2, so both agree bit for
 //       bit); ccert[h] = k + 1 (after a store wait);
 //   P1  once every ccert of team t is k + 1, every ccert of team 1-t is k and
-//       done[t] = k + 1: atom k+2's column table without atom k+1, its
-//       patches, and every other atom's share subtracted (atoms <= k final,
-//       the others as they were); then, once every ecert of team 1-t is
-//       k + 2, the columns of E_{k+1} that atom k+2's hand-over needs.
+//       done[t] = k + 1: atom k+2's patches, and every other atom's share
+//       subtracted (atoms <= k final, the others as they were).
 // Users beyond the cap get coefficient 0 after the sweep (pa_zero_kernel);
 // until then the column tables drop their entries.
 constexpr int PT_T = 1024;               // threads per helper workgroup
@@ -1492,7 +1495,10 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         // e_t . u_{i-1} and its partial sum of e_t v_t; w_h = the PT_PW
         // partials in wavefront order, stored as pairs
         const bool piw = wave < PT_PW;
-        auto issue = [&](unsigned s) {
+        // cert: this is exchange 2, and the E_k columns stored after exchange
+        // 1's vectors are certified at its barrier (ecert), long before team
+        // 1-t reads them at atom k+1's start
+        auto issue = [&](unsigned s, bool cert) {
             unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
             if (piw) {
                 float pw[3] = {0.f, 0.f, 0.f};
@@ -1526,7 +1532,13 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     if (e < dim) wpw[wave * dpad + e] = pw[j];
                 }
             }
+            if (cert) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the E_k columns are out
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
             __syncthreads();
+            if (cert && tid == 0) st_flag(ecert + me, (unsigned)(k + 1));
             if (tid < dim) {
                 float w = 0.f;
 #pragma unroll
@@ -1539,7 +1551,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
         // returns -1 to leave (abort), 1 when the iteration stops, else 0
         int it = 0;
         float d2prev = 0.f;
-        auto collect = [&](unsigned s, bool cert) -> int {
+        auto collect = [&](unsigned s) -> int {
             const unsigned long long *Wb = Wt + (size_t)(s & 1) * PT_HT * dpad;
             const int ngrp = min(PT_T / dim, PT_GRP);
             const int gq = tid / dim, ge = tid - gq * dim;
@@ -1583,14 +1595,7 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
                     if (gq + ngrp * j < H) z += __uint_as_float((unsigned)pv[j]);
                 wx[gq * dpad + ge] = z;
             }
-            if (cert) {
-                // exchange 1: the E_k columns stored before it are out by now
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            }
             __syncthreads();
-            if (cert && tid == 0) st_flag(ecert + me, (unsigned)(k + 1));
             if (*bail) return -1;
             // every wavefront forms z_i, |z_i|^2, u_i and |u_i - u_{i-1}|^2
             // (each the same, bit for bit) and takes the decision itself, so
@@ -1629,14 +1634,18 @@ __device__ void pt_helper(const SweepArgs &a, unsigned *flags, int team, int h, 
             return dec != 0 ? 1 : 0;
         };
         if (m > 0) {
-            issue(++seq);
+            issue(++seq, false);
             store_e(false);
-            int r = collect(seq, true);
+            int r = collect(seq);
+            bool certified = false;
             while (r == 0) {
-                issue(++seq);
-                r = collect(seq, false);
+                issue(++seq, !certified);
+                certified = true;
+                r = collect(seq);
             }
             if (r < 0) return;
+            // one exchange only: certify the columns now
+            if (!certified) ps_publish(ecert + me, (unsigned)(k + 1));
             iters += (unsigned long long)it;
         } else {
             store_e(true);
