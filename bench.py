@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (92743.2 * 2 + 13394.1) * 1024, "omp": (131707.4 * 2 + 19818.1) * 1024},
+    "2048": {"sweep": (91771.2 * 2 + 11629.7) * 1024, "omp": (131707.4 * 2 + 19818.2) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
