@@ -1,7 +1,9 @@
 /*
  * ksvd_hip.h -- C ABI of the MI355X (gfx950) K-SVD denoising hot path.
  *
- * One opaque context per host thread; every call returns an int status
+ * One opaque context per host thread (contexts on several threads may share a
+ * device: their dictionary updates are ordered on the GPU, and allocations,
+ * frees and graph captures take turns); every call returns an int status
  * (KSVD_OK = 0, negative on error) and keeps a per-context error string.
  * No exceptions cross this boundary; no torch or HIP types appear in it.
  * Host buffers passed in are caller-owned and copied; device memory is owned
