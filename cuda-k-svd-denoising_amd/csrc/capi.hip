@@ -272,6 +272,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
     update_release(ctx);
     std::lock_guard<std::mutex> lock(device_op_mutex());
     if (ctx->gate_ev) (void)hipEventDestroy((hipEvent_t)ctx->gate_ev);
+    if (ctx->batch_ev) (void)hipEventDestroy((hipEvent_t)ctx->batch_ev);
     if (ctx->gate_host) (void)hipHostFree(ctx->gate_host);
     DevBuf *bufs[] = {&ctx->img,  &ctx->Y,      &ctx->D,     &ctx->G,       &ctx->idx,
                       &ctx->coef, &ctx->cnt,    &ctx->counters, &ctx->blockcnt, &ctx->ucount, &ctx->uoff,
@@ -280,7 +281,7 @@ int ksvd_destroy(ksvd_ctx *ctx)
                       &ctx->mlp_theta, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_wb, &ctx->mlp_grad, &ctx->mlp_xb,
                       &ctx->mlp_h1b, &ctx->mlp_h2b, &ctx->mlp_zb, &ctx->mlp_dz, &ctx->mlp_dh2, &ctx->mlp_dh1,
                       &ctx->mlp_lpart, &ctx->mlp_tr, &ctx->mlp_x, &ctx->mlp_perm, &ctx->mlp_h1c, &ctx->mlp_h2c, &ctx->mlp_zc,
-                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->mlp_rej, &ctx->mlp_bcnt, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt, &ctx->c0};
+                      &ctx->mlp_fb, &ctx->mlp_cnt, &ctx->mlp_rej, &ctx->mlp_bcnt, &ctx->multi, &ctx->snap_D, &ctx->snap_idx, &ctx->snap_coef, &ctx->snap_cnt, &ctx->c0};
     for (DevBuf *b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -672,6 +673,97 @@ int ksvd_iterate(ksvd_ctx *ctx, int iters)
         return check_sweep(ctx);
     }
     return rc;
+}
+
+// Several pictures' iterations at once (ksvd_iterate_batch): each picture's
+// pursuit on its own stream, then one multi-picture sweep with picture i on
+// XCD i.  Contexts that cannot share a sweep (other sizes or forms, more
+// pictures than XCDs) iterate one after another instead.
+int ksvd_iterate_batch(ksvd_ctx *const *ctxs, int n, int iters)
+{
+    if (!ctxs || n <= 0) return KSVD_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return KSVD_ERR_ARG;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return ksvd_fail(ctxs[i], KSVD_ERR_ARG, "a context appears twice in the batch");
+    }
+    if (iters < 0) return ksvd_fail(ctxs[0], KSVD_ERR_ARG, "iters must be >= 0");
+    if (iters == 0) return KSVD_OK;
+    bool shared = n > 1;
+    for (int i = 0; i < n; i++) {
+        ksvd_ctx *c = ctxs[i];
+        if (!c->has_dict) return ksvd_fail(c, KSVD_ERR_STATE, "dictionary required");
+        if (c->prm.precision != KSVD_PREC_FP32 || c->prm.proposer || c->device != ctxs[0]->device) shared = false;
+    }
+    int rc;
+    if (shared) {
+        if ((rc = set_device(ctxs[0]))) return rc;
+        rc = sweep_multi_tol_setup(ctxs, n);
+        if (rc < 0) return rc;
+        shared = rc == 1;
+    }
+    if (!shared) {
+        for (int i = 0; i < n; i++)
+            if ((rc = ksvd_iterate(ctxs[i], iters))) return rc;
+        return KSVD_OK;
+    }
+    std::vector<int64_t> done0(n);
+    for (int i = 0; i < n; i++) {
+        if ((rc = take_snapshot(ctxs[i], false))) return rc;
+        done0[i] = ctxs[i]->iters_done;
+    }
+    std::vector<Span> sp(n);
+    for (int it = 0; it < iters; it++) {
+        for (int i = 0; i < n; i++)
+            if ((rc = ksvd_omp(ctxs[i]))) return rc;
+        for (int i = 0; i < n; i++) {
+            if (ctxs[i]->kmax > 8)
+                return ksvd_fail(ctxs[i], KSVD_ERR_UNSUPPORTED, "the dictionary sweep handles ompIter <= 8");
+            if ((rc = span_begin(ctxs[i], &sp[i], SPAN_UPDATE))) return rc;
+        }
+        if ((rc = launch_update_batch(ctxs, n))) return rc;
+        for (int i = 0; i < n; i++) {
+            if ((rc = span_end(ctxs[i], &sp[i]))) return rc;
+            ctxs[i]->has_rebuild = false;
+            ctxs[i]->iters_done++;
+        }
+    }
+    // a picture whose sweep gave up is undone and iterated again alone with
+    // the two-kernel sweep
+    for (int i = 0; i < n; i++) {
+        ksvd_ctx *c = ctxs[i];
+        rc = check_sweep(c);
+        if (rc && c->sweep_failed) {
+            c->sweep_failed = false;
+            c->err.clear();
+            if ((rc = restore_snapshot(c, false))) return rc;
+            c->iters_done = done0[i];
+            c->force_two_kernel = true;
+            rc = iterate_async(c, iters);
+            c->force_two_kernel = false;
+            if (rc) return rc;
+            c->tm.sweep_fallbacks++;
+            rc = check_sweep(c);
+        }
+        if (rc) return rc;
+    }
+    return KSVD_OK;
+}
+
+int ksvd_run_batch(ksvd_ctx *const *ctxs, int n)
+{
+    if (!ctxs || n <= 0) return KSVD_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (!ctxs[i]) return KSVD_ERR_ARG;
+    // the pictures share the batch's iteration count (ctxs[0]'s ksvd_iter)
+    for (int i = 1; i < n; i++)
+        if (ctxs[i]->prm.ksvd_iter != ctxs[0]->prm.ksvd_iter)
+            return ksvd_fail(ctxs[i], KSVD_ERR_ARG, "the batch's contexts differ in ksvd_iter");
+    int rc = ksvd_iterate_batch(ctxs, n, ctxs[0]->prm.ksvd_iter);
+    if (rc) return rc;
+    for (int i = 0; i < n; i++)
+        if ((rc = ksvd_run_final_code(ctxs[i]))) return rc;
+    return KSVD_OK;
 }
 
 int ksvd_run_final_code(ksvd_ctx *ctx)

@@ -77,6 +77,7 @@ struct ksvd_ctx {
     int64_t mlp_perm_n = -1;
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     DevBuf mlp_rej, mlp_bcnt;                                // per-patch fallback flags, per-block counts
+    DevBuf multi;                                            // a batch's pictures for the multi-picture sweep
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
     // The two-team sweep leaves the coefficients of the users beyond the cap
@@ -95,6 +96,7 @@ struct ksvd_ctx {
     // behind gate_ev and read at the next learned-form iteration
     unsigned *gate_host = nullptr;
     void *gate_ev = nullptr;  // hipEvent_t
+    void *batch_ev = nullptr;  // hipEvent_t: this context's work before a multi-picture sweep
     bool gate_pending = false, gate_probe = false;
     int64_t gate_n = 0;
     int64_t mlp_chunk_rows = 0;
@@ -179,6 +181,11 @@ int mlp_propose(ksvd_ctx *ctx, float accept, int64_t n, bool write);
 size_t bf16_dict_bytes(int dim);
 size_t bf16_dictq_bytes(int dim);
 int launch_update(ksvd_ctx *ctx);
+int launch_update_batch(ksvd_ctx *const *ctxs, int n);  // one update of several pictures (multi-picture sweep)
+int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol);
+int sweep_multi_tol_setup(ksvd_ctx *const *ctxs, int n);  // sweep_multi_setup at the update's tolerance
+int sweep_multi_reset(ksvd_ctx *const *ctxs, int n);
+int launch_sweep_multi(ksvd_ctx *const *ctxs, int n);
 void update_release(ksvd_ctx *ctx);  // at destroy: the device's last-update event, if this context's
 std::mutex &device_op_mutex();       // held over allocations, frees and graph captures
 int prepare_update_graph(ksvd_ctx *ctx);  // capture the update's launches for the current buffers

@@ -1300,6 +1300,7 @@ __global__ void pa_zero_kernel(SweepArgs a)
 // A team member's table of one atom's columns (pa_table) and the atom's
 // starting vector; two sets, so that atom k+2's is loaded while atom k runs.
 constexpr int PT_TABW = 2 * PS_CWMAX * PS_NOMAX + 5 * PS_CWMAX + 1;  // ints of one set
+constexpr int PT_MULTI_MAX = 16;  // pictures of one multi-picture sweep at most
 constexpr int PT_EH0 = 192;  // threads [PT_EH0, PT_TB0): the hand-over columns at an atom's start
 constexpr int PT_TB0 = 512;  // threads [PT_TB0, PT_T): the next atom's table at an atom's start
 
@@ -1722,6 +1723,40 @@ __global__ __launch_bounds__(PT_T) void pt_sweep_kernel(SweepArgs a, unsigned *f
     if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
+// Several pictures' two-team sweeps in one launch, picture i on XCD i: the
+// launch has n_xcd x 2 PT_HT workgroups, the XCDs take them round robin, and
+// the 2 PT_HT on XCD i < n take the roles of picture i's sweep (its own flags,
+// pairs and buffers); the rest leave.  Each picture's sweep is the two-team
+// sweep above, bit for bit.
+struct PtPicture {
+    SweepArgs a;
+    unsigned *flags;
+    unsigned long long *wx;
+};
+
+template <int NO, int BU>
+__global__ __launch_bounds__(PT_T) void pt_multi_kernel(const PtPicture *__restrict__ pics, int n)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+    if (xcc >= (unsigned)n) return;
+    unsigned *flags = pics[xcc].flags;
+    if (threadIdx.x == 0) {
+        (void)atomicCAS(&flags[PSF_XSEL], 0u, xcc + 1u);
+        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+        s_role = slot < (unsigned)(2 * PT_HT) ? (int)slot : -1;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    const SweepArgs a = pics[xcc].a;
+    pt_helper<NO, BU>(a, flags, role & 1, role >> 1, pics[xcc].wx, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+}
+
 // Clears the sweep's flags (all but the sticky abort word and the role
 // counts) and the nw exchange pairs before a launch, and adds the launch's
 // nroles roles to the started count.
@@ -2141,14 +2176,12 @@ int flush_cap_zero(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
-int launch_sweep(ksvd_ctx *ctx, float tol)
+// The sweep's view of a context's buffers.
+static SweepArgs sweep_args(ksvd_ctx *ctx, float tol)
 {
-    ctx->sweep_defers_zero = false;
-    const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
+    const int dim = ctx->dim;
     const int dpad = (dim + 15) / 16 * 16;
     const int nbt = dpad / 16;
-    const int ntiles = nbt * (nbt + 1) / 2;
-    const bool large = ntiles > 16 * MAX_TPW;  // dpad > 192: the LDS-resident sweeps cannot hold it
     SweepArgs a;
     a.Y = (const float *)ctx->Y.ptr;
     a.D = (float *)ctx->D.ptr;
@@ -2168,14 +2201,42 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     a.counters = (unsigned long long *)ctx->counters.ptr;
     a.dim = dim;
     a.dpad = dpad;
-    a.K = K;
-    a.kmax = kmax;
+    a.K = ctx->K;
+    a.kmax = ctx->kmax;
     a.cap = ctx->prm.user_cap;
     a.nbt = nbt;
-    a.ntiles = ntiles;
+    a.ntiles = nbt * (nbt + 1) / 2;
     a.maxit = 300;
     a.tol2 = tol * tol;
     a.est2 = 1e-12f;  // estimated error of the kept atom below 1e-6
+    return a;
+}
+
+// LDS bytes of the two-team sweep for this context's sizes
+static size_t pt_lds_bytes(const ksvd_ctx *ctx)
+{
+    const size_t dpad = (size_t)(ctx->dim + 15) / 16 * 16;
+    return sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (3 + PT_W + PT_GRP) * dpad) +
+           sizeof(int) * (2 * (size_t)PT_TABW + 2 * ((size_t)ctx->K + 1));
+}
+
+// Whether the two-team sweep applies to this context (cap up to 1024, OMP
+// limit up to 8, padded dim up to 192, its LDS within 160 KB).
+static bool pt_applies(const ksvd_ctx *ctx, bool two_kernel)
+{
+    const int cap = ctx->prm.user_cap, nbt = (ctx->dim + 15) / 16;
+    return cap > 0 && cap <= PT_HT * PS_CWMAX && ctx->kmax <= PS_NOMAX + 1 && nbt * (nbt + 1) / 2 <= 16 * MAX_TPW &&
+           !two_kernel && pt_lds_bytes(ctx) <= 160 * 1024;
+}
+
+int launch_sweep(ksvd_ctx *ctx, float tol)
+{
+    ctx->sweep_defers_zero = false;
+    const int dim = ctx->dim, K = ctx->K, kmax = ctx->kmax;
+    SweepArgs a = sweep_args(ctx, tol);
+    const int dpad = a.dpad;
+    const int ntiles = a.ntiles;
+    const bool large = ntiles > 16 * MAX_TPW;  // dpad > 192: the LDS-resident sweeps cannot hold it
     const int tpw = (ntiles + 15) / 16;
     const bool wide = kmax > 5;
 
@@ -2239,10 +2300,9 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     if (large) return launch_large();
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
-    if (cap > 0 && cap <= PT_HT * PS_CWMAX && kmax <= PS_NOMAX + 1 && !two_kernel) {
-        const size_t lds = sizeof(float) * (2 * (size_t)PS_CWMAX * dpad + (3 + PT_W + PT_GRP) * (size_t)dpad) +
-                           sizeof(int) * (2 * (size_t)PT_TABW + 2 * ((size_t)K + 1));
-        if (lds <= 160 * 1024) {
+    if (pt_applies(ctx, two_kernel)) {
+        const size_t lds = pt_lds_bytes(ctx);
+        {
             typedef void (*PtFn)(SweepArgs, unsigned *, unsigned long long *);
             // BU = 5: a member's columns (at most 64) in two passes of 35 with every
             // load in flight (2.33 -> 2.28 ms per update at 2048 x 2048 against BU = 4)
@@ -2315,6 +2375,69 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
 // Reads and clears the persistent sweep's sticky abort word and role counts
 // (synchronises the context's stream).  Returns KSVD_OK, or KSVD_ERR_HIP if a
 // sweep gave up or left roles unfinished (no workgroup took them).
+// Sets up the multi-picture sweep for a batch call: whether it applies to
+// these contexts (same sizes, the two-team sweep for each, at most one picture
+// per XCD), and the pictures' sweep arguments in ctxs[0]'s device buffer
+// (copied once, waiting for ctxs[0]'s stream).  Returns 1 when it applies, 0
+// when not, or a negative status.
+int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
+{
+    ksvd_ctx *c0 = ctxs[0];
+    const bool two_kernel = env_flag("KSVD_SWEEP_TWO_KERNEL");
+    if (n < 2 || n > c0->n_xcd || n > PT_MULTI_MAX) return 0;
+    for (int i = 0; i < n; i++) {
+        const ksvd_ctx *c = ctxs[i];
+        if (c->device != c0->device || c->dim != c0->dim || c->K != c0->K || c->kmax != c0->kmax ||
+            c->prm.user_cap != c0->prm.user_cap || c->force_two_kernel || !pt_applies(c, two_kernel))
+            return 0;
+    }
+    PtPicture h[PT_MULTI_MAX];
+    for (int i = 0; i < n; i++) {
+        h[i].a = sweep_args(ctxs[i], tol);
+        h[i].flags = (unsigned *)ctxs[i]->ticket.ptr;
+        h[i].wx = (unsigned long long *)h[i].a.Gpart;
+    }
+    int rc = ksvd_reserve(c0, c0->multi, sizeof(PtPicture) * PT_MULTI_MAX);
+    if (rc) return rc;
+    KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(PtPicture) * n, hipMemcpyHostToDevice));
+    return 1;
+}
+
+// Each picture's flag and pair reset on its own stream, before an update.
+int sweep_multi_reset(ksvd_ctx *const *ctxs, int n)
+{
+    unsigned spin = (unsigned)PS_SPIN_MAX;
+    if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(0, atoi(sv));
+    const int dpad = (ctxs[0]->dim + 15) / 16 * 16;
+    const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;
+    for (int i = 0; i < n; i++) {
+        ksvd_ctx *c = ctxs[i];
+        c->sweep_defers_zero = false;
+        hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0, c->stream,
+                           (unsigned *)c->ticket.ptr, (unsigned long long *)c->gpart.ptr, nw, (unsigned)(2 * PT_HT),
+                           spin);
+        KSVD_HIP_CHECK(c, hipGetLastError());
+    }
+    return KSVD_OK;
+}
+
+// The multi-picture sweep on ctxs[0]'s stream (after sweep_multi_setup and
+// sweep_multi_reset, and after that stream waits for the other pictures'
+// streams).
+int launch_sweep_multi(ksvd_ctx *const *ctxs, int n)
+{
+    ksvd_ctx *c0 = ctxs[0];
+    const bool wide = c0->kmax > 5;
+    typedef void (*PmFn)(const PtPicture *, int);
+    PmFn pm = wide ? pt_multi_kernel<7, 2> : pt_multi_kernel<4, 5>;
+    KSVD_HIP_CHECK(c0, allow_max_lds((const void *)pm));
+    hipLaunchKernelGGL(pm, dim3(c0->n_xcd * 2 * PT_HT), dim3(PT_T), pt_lds_bytes(c0), c0->stream,
+                       (const PtPicture *)c0->multi.ptr, n);
+    KSVD_HIP_CHECK(c0, hipGetLastError());
+    for (int i = 0; i < n; i++) ctxs[i]->sweep_defers_zero = true;
+    return KSVD_OK;
+}
+
 int check_sweep(ksvd_ctx *ctx)
 {
     unsigned v[4] = {0, 0, 0, 0};  // sticky, XCD, started, finished

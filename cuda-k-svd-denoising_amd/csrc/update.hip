@@ -218,9 +218,13 @@ __global__ __launch_bounds__(256) void cols_table_kernel(int K, int kmax, int ca
 
 int launch_sweep(ksvd_ctx *ctx, float tol);
 
-// Enqueue one sweep: user lists, column table, then the per-atom kernels.
-// Everything is stream-ordered; no host synchronisation.
-static int enqueue_update(ksvd_ctx *ctx)
+// The power iteration's tolerance on |u_i - u_{i-1}| (power_stop also stops
+// on its a-posteriori error estimate); a much smaller value would sit below
+// the rounding floor of the FP32 products and never trigger.
+constexpr float SWEEP_TOL = 1e-5f;
+
+// The user lists and the column table of a sweep, on the context's stream.
+static int enqueue_users(ksvd_ctx *ctx)
 {
     const int K = ctx->K, kmax = ctx->kmax;
     const int64_t P = ctx->P;
@@ -244,11 +248,16 @@ static int enqueue_update(ksvd_ctx *ctx)
                        (const int *)ctx->srank.ptr, (int *)ctx->tab_p.ptr, (int *)ctx->tab_self.ptr,
                        (int4 *)ctx->tab_oth.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
-    // Power iteration stops when two successive unit iterates differ by less
-    // than tol.  The error of the kept iterate is about tol * l2/l1 / (1 -
-    // l2/l1), within 1e-4 whenever l2/l1 <= 0.9; a much smaller value would
-    // sit below the rounding floor of the FP32 products and never trigger.
-    return launch_sweep(ctx, 1e-5f);
+    return KSVD_OK;
+}
+
+// Enqueue one sweep: user lists, column table, then the sweep's kernels.
+// Everything is stream-ordered; no host synchronisation.
+static int enqueue_update(ksvd_ctx *ctx)
+{
+    int rc = enqueue_users(ctx);
+    if (rc) return rc;
+    return launch_sweep(ctx, SWEEP_TOL);
 }
 
 void drop_update_graph(ksvd_ctx *ctx)
@@ -356,5 +365,44 @@ int launch_update(ksvd_ctx *ctx)
     upd_owner[ctx->device] = ctx;
     ctx->cap_zero_pending = ctx->sweep_defers_zero;
     ctx->gram_valid = false;
+    return KSVD_OK;
+}
+
+int sweep_multi_tol_setup(ksvd_ctx *const *ctxs, int n) { return sweep_multi_setup(ctxs, n, SWEEP_TOL); }
+
+// One update of several pictures: each picture's user lists and column table
+// on its own stream, then one multi-picture sweep on ctxs[0]'s stream once
+// the other pictures' streams and the device's last update are done; the
+// other streams then wait for it.  Ordered with single updates under the same
+// lock.
+int launch_update_batch(ksvd_ctx *const *ctxs, int n)
+{
+    int rc;
+    for (int i = 0; i < n; i++) {
+        if ((rc = flush_cap_zero(ctxs[i]))) return rc;
+        if ((rc = enqueue_users(ctxs[i]))) return rc;
+    }
+    if ((rc = sweep_multi_reset(ctxs, n))) return rc;
+    ksvd_ctx *c0 = ctxs[0];
+    if (c0->device < 0 || c0->device >= UPD_MAX_DEVICES)
+        return ksvd_fail(c0, KSVD_ERR_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(upd_mu);
+    for (int i = 1; i < n; i++) {
+        ksvd_ctx *c = ctxs[i];
+        if (!c->batch_ev) KSVD_HIP_CHECK(c, hipEventCreateWithFlags((hipEvent_t *)&c->batch_ev, hipEventDisableTiming));
+        KSVD_HIP_CHECK(c, hipEventRecord((hipEvent_t)c->batch_ev, c->stream));
+        KSVD_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, (hipEvent_t)c->batch_ev, 0));
+    }
+    hipEvent_t &last = upd_last[c0->device];
+    if (!last) KSVD_HIP_CHECK(c0, hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    else KSVD_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, last, 0));
+    if ((rc = launch_sweep_multi(ctxs, n))) return rc;
+    KSVD_HIP_CHECK(c0, hipEventRecord(last, c0->stream));
+    upd_owner[c0->device] = c0;
+    for (int i = 1; i < n; i++) KSVD_HIP_CHECK(ctxs[i], hipStreamWaitEvent(ctxs[i]->stream, last, 0));
+    for (int i = 0; i < n; i++) {
+        ctxs[i]->cap_zero_pending = ctxs[i]->sweep_defers_zero;
+        ctxs[i]->gram_valid = false;
+    }
     return KSVD_OK;
 }

@@ -87,6 +87,8 @@ SIGNATURES = {
     "ksvd_update_dictionary": (ctypes.c_int, [_P]),
     "ksvd_iterate": (ctypes.c_int, [_P, ctypes.c_int]),
     "ksvd_run": (ctypes.c_int, [_P]),
+    "ksvd_iterate_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int]),
+    "ksvd_run_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
     "ksvd_run_final_code": (ctypes.c_int, [_P]),
     "ksvd_rebuild": (ctypes.c_int, [_P, ctypes.c_float, _FP]),
     "ksvd_get_rebuild_parts": (ctypes.c_int, [_P, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double)]),
@@ -361,6 +363,28 @@ class KSvd:
 
     def synchronize(self):
         self._check(lib().ksvd_synchronize(self._h))
+
+
+def _handles(ctxs):
+    return (_P * len(ctxs))(*[k._h for k in ctxs])
+
+
+def _batch_check(ctxs, rc):
+    if rc != KSVD_OK:
+        msgs = [lib().ksvd_last_error(k._h).decode() for k in ctxs]
+        raise KsvdError(rc, next((m for m in msgs if m), ""))
+
+
+def iterate_batch(ctxs, iters):
+    """ksvd_iterate_batch: several pictures' iterations at once (one
+    dictionary sweep launch with picture i on XCD i)."""
+    _batch_check(ctxs, lib().ksvd_iterate_batch(_handles(ctxs), len(ctxs), iters))
+
+
+def run_batch(ctxs):
+    """ksvd_run_batch: ksvd_iter iterations of every picture at once, then
+    each picture's final code."""
+    _batch_check(ctxs, lib().ksvd_run_batch(_handles(ctxs), len(ctxs)))
 
 
 def omp_noalloc(Y, dim, P, D, K, kmax, idx, coef, cnt, stream=0):

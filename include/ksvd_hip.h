@@ -176,6 +176,20 @@ int ksvd_iterate(ksvd_ctx *ctx, int iters);
  * ksvd_iter iterations, then the final OMP. */
 int ksvd_run(ksvd_ctx *ctx);
 
+/* Several pictures' K-SVD at once.  The reference denoises a batch's files
+ * one after another (the loop over BatchDenoiser::getDenoiserList in
+ * main.cpp:60-68, each file's Denoiser::denoising running
+ * CudaKSvdDenoiser::kSvd); here n contexts of one device
+ * with the same patch size, atoms, OMP limit and user cap (plain FP32 form)
+ * run their iterations together: each picture's pursuit on its own stream,
+ * then one dictionary sweep launch with picture i on XCD i (n at most the
+ * device's XCD count).  Each picture's result is the one ksvd_iterate gives
+ * it alone, bit for bit.  Contexts that cannot share a sweep iterate one
+ * after another.  ksvd_run_batch: ksvd_iter iterations (the same for all),
+ * then each picture's final code. */
+int ksvd_iterate_batch(ksvd_ctx *const *ctxs, int n, int iters);
+int ksvd_run_batch(ksvd_ctx *const *ctxs, int n);
+
 /* The final sparse code of kSvd (cuda_svd_denoiser.cu:400-401): ksvd_omp,
  * or, in the BF16 + learned form once the proposer is trained,
  * ksvd_proposer_omp. */

@@ -667,3 +667,49 @@ def test_sweep_that_gives_up_is_undone_and_run_again(kl, monkeypatch):
     np.testing.assert_array_equal(da, ks.dictionary())
     np.testing.assert_array_equal(ca, ks.codes()[1])
     ks.close()
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_batch_iterations_give_each_picture_its_bits(kl, n):
+    """ksvd_iterate_batch / ksvd_run_batch over n pictures of different sizes
+    (one sweep launch, picture i on XCD i) give every picture the dictionary
+    and codes ksvd_iterate / ksvd_run give it alone, bit for bit, with no
+    fallback; a batch that cannot share a sweep (a context twice) is refused."""
+    jobs = []
+    for i in range(n):
+        W, H = 96 + 16 * (i % 3), 80 + 16 * (i % 4)
+        jobs.append((kl.image_to_frame(kl.synth_image(30 + i, W, H, 20.0)[1]), W, H))
+
+    def ctx(f, W, H):
+        ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5, ksvd_iter=3)
+        ks.set_frame(f, W, H)
+        ks.extract()
+        ks.init_dictionary()
+        return ks
+
+    alone = []
+    for f, W, H in jobs:
+        ks = ctx(f, W, H)
+        ks.run()
+        alone.append((ks.dictionary(), ks.codes()))
+        ks.close()
+    ctxs = [ctx(f, W, H) for f, W, H in jobs]
+    kl.run_batch(ctxs)
+    for i, ks in enumerate(ctxs):
+        np.testing.assert_array_equal(ks.dictionary(), alone[i][0], err_msg=f"picture {i}")
+        for x, y in zip(ks.codes(), alone[i][1]):
+            np.testing.assert_array_equal(x, y, err_msg=f"picture {i}")
+        assert ks.timings()["sweep_fallbacks"] == 0
+    # two more iterations of the same batch, against two alone
+    kl.iterate_batch(ctxs, 2)
+    d_b = [ks.dictionary() for ks in ctxs]
+    with pytest.raises(kl.KsvdError):
+        kl.iterate_batch([ctxs[0], ctxs[0]], 1)
+    for ks in ctxs:
+        ks.close()
+    for i, (f, W, H) in enumerate(jobs):
+        ks = ctx(f, W, H)
+        ks.run()
+        ks.iterate(2)
+        np.testing.assert_array_equal(ks.dictionary(), d_b[i], err_msg=f"picture {i}, iterations 4-5")
+        ks.close()
