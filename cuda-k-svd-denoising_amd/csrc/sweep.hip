@@ -1791,7 +1791,7 @@ constexpr int KL_T = 1024;
 constexpr int KG_TB = 64, KG_KC = 64;
 constexpr int KG_SPLIT = 4;  // column ranges of the large-patch Gram when the persistent power iteration reads it
 
-__global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, int k)
+__device__ __forceinline__ void kl_build_body(const SweepArgs &a, float *E, int k)
 {
     int c;
     const int m = atom_m(a.uoff, k, a.cap, &c);
@@ -1813,6 +1813,25 @@ __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, i
     }
 }
 
+__global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, int k) { kl_build_body(a, E, k); }
+
+// A batch's picture in the large-patch kernels: its sweep arguments, E_k, its
+// (partial) Grams and the power iteration's flags, pairs and per-atom words.
+struct KlPicture {
+    SweepArgs a;
+    float *E, *G;
+    unsigned *flags;
+    unsigned long long *wx;
+    unsigned *kw;
+};
+
+// picture blockIdx.z of the batch
+__global__ __launch_bounds__(KL_T) void kl_build_multi(const KlPicture *__restrict__ pics, int k)
+{
+    const KlPicture *p = pics + blockIdx.z;
+    kl_build_body(p->a, p->E, k);
+}
+
 // Tile (bi, bj), bi <= bj, of G = E E^T (dim x dim, pitch gp), four
 // wavefronts of 32 x 32; the columns of E in chunks of KG_KC staged in LDS,
 // the next chunk's loads in flight (registers) while the current one is on
@@ -1821,8 +1840,8 @@ __global__ __launch_bounds__(KL_T) void kl_build_kernel(SweepArgs a, float *E, i
 // partials in order.
 constexpr int KG_LD = (KG_KC * KG_TB) / 256;  // values of each operand a thread stages per chunk
 
-__global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *__restrict__ E, float *__restrict__ G,
-                                                      int gp, int k)
+__device__ __forceinline__ void kl_gram_body(const SweepArgs &a, const float *__restrict__ E, float *__restrict__ G,
+                                             int gp, int k)
 {
     __shared__ float Ea[KG_KC][KG_TB + 4];
     __shared__ float Eb[KG_KC][KG_TB + 4];
@@ -1896,6 +1915,18 @@ __global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *
                     G[(int64_t)f * gp + e] = acc[x][y][i];
                 }
             }
+}
+
+__global__ __launch_bounds__(256) void kl_gram_kernel(SweepArgs a, const float *__restrict__ E, float *__restrict__ G,
+                                                      int gp, int k)
+{
+    kl_gram_body(a, E, G, gp, k);
+}
+
+__global__ __launch_bounds__(256) void kl_gram_multi(const KlPicture *__restrict__ pics, int gp, int k)
+{
+    const KlPicture *p = pics + blockIdx.z;
+    kl_gram_body(p->a, p->E, p->G, gp, k);
 }
 
 // Power iteration on G (global, symmetric, row pitch gp = dim rounded up to
@@ -2014,30 +2045,13 @@ __device__ __forceinline__ float kq_block_sum(float v, float *red)
     return s;
 }
 
-__global__ __launch_bounds__(KQ_T) void kq_power_kernel(SweepArgs a, const float *__restrict__ G, int gp, int nsplit,
-                                                        const float *__restrict__ E, int k, unsigned *flags,
-                                                        unsigned long long *wx, unsigned *kw)
+// The power iteration and coefficients of one role (see kq_power_kernel).
+__device__ __forceinline__ void kq_power_body(const SweepArgs &a, const float *__restrict__ G, int gp, int nsplit,
+                                              const float *__restrict__ E, int k, unsigned *flags,
+                                              unsigned long long *wx, int role, int *s_bail, float *gs, float *u,
+                                              float *red)
 {
-    extern __shared__ __attribute__((aligned(16))) float gs[];  // rpr x gp
-    __shared__ float u[KSVD_MAX_DIM];
-    __shared__ float red[KQ_T / 64];
-    __shared__ int s_role, s_bail[2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) {
-        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
-        const unsigned old = atomicCAS(&kw[k], 0u, xcc + 1u);
-        int role = -1;
-        if ((old == 0u ? xcc + 1u : old) == xcc + 1u) {
-            const unsigned r = atomicAdd(&kw[a.K + k], 1u);
-            if (r < (unsigned)KQ_R) role = (int)r;
-        }
-        s_role = role;
-        s_bail[0] = 0;
-        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
-    }
-    __syncthreads();
-    const int role = s_role;
-    if (role < 0) return;
     int c;
     const int m = atom_m(a.uoff, k, a.cap, &c);
     const int dim = a.dim;
@@ -2131,6 +2145,54 @@ __global__ __launch_bounds__(KQ_T) void kq_power_kernel(SweepArgs a, const float
         if (role == 0) atomicAdd(&a.counters[CNT_EIG_IT], (unsigned long long)it);
         atomicAdd(&flags[PSF_FINISHED], 1u);
     }
+}
+
+__global__ __launch_bounds__(KQ_T) void kq_power_kernel(SweepArgs a, const float *__restrict__ G, int gp, int nsplit,
+                                                        const float *__restrict__ E, int k, unsigned *flags,
+                                                        unsigned long long *wx, unsigned *kw)
+{
+    extern __shared__ __attribute__((aligned(16))) float gs[];  // rpr x gp
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float red[KQ_T / 64];
+    __shared__ int s_role, s_bail[2];
+    if (threadIdx.x == 0) {
+        const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+        const unsigned old = atomicCAS(&kw[k], 0u, xcc + 1u);
+        int role = -1;
+        if ((old == 0u ? xcc + 1u : old) == xcc + 1u) {
+            const unsigned r = atomicAdd(&kw[a.K + k], 1u);
+            if (r < (unsigned)KQ_R) role = (int)r;
+        }
+        s_role = role;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    if (s_role < 0) return;
+    kq_power_body(a, G, gp, nsplit, E, k, flags, wx, s_role, s_bail, gs, u, red);
+}
+
+// Several pictures' power iterations in one launch, picture i on XCD i (its
+// roles counted in its own per-atom word).
+__global__ __launch_bounds__(KQ_T) void kq_power_multi(const KlPicture *__restrict__ pics, int n, int gp, int nsplit,
+                                                       int k)
+{
+    extern __shared__ __attribute__((aligned(16))) float gs[];  // rpr x gp
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float red[KQ_T / 64];
+    __shared__ int s_role, s_bail[2];
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+    if (xcc >= (unsigned)n) return;
+    const KlPicture *p = pics + xcc;
+    if (threadIdx.x == 0) {
+        const unsigned r = atomicAdd(&p->kw[p->a.K + k], 1u);
+        s_role = r < (unsigned)KQ_R ? (int)r : -1;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(p->flags + PSF_SPIN);
+    }
+    __syncthreads();
+    if (s_role < 0) return;
+    kq_power_body(p->a, p->G, gp, nsplit, p->E, k, p->flags, p->wx, s_role, s_bail, gs, u, red);
 }
 
 __global__ __launch_bounds__(256) void kl_coef_kernel(SweepArgs a, const float *__restrict__ E, int k)
@@ -2229,6 +2291,33 @@ static bool pt_applies(const ksvd_ctx *ctx, bool two_kernel)
            !two_kernel && pt_lds_bytes(ctx) <= 160 * 1024;
 }
 
+// Where the large-patch kernels keep their data in a context's partial-Gram
+// buffer: the Gram (nsplit partials, row pitch gp), then the power
+// iteration's pairs (2 gp) and per-atom words (K + K), cleared as nw 64-bit
+// words; and the power iteration's LDS.  False when it does not fit.
+struct KlLayout {
+    int gp, nsplit, nw;
+    size_t lds_q;
+    float *G;
+    unsigned long long *wx;
+    unsigned *kw;
+};
+
+static bool kl_layout(const ksvd_ctx *ctx, int nsplit, KlLayout *L)
+{
+    const int dim = ctx->dim;
+    L->gp = (dim + 3) / 4 * 4;
+    L->nsplit = nsplit;
+    L->G = (float *)ctx->gpart.ptr;
+    unsigned long long *wx = (unsigned long long *)(L->G + (size_t)nsplit * dim * L->gp + 4);
+    L->wx = (unsigned long long *)(((uintptr_t)wx + 15) & ~(uintptr_t)15);
+    L->kw = (unsigned *)(L->wx + 2 * (size_t)L->gp);
+    L->nw = 2 * L->gp + ctx->K;
+    L->lds_q = sizeof(float) * (size_t)((dim + KQ_R - 1) / KQ_R) * L->gp;
+    return dim <= KSVD_MAX_DIM && (char *)(L->wx + L->nw) <= (char *)ctx->gpart.ptr + ctx->gpart.bytes &&
+           L->lds_q <= 150 * 1024;
+}
+
 int launch_sweep(ksvd_ctx *ctx, float tol)
 {
     ctx->sweep_defers_zero = false;
@@ -2250,10 +2339,6 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     auto launch_large = [&]() -> int {
         if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
         float *E = (float *)ctx->E.ptr;  // m x dim, m <= min(users, cap): the E buffer's columns
-        float *G = a.Gpart;              // dim x gp
-        const int gp = (dim + 3) / 4 * 4;
-        if (sizeof(float) * (size_t)dim * gp > ctx->gpart.bytes)
-            return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: Gram buffer too small");
         const int nb = (dim + KG_TB - 1) / KG_TB;
         const int ntg = nb * (nb + 1) / 2;
         // enough workgroups for one pass over the most columns an atom can
@@ -2267,16 +2352,18 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
         const bool persistent = !two_kernel;
         // the persistent power iteration reads the Gram once per atom, so the
         // Gram is split over KG_SPLIT column ranges (more workgroups on MFMA)
-        const int nsplit = persistent ? KG_SPLIT : 1;
+        KlLayout L;
+        const bool fits = kl_layout(ctx, persistent ? KG_SPLIT : 1, &L);
+        const int gp = L.gp, nsplit = L.nsplit, nw = L.nw;
+        const size_t lds_q = L.lds_q;
+        float *G = L.G;
         unsigned *flags = (unsigned *)ctx->ticket.ptr;
-        unsigned long long *wx = (unsigned long long *)(G + (size_t)nsplit * dim * gp + 4);
-        wx = (unsigned long long *)(((uintptr_t)wx + 15) & ~(uintptr_t)15);
-        unsigned *kw = (unsigned *)(wx + 2 * (size_t)gp);
-        const int nw = 2 * gp + K;  // 64-bit words cleared: the pairs, then K + K 32-bit per-atom words
-        const size_t lds_q = sizeof(float) * (size_t)((dim + KQ_R - 1) / KQ_R) * gp;
+        unsigned long long *wx = L.wx;
+        unsigned *kw = L.kw;
+        if (!persistent && sizeof(float) * (size_t)dim * gp > ctx->gpart.bytes)
+            return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: Gram buffer too small");
         if (persistent) {
-            if ((char *)(wx + nw) > (char *)a.Gpart + ctx->gpart.bytes || lds_q > 150 * 1024)
-                return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: large-patch power iteration does not fit");
+            if (!fits) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "sweep: large-patch power iteration does not fit");
             KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kq_power_kernel));
             hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
                                ctx->stream, flags, wx, nw, (unsigned)(K * KQ_R), spin);
@@ -2384,22 +2471,43 @@ int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
 {
     ksvd_ctx *c0 = ctxs[0];
     const bool two_kernel = env_flag("KSVD_SWEEP_TWO_KERNEL");
-    if (n < 2 || n > c0->n_xcd || n > PT_MULTI_MAX) return 0;
+    c0->multi_mode = 0;
+    if (n < 2 || n > c0->n_xcd || n > PT_MULTI_MAX || two_kernel) return 0;
+    const int nbt = (c0->dim + 15) / 16;
+    const bool large = nbt * (nbt + 1) / 2 > 16 * MAX_TPW;
+    KlLayout L;
     for (int i = 0; i < n; i++) {
         const ksvd_ctx *c = ctxs[i];
         if (c->device != c0->device || c->dim != c0->dim || c->K != c0->K || c->kmax != c0->kmax ||
-            c->prm.user_cap != c0->prm.user_cap || c->force_two_kernel || !pt_applies(c, two_kernel))
+            c->prm.user_cap != c0->prm.user_cap || c->force_two_kernel)
             return 0;
+        if (large ? !kl_layout(c, KG_SPLIT, &L) : !pt_applies(c, two_kernel)) return 0;
     }
-    PtPicture h[PT_MULTI_MAX];
-    for (int i = 0; i < n; i++) {
-        h[i].a = sweep_args(ctxs[i], tol);
-        h[i].flags = (unsigned *)ctxs[i]->ticket.ptr;
-        h[i].wx = (unsigned long long *)h[i].a.Gpart;
-    }
-    int rc = ksvd_reserve(c0, c0->multi, sizeof(PtPicture) * PT_MULTI_MAX);
+    const size_t bytes = std::max(sizeof(PtPicture), sizeof(KlPicture)) * PT_MULTI_MAX;
+    int rc = ksvd_reserve(c0, c0->multi, bytes);
     if (rc) return rc;
-    KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(PtPicture) * n, hipMemcpyHostToDevice));
+    if (!large) {
+        PtPicture h[PT_MULTI_MAX];
+        for (int i = 0; i < n; i++) {
+            h[i].a = sweep_args(ctxs[i], tol);
+            h[i].flags = (unsigned *)ctxs[i]->ticket.ptr;
+            h[i].wx = (unsigned long long *)h[i].a.Gpart;
+        }
+        KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(PtPicture) * n, hipMemcpyHostToDevice));
+    } else {
+        KlPicture h[PT_MULTI_MAX];
+        for (int i = 0; i < n; i++) {
+            kl_layout(ctxs[i], KG_SPLIT, &L);
+            h[i].a = sweep_args(ctxs[i], tol);
+            h[i].E = (float *)ctxs[i]->E.ptr;
+            h[i].G = L.G;
+            h[i].flags = (unsigned *)ctxs[i]->ticket.ptr;
+            h[i].wx = L.wx;
+            h[i].kw = L.kw;
+        }
+        KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(KlPicture) * n, hipMemcpyHostToDevice));
+    }
+    c0->multi_mode = large ? 2 : 1;
     return 1;
 }
 
@@ -2408,14 +2516,23 @@ int sweep_multi_reset(ksvd_ctx *const *ctxs, int n)
 {
     unsigned spin = (unsigned)PS_SPIN_MAX;
     if (const char *sv = getenv("KSVD_SWEEP_SPIN")) spin = (unsigned)std::max(0, atoi(sv));
+    const bool large = ctxs[0]->multi_mode == 2;
     const int dpad = (ctxs[0]->dim + 15) / 16 * 16;
-    const int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;
     for (int i = 0; i < n; i++) {
         ksvd_ctx *c = ctxs[i];
         c->sweep_defers_zero = false;
+        unsigned long long *wx = (unsigned long long *)c->gpart.ptr;
+        int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;
+        unsigned nroles = 2 * PT_HT;
+        if (large) {
+            KlLayout L;
+            kl_layout(c, KG_SPLIT, &L);
+            wx = L.wx;
+            nw = L.nw;
+            nroles = (unsigned)(c->K * KQ_R);
+        }
         hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0, c->stream,
-                           (unsigned *)c->ticket.ptr, (unsigned long long *)c->gpart.ptr, nw, (unsigned)(2 * PT_HT),
-                           spin);
+                           (unsigned *)c->ticket.ptr, wx, nw, nroles, spin);
         KSVD_HIP_CHECK(c, hipGetLastError());
     }
     return KSVD_OK;
@@ -2427,6 +2544,29 @@ int sweep_multi_reset(ksvd_ctx *const *ctxs, int n)
 int launch_sweep_multi(ksvd_ctx *const *ctxs, int n)
 {
     ksvd_ctx *c0 = ctxs[0];
+    if (c0->multi_mode == 2) {
+        // large patches: each atom's three launches cover every picture
+        const int dim = c0->dim, K = c0->K;
+        KlLayout L;
+        kl_layout(c0, KG_SPLIT, &L);
+        const int nb = (dim + KG_TB - 1) / KG_TB;
+        const int ntg = nb * (nb + 1) / 2;
+        const int cpt = KL_T / dim;
+        int64_t ecols = 0;
+        for (int i = 0; i < n; i++)
+            ecols = std::max<int64_t>(ecols, ctxs[i]->prm.user_cap > 0 ? ctxs[i]->prm.user_cap : ctxs[i]->P);
+        const unsigned gb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (ecols + cpt - 1) / cpt));
+        const KlPicture *pics = (const KlPicture *)c0->multi.ptr;
+        KSVD_HIP_CHECK(c0, allow_max_lds((const void *)kq_power_multi));
+        for (int k = 0; k < K; k++) {
+            hipLaunchKernelGGL(kl_build_multi, dim3(gb, 1, n), dim3(KL_T), 0, c0->stream, pics, k);
+            hipLaunchKernelGGL(kl_gram_multi, dim3(ntg, L.nsplit, n), dim3(256), 0, c0->stream, pics, L.gp, k);
+            hipLaunchKernelGGL(kq_power_multi, dim3(c0->n_xcd * KQ_R), dim3(KQ_T), L.lds_q, c0->stream, pics, n, L.gp,
+                               L.nsplit, k);
+        }
+        KSVD_HIP_CHECK(c0, hipGetLastError());
+        return KSVD_OK;
+    }
     const bool wide = c0->kmax > 5;
     typedef void (*PmFn)(const PtPicture *, int);
     PmFn pm = wide ? pt_multi_kernel<7, 2> : pt_multi_kernel<4, 5>;

@@ -669,19 +669,21 @@ def test_sweep_that_gives_up_is_undone_and_run_again(kl, monkeypatch):
     ks.close()
 
 
-@pytest.mark.parametrize("n", [2, 8])
-def test_batch_iterations_give_each_picture_its_bits(kl, n):
+@pytest.mark.parametrize("patch,n", [((6, 6), 2), ((6, 6), 8), ((22, 22), 2), ((22, 22), 8)],
+                         ids=["6x6_2", "6x6_8", "22x22_2", "22x22_8"])
+def test_batch_iterations_give_each_picture_its_bits(kl, patch, n):
     """ksvd_iterate_batch / ksvd_run_batch over n pictures of different sizes
-    (one sweep launch, picture i on XCD i) give every picture the dictionary
-    and codes ksvd_iterate / ksvd_run give it alone, bit for bit, with no
-    fallback; a batch that cannot share a sweep (a context twice) is refused."""
+    (one sweep launch per update, or per atom for 22 x 22 patches, picture i
+    on XCD i) give every picture the dictionary and codes ksvd_iterate /
+    ksvd_run give it alone, bit for bit, with no fallback; a batch with a
+    context twice is refused."""
     jobs = []
     for i in range(n):
         W, H = 96 + 16 * (i % 3), 80 + 16 * (i % 4)
         jobs.append((kl.image_to_frame(kl.synth_image(30 + i, W, H, 20.0)[1]), W, H))
 
     def ctx(f, W, H):
-        ks = kl.KSvd(patch=(6, 6), slide=(2, 2), atoms=64, omp_iter=5, ksvd_iter=3)
+        ks = kl.KSvd(patch=patch, slide=(2, 2), atoms=64, omp_iter=5, ksvd_iter=3)
         ks.set_frame(f, W, H)
         ks.extract()
         ks.init_dictionary()
