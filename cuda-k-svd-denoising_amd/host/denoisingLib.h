@@ -111,10 +111,19 @@ class denoising::HipKSvdDenoiser : public denoising::Denoiser {
 public:
     ~HipKSvdDenoiser() override;
     signed char denoising() override;
+    // Denoises the batch's files whose K-SVD can share dictionary-sweep
+    // launches (the plain form, no sub-images, the same patch and sliding
+    // sizes, atoms, OMP and K-SVD iterations, user cap, geometry and device),
+    // up to eight at a time (one per XCD) through ksvd_run_batch, loading and
+    // saving each as denoising() does; their later denoising() calls return
+    // the stored status.  Each output is the bits denoising() gives alone.
+    static void denoiseGroups(const std::vector<Denoiser *> &list);
     // Throughput of the last run: patches * ksvdIter / time of the K-SVD
     // iterations (device events), and the patch count.
     double patchItersPerSec() const { return patchIters; }
     long long patches() const { return nPatches; }
+    // Pictures of the sweep launches this file's K-SVD shared (1: alone).
+    int groupSize() const { return groupN; }
 
 protected:
     bool loadImage() override;
@@ -127,7 +136,18 @@ private:
     long long nPatches = 0;
     double iterMs = 0, patchIterCount = 0;
 
+    bool grouped = false;         // denoised by denoiseGroups
+    int groupN = 1;               // pictures of its group
+    signed char groupStatus = 0;  // its denoising() status
+
     HipKSvdDenoiser();
+    // Whether this file's K-SVD can share sweep launches with others, and
+    // the key of the files it can share them with.
+    bool groupable() const;
+    std::string groupKey() const;
+    // The rest of internalDenoising after the K-SVD (final code done): the
+    // rebuild from ctx into the output picture.
+    bool finishFrame(ksvd_ctx *ctx, int W, int H);
     // A context with this denoiser's parameters (nullptr and error on failure).
     ksvd_ctx *makeContext();
     // One frame buffer (m = W, n = H) through the whole K-SVD path on ctx;

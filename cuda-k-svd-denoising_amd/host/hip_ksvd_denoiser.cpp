@@ -2,6 +2,7 @@
 // cuda_svd_denoiser.cu:90-216) on gfx950 through the C ABI of libksvd_hip.so.
 // Everything from the patch gather to the lambda blend stays on the device;
 // the host sends the frame buffer and receives the denoised buffer.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <iostream>
@@ -19,6 +20,7 @@ HipKSvdDenoiser::~HipKSvdDenoiser() {}
 // cuda_svd_denoiser.cu:90-102
 signed char HipKSvdDenoiser::denoising()
 {
+    if (grouped) return groupStatus;
     if (!loadImage()) return -1;
     if (!internalDenoising()) return -2;
     if (!saveImage()) return -3;
@@ -161,4 +163,122 @@ bool HipKSvdDenoiser::internalDenoising()
     timeElapsed->working =
         std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
     return true;
+}
+
+bool HipKSvdDenoiser::groupable() const
+{
+    return type != HIP_K_SVD_BF16 && type != HIP_K_SVD_BF16_LEARNED && (subImageWidthDim == 0 || subImageHeightDim == 0);
+}
+
+std::string HipKSvdDenoiser::groupKey() const
+{
+    return std::to_string(patchWidthDim) + "," + std::to_string(patchHeightDim) + "," + std::to_string(slidingWidth) +
+           "," + std::to_string(slidingHeight) + "," + std::to_string(atoms) + "," + std::to_string(ompIter) + "," +
+           std::to_string(iter) + "," + std::to_string(userCap) + "," + std::to_string(geometry) + "," +
+           std::to_string(device);
+}
+
+bool HipKSvdDenoiser::finishFrame(ksvd_ctx *ctx, int W, int H)
+{
+    int rc;
+    ksvd_timings t;
+    if ((rc = ksvd_get_timings(ctx, &t))) {
+        error = std::string("HIP backend: ") + ksvd_last_error(ctx) + " (status " + std::to_string(rc) + ")";
+        return false;
+    }
+    int dim = 0, K = 0, kmax = 0;
+    int64_t P = 0;
+    ksvd_get_dims(ctx, &dim, &P, &K, &kmax);
+    nPatches = P;
+    iterMs = t.omp_ms + t.update_ms;
+    patchIterCount = (double)P * iter;
+    patchIters = iterMs > 0 ? patchIterCount / (iterMs * 1e-3) : 0;
+    std::vector<float> out((size_t)W * H, 0.f);
+    if ((rc = ksvd_rebuild(ctx, sigma, out.data()))) {
+        error = std::string("HIP backend: ") + ksvd_last_error(ctx) + " (status " + std::to_string(rc) + ")";
+        return false;
+    }
+    if (speckle) ksvd_speckle_exp(out.data(), (int64_t)out.size());
+    std::vector<float> *pic = new std::vector<float>((size_t)W * H);
+    ksvd_frame_to_image(out.data(), W, H, pic->data());
+    outputMatrix = new Matrix(H, W, H, pic);
+    return true;
+}
+
+void HipKSvdDenoiser::denoiseGroups(const std::vector<Denoiser *> &list)
+{
+    // the groupable files by key, in file order
+    std::vector<std::pair<std::string, std::vector<HipKSvdDenoiser *>>> groups;
+    for (Denoiser *d : list) {
+        auto *h = dynamic_cast<HipKSvdDenoiser *>(d);
+        if (!h || !h->groupable()) continue;
+        const std::string key = h->groupKey();
+        auto it = std::find_if(groups.begin(), groups.end(), [&](const auto &g) { return g.first == key; });
+        if (it == groups.end()) groups.push_back({key, {h}});
+        else it->second.push_back(h);
+    }
+    const int per = 8;  // pictures per sweep launch: one per XCD
+    for (auto &g : groups) {
+        for (size_t g0 = 0; g0 + 1 < g.second.size(); g0 += per) {
+            const size_t gn = std::min(g.second.size() - g0, (size_t)per);
+            if (gn < 2) break;
+            std::vector<HipKSvdDenoiser *> mem(g.second.begin() + g0, g.second.begin() + g0 + gn);
+            std::vector<HipKSvdDenoiser *> live;
+            std::vector<ksvd_ctx *> ctxs;
+            for (HipKSvdDenoiser *h : mem) {
+                h->grouped = true;
+                if (!h->loadImage()) {
+                    h->groupStatus = -1;
+                    continue;
+                }
+                ksvd_ctx *ctx = h->makeContext();
+                int rc = ctx ? 0 : -1;
+                const int W = h->inputMatrix->m, H = h->inputMatrix->n;
+                if (!rc && ((rc = ksvd_set_image(ctx, h->inputMatrix->hostVector->data(), W, H)) ||
+                            (rc = ksvd_extract_patches(ctx)) || (rc = ksvd_init_dictionary(ctx)) ||
+                            (rc = ksvd_reset_timings(ctx)))) {
+                    h->error = std::string("HIP backend: ") + ksvd_last_error(ctx) + " (status " + std::to_string(rc) + ")";
+                    ksvd_destroy(ctx);
+                    ctx = nullptr;
+                }
+                if (!ctx) {
+                    h->groupStatus = -2;
+                    continue;
+                }
+                live.push_back(h);
+                ctxs.push_back(ctx);
+            }
+            if (!ctxs.empty()) {
+                const auto t1 = std::chrono::steady_clock::now();
+                const int rc = ksvd_run_batch(ctxs.data(), (int)ctxs.size());
+                const double ms =
+                    (double)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t1)
+                        .count() * 1e-3;
+                for (size_t i = 0; i < live.size(); i++) {
+                    HipKSvdDenoiser *h = live[i];
+                    h->groupN = (int)live.size();
+                    if (rc) {
+                        std::string msg;
+                        for (ksvd_ctx *c : ctxs)
+                            if (msg.empty()) msg = ksvd_last_error(c);
+                        h->error = "HIP backend: " + msg + " (status " + std::to_string(rc) + ")";
+                        h->groupStatus = -2;
+                        continue;
+                    }
+                    const auto t2 = std::chrono::steady_clock::now();
+                    if (!h->finishFrame(ctxs[i], h->inputMatrix->m, h->inputMatrix->n)) {
+                        h->groupStatus = -2;
+                        continue;
+                    }
+                    // working time: the shared K-SVD shared out evenly, plus this picture's rebuild
+                    h->timeElapsed->working =
+                        (int64_t)(ms / (double)live.size()) +
+                        std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t2)
+                            .count();
+                    h->groupStatus = h->saveImage() ? 0 : -3;
+                }
+            }
+            for (ksvd_ctx *c : ctxs) ksvd_destroy(c);
+        }
+    }
 }

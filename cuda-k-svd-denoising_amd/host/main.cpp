@@ -13,6 +13,10 @@ static int testBatchDenoiser(const std::string &file)
     BatchDenoiser *batch = BatchDenoiser::factory(file);
     if (!batch) return 1;
     std::vector<Denoiser *> list = batch->getDenoiserList();
+    // files whose K-SVD can share dictionary-sweep launches run together
+    // (their outputs are the bits each gets alone); the report below is
+    // per file as before
+    HipKSvdDenoiser::denoiseGroups(list);
     int failures = 0;
     for (size_t i = 0; i < list.size(); i++) {
         Denoiser *d = list[i];
@@ -35,7 +39,9 @@ static int testBatchDenoiser(const std::string &file)
             }
             if (auto *h = dynamic_cast<HipKSvdDenoiser *>(d))
                 std::cout << "    K-SVD patch-iters/s: " << h->patchItersPerSec() << " (" << h->patches()
-                          << " patches)" << std::endl;
+                          << " patches" << (h->groupSize() > 1 ? ", sweeps shared by " + std::to_string(h->groupSize()) +
+                                                                     " pictures" : std::string())
+                          << ")" << std::endl;
             break;
         }
         case -1:

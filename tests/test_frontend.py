@@ -317,9 +317,10 @@ def test_cli_batch_uneven(denoising_bin, tmp_path, kl, orc):
     """A reduced batch of the uneven workload through `denoising config.json`:
     a colour picture with speckle, a grayscale one at 16 x 12 patches, one
     with a per-file 22 x 22 override and a non-square one, in one config
-    (batch_denoiser.cpp:77-112 overrides).  Each output must be the bits of
-    the same steps through the C ABI, and each C ABI result's raw PSNR within
-    0.05 dB of the FP32 oracle's (3 K-SVD iterations)."""
+    (batch_denoiser.cpp:77-112 overrides).  The three 16 x 12 files share
+    their sweep launches.  Each output must be the bits of the same steps
+    through the C ABI for that file alone, and each C ABI result's raw PSNR
+    within 0.05 dB of the FP32 oracle's (3 K-SVD iterations)."""
     rng = np.random.default_rng(17)
     specs = [dict(name="rgb_speckle.pfm", W=192, H=160, rgb=True, seed=61),
              dict(name="gray16x12.pfm", W=256, H=256, sigma=30.0, seed=62),
@@ -351,6 +352,8 @@ def test_cli_batch_uneven(denoising_bin, tmp_path, kl, orc):
     r = run(denoising_bin, cfg, tmp_path)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the three 16 x 12 files share their sweep launches (ksvd_run_batch)
+    assert r.stdout.count("sweeps shared by 3 pictures") == 3
     for sp in specs:
         pw, ph = sp.get("patch", (16, 12))
         W, H = sp["W"], sp["H"]
