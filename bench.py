@@ -342,6 +342,7 @@ def main():
         print(json.dumps({"form": name, "patch_iters_per_s": round(rate, 1), "ms_per_step": round(T * 1e3 / args.steps, 4),
                           "omp_ms_per_step": round(tm["omp_ms"] / args.steps, 4),
                           "update_ms_per_step": round(tm["update_ms"] / args.steps, 4),
+                          "sweep_fallbacks": tm["sweep_fallbacks"],
                           "psnr_raw": round(praw, 4)}), file=sys.stderr, flush=True)
     plain = res["plain"]
     head = res.get("learned", plain)

@@ -1757,6 +1757,880 @@ __global__ __launch_bounds__(PT_T) void pt_multi_kernel(const PtPicture *__restr
     if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
 }
 
+// ===========================================================================
+// Eight-team sweep (et_sweep_kernel, the default for user caps up to 1024).
+// The 32 workgroups of one XCD form 8 teams of 4 members; atom k belongs to
+// team k mod 8.  A member holds its quarter of atom k's capped columns (at
+// most 256) in registers: thread (row lane rb = lane & 15, column lane
+// q = 4 wave + lane / 16) holds rows rb + 16 i (i < NR) of columns
+// tl = 32 j + q (j < ET_CB).  An exchange of the power iteration forms
+// E_h (E_h^T u) from registers, and four members exchange their vectors.
+// A team has seven other atoms' time to prepare its next atom k:
+//   * build, once atoms up to k - 8 are certified: the columns without the
+//     shares of atoms k-7 .. k-1 (old values for the atoms after k);
+//   * window: the shares of atoms k-7 .. k-3, each once its team has
+//     certified its vector and coefficients;
+//   * speculation: the first exchange, from d_k, on the columns still without
+//     atoms k-2's and k-1's shares.  Each member publishes those of its
+//     columns that the two shares change (the list of the atom), so that
+//     every member can correct the speculative sum exactly once d_{k-1} is in:
+//     + e (e . d_k) - e' (e' . d_k) for each listed column, e' as published,
+//     e = e' - d_{k-2} x_{k-2} - d_{k-1} x_{k-1};
+//   * critical path: d_{k-1} in, atom k-1's coefficient of each listed
+//     column as e . d_{k-1} from the column of E_{k-1} that atom k-1's team
+//     stored (the same dot, bit for bit, as the stored coefficient), the
+//     correction, then the remaining exchanges.
+// Then d_k out as tagged pairs to team k+1, and the coefficients of atom
+// k's columns.  A team list longer than the stage (or an atom without users)
+// takes the plain path: atom k-2's share in the window, atom k-1's share,
+// then every exchange after it.
+
+constexpr int ET_T = 1024;                 // threads per member
+constexpr int ET_W = ET_T / 64;            // wavefronts per member
+constexpr int ET_QL = 4 * ET_W;            // column lanes (4 lane rows of 16 lanes per wavefront)
+constexpr int ET_HT = 4;                   // members per team
+constexpr int ET_NT = 8;                   // teams (ET_HT x ET_NT = one XCD's 32 CUs)
+constexpr int ET_CB = 4;                   // column slots per thread
+constexpr int ET_COLS = ET_QL * ET_CB;     // columns per member (a cap of up to 1024 per atom)
+constexpr int ET_WIN = ET_NT - 1;          // atoms whose shares are deferred past the build
+constexpr int ET_LMAX = 48;                // listed columns per team (global layout)
+constexpr int ET_LE = 1 + PS_NOMAX;        // ints per list entry (global layout)
+constexpr int ET_LM = 1 + ET_LMAX * ET_LE; // ints of a member's list: its count, then the entries
+
+// Listed columns a team stages in LDS (the speculative path's limit; a
+// longer list takes the plain path).
+__host__ __device__ constexpr int et_hch(int) { return ET_LMAX; }
+
+// Sum over the 16 lanes of a lane row, returned to each of them (rotations
+// by 8, 4, 2, 1: every lane ends with the same bits).
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += dpp_f<0x128, 0xF>(v);  // row_ror:8
+    v += dpp_f<0x124, 0xF>(v);  // row_ror:4
+    v += dpp_f<0x122, 0xF>(v);  // row_ror:2
+    v += dpp_f<0x121, 0xF>(v);  // row_ror:1
+    return v;
+}
+
+// e . d for a column held by the 16 lanes of a lane row as e[i] = row
+// rb + 16 i: fmaf over i in order, then row_sum16.  The coefficients (P4),
+// the hand-over (P2) and the speculative correction use it on the same
+// values.
+template <int NR>
+__device__ __forceinline__ float et_dot(const float *e, const float *d)
+{
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < NR; i++) s = fmaf(e[i], d[i], s);
+    return row_sum16(s);
+}
+
+// A member's table of one atom's columns, in LDS.
+struct EtTab {
+    int *sp;    // ET_COLS: patch
+    int *ss;    // ET_COLS: the column's own coefficient slot
+    int *sx;    // ET_COLS: 1 when the patch is also among atom k+1's first m_{k+1} users
+    int *hidx;  // ET_COLS: position in the hand-over list, -1 for none
+    int *hr;    // ET_COLS: rank among atom k-1's first m_{k-1} users (hand-over columns), -1 for none
+    int *hl;    // ET_COLS: the hand-over columns
+    int *s2;    // ET_COLS: 1 when a support entry of atom k-2 is deferred
+    int *sidx;  // ET_COLS: position in the list (hand-over or atom k-2 entry), -1 for none
+    int *sl;    // ET_COLS: the listed columns
+    int *eat;   // ET_COLS x NO: atom of a support entry, -1 for none
+    int *eci;   // ET_COLS x NO: its coefficient index
+    int *ek;    // ET_COLS x NO: 1 subtracted at the build, 2 deferred to the window
+    int *nh;    // the hand-over list's length
+    int *ns;    // the list's length
+};
+
+// Column table of this member's columns [tb, tb + nc) of atom kb, one thread
+// per column (threads 0 .. ET_COLS-1).  Entries of atoms updated before kb
+// (j < kb) whose patch is beyond atom j's first m_j users have coefficient 0
+// from atom j's update on and are dropped; atom kb-1's entry becomes the
+// hand-over; atoms kb-7 .. kb-2 are deferred; the rest are subtracted at the
+// build (atoms up to kb-8 with their new values, atoms after kb with their
+// old ones).  The lists follow (et_lists).
+template <int NO>
+__device__ __forceinline__ void et_table(const SweepArgs &a, const int *suo, const int *smo, int kb, int tb, int nc, const EtTab &T)
+{
+    const int no = a.kmax > 1 ? a.kmax - 1 : 1;
+    const int mprev = kb >= 1 ? pa_m(suo, kb - 1, a.cap) : 0;
+    const int mnext = kb + 1 < a.K ? pa_m(suo, kb + 1, a.cap) : 0;
+    const int64_t cb = (int64_t)smo[kb] + tb;
+    const int tl = (int)threadIdx.x;
+    if (tl >= nc) return;
+    const int64_t col = cb + tl;
+    T.sp[tl] = a.tab_p[col];
+    T.ss[tl] = a.tab_self[col];
+    int4 o[NO];
+#pragma unroll
+    for (int i = 0; i < NO; i++) o[i] = a.tab_oth[col * no + min(i, no - 1)];
+    int hr = -1, ex = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < NO; i++) {
+        const int at = i < no ? o[i].x : -1;
+        const int rk = o[i].z;
+        int kind = 0;
+        if (at == kb + 1 && rk >= 0 && rk < mnext) ex = 1;
+        if (at >= 0 && rk >= 0) {
+            if (at == kb - 1) {
+                if (rk < mprev) hr = rk;
+            } else if (at < kb) {
+                if (rk < pa_m(suo, at, a.cap)) kind = at >= kb - ET_WIN ? 2 : 1;
+            } else {
+                kind = 1;
+            }
+        }
+        if (kind == 2 && at == kb - 2) s2 = 1;
+        T.eat[tl * NO + i] = kind ? at : -1;
+        T.eci[tl * NO + i] = kind ? o[i].y : 0;
+        T.ek[tl * NO + i] = kind;
+    }
+    T.hr[tl] = hr;
+    T.sx[tl] = ex;
+    T.s2[tl] = s2;
+}
+
+// The hand-over list (columns with atom kb-1's entry) and the list (atom
+// kb-1's or kb-2's), both in ascending column order (the speculative
+// correction sums the listed columns in list order, so the order is fixed):
+// flags counted per wavefront by ballot, then a prefix over the four
+// wavefronts of the ET_COLS columns.  Every thread calls it after the table
+// is written.
+__device__ void et_lists(const EtTab &T, int nc, int *wcnt)
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool fh = tid < nc && T.hr[tid] >= 0;
+    const bool fs = tid < nc && (T.hr[tid] >= 0 || T.s2[tid] != 0);
+    const unsigned long long bh = __ballot(fh), bs = __ballot(fs);
+    if (tid < ET_COLS && lane == 0) {
+        wcnt[w] = __popcll(bh);
+        wcnt[4 + w] = __popcll(bs);
+    }
+    __syncthreads();
+    if (tid < ET_COLS) {
+        const unsigned long long lo = (1ull << lane) - 1ull;
+        int ph = 0, ps = 0;
+        for (int x = 0; x < w; x++) {
+            ph += wcnt[x];
+            ps += wcnt[4 + x];
+        }
+        ph += __popcll(bh & lo);
+        ps += __popcll(bs & lo);
+        T.hidx[tid] = fh ? ph : -1;
+        T.sidx[tid] = fs ? ps : -1;
+        if (fh) T.hl[ph] = tid;
+        if (fs) T.sl[ps] = tid;
+        if (tid == 0) {
+            *T.nh = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            *T.ns = wcnt[4] + wcnt[5] + wcnt[6] + wcnt[7];
+        }
+    }
+    __syncthreads();
+}
+
+// The build: E[i][j] = y_p - sum of the shares marked 1 (plain loads: the
+// caller has dropped its L1 lines behind the certificates of the atoms whose
+// new values it reads).
+template <int NR, int NO>
+__device__ __forceinline__ void et_build(const SweepArgs &a, float (&E)[NR][ET_CB], int nc, const EtTab &T, int q,
+                                         int rb)
+{
+    const int dim = a.dim;
+    // the patches, then one support entry at a time for the four column
+    // slots together (every load of a step in flight; a bounded register
+    // set); an entry that does not apply subtracts 0 x d
+#pragma unroll
+    for (int j = 0; j < ET_CB; j++) {
+        const int tl = ET_QL * j + q;
+        const int p = tl < nc ? T.sp[tl] : 0;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int r = rb + 16 * i;
+            const float y = a.Y[(int64_t)p * dim + min(r, dim - 1)];
+            E[i][j] = tl < nc && r < dim ? y : 0.f;
+        }
+    }
+#pragma unroll 1
+    for (int e = 0; e < NO; e++) {
+        float xv[ET_CB], dv[ET_CB][NR];
+#pragma unroll
+        for (int j = 0; j < ET_CB; j++) {
+            const int tl = ET_QL * j + q;
+            const int tc = min(tl, max(nc - 1, 0));
+            const bool k1 = tl < nc && T.ek[tc * NO + e] == 1;
+            const float x = a.coef[k1 ? T.eci[tc * NO + e] : 0];
+            xv[j] = k1 ? x : 0.f;
+            const float *dr = a.D + (int64_t)(k1 ? T.eat[tc * NO + e] : 0) * dim;
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                const float d = dr[min(rb + 16 * i, dim - 1)];
+                dv[j][i] = rb + 16 * i < dim ? d : 0.f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ET_CB; j++)
+#pragma unroll
+            for (int i = 0; i < NR; i++) E[i][j] = fmaf(-dv[j][i], xv[j], E[i][j]);
+    }
+}
+
+// The deferred shares of atom j (certified; plain loads after an L1 drop).
+template <int NR, int NO>
+__device__ __forceinline__ void et_window(const SweepArgs &a, float (&E)[NR][ET_CB], int nc, const EtTab &T, int q,
+                                          int rb, int j)
+{
+    const int dim = a.dim;
+    float dv[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const float d = a.D[(int64_t)j * dim + min(rb + 16 * i, dim - 1)];
+        dv[i] = rb + 16 * i < dim ? d : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < ET_CB; s++) {
+        const int tl = ET_QL * s + q;
+        const int tc = min(tl, max(nc - 1, 0));
+#pragma unroll
+        for (int e = 0; e < NO; e++) {
+            const bool mt = tl < nc && T.ek[tc * NO + e] == 2 && T.eat[tc * NO + e] == j;
+            if (!__any(mt)) continue;
+            const float c = a.coef[mt ? T.eci[tc * NO + e] : 0];
+            const float x = mt ? c : 0.f;
+#pragma unroll
+            for (int i = 0; i < NR; i++) E[i][s] = fmaf(-dv[i], x, E[i][s]);
+        }
+    }
+}
+
+// Waits until flags[f0 + t] >= v for the threads t < n (ps_wait3); false
+// after an abort or a wait that ran out.
+__device__ bool et_wait(unsigned *flags, int f0, int n, unsigned v, int *bail)
+{
+    return ps_wait3(flags, f0, n, v, 0, 0, 0u, 0, 0, 0u, bail);
+}
+
+// Waits until flags[f0 + i] >= v for i < n (the calling thread polls all n);
+// false after an abort or a wait that ran out.
+__device__ bool et_poll(unsigned *flags, int f0, int n, unsigned v, int *bail)
+{
+    for (int i = 0; i < n; i++) {
+        long spins = 0;
+        while (ld_flag(flags + f0 + i) < v) {
+            __builtin_amdgcn_s_sleep(2);
+            if ((++spins & 15) == 0) {
+                if (ld_flag(flags + PSF_ABORT)) {
+                    *bail = 1;
+                    return false;
+                }
+                if (spins > spin_limit(flags)) {
+                    st_flag(flags + PSF_STICKY, 1u);
+                    st_flag(flags + PSF_ABORT, 1u);
+                    *bail = 1;
+                    return false;
+                }
+            }
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ int ld_i(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Flag words of the eight-team sweep besides the certificates: per member,
+// its list and listed columns are out (atom + 1).
+constexpr int ETF_SCERT = 96;
+
+// Global scratch of the eight-team sweep in the partial-Gram buffer: the
+// exchange pairs (8 teams x 2 buffers x 4 members x dpad) and the d_k pairs
+// (8 x dpad), then each member's listed columns (ET_LMAX x dpad floats) and
+// its list (ET_LM ints).
+__host__ __device__ constexpr size_t et_pairs_n(int dpad) { return (size_t)(ET_NT * 2 * ET_HT + ET_NT) * dpad; }
+
+// (Inlined into its kernels: a call would take the kernel argument's address
+// and copy it to private memory, and every load of a buffer pointer would go
+// through it.)
+template <int NR, int NO>
+__device__ __forceinline__ void et_member(const SweepArgs &a, unsigned *flags, int team, int h,
+                                          unsigned long long *Wx, float *sm, int *bail)
+{
+    constexpr int RP = 16 * NR;      // row pitch of the LDS vectors
+    constexpr int HCH = et_hch(NR);  // listed columns staged (the plain path's hand-over chunk too)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int rb = lane & 15, q = wave * 4 + (lane >> 4);
+    const int K = a.K, cap = a.cap;
+    int dim = a.dim, dpad = a.dpad;
+    // LDS: every offset a compile-time constant except the two K-sized
+    // arrays at the end (fewer live registers)
+    constexpr int F_ZS = ET_W * RP, F_ZSP = F_ZS + RP, F_DPREV = F_ZSP + RP, F_D0 = F_DPREV + RP,
+                  F_DK2 = F_D0 + RP, F_SS = F_DK2 + RP, F_SH = F_SS + HCH * RP, F_CP = F_SH + HCH * RP,
+                  F_X2 = F_CP + HCH * RP, F_END = F_X2 + HCH * NO;
+    float *part = sm;            // ET_W x RP: per-wavefront partial sums of E (E^T u)
+    float *zs = sm + F_ZS;       // RP: the summed exchange, rows past dim 0
+    float *zsp = sm + F_ZSP;     // RP: the speculative exchange's sum
+    float *dprev = sm + F_DPREV; // RP: d_{k-1}
+    float *d0 = sm + F_D0;       // RP: the atom's starting vector
+    float *dk2 = sm + F_DK2;     // RP: d_{k-2}
+    float *stageS = sm + F_SS;   // HCH x RP: the listed columns as published
+    float *stageH = sm + F_SH;   // HCH x RP: their columns of E_{k-1} (hand-over; plain path: own)
+    float *cpart = sm + F_CP;    // HCH x RP: each listed column's correction of the speculative sum
+    float *x2 = sm + F_X2;       // HCH: atom k-2's coefficient of each listed column (entries summed)
+    int *ti = (int *)(sm + F_END);
+    constexpr int I_SS = ET_COLS, I_SX = 2 * ET_COLS, I_HIDX = 3 * ET_COLS, I_HR = 4 * ET_COLS,
+                  I_HL = 5 * ET_COLS, I_S2 = 6 * ET_COLS, I_SIDX = 7 * ET_COLS, I_SL = 8 * ET_COLS,
+                  I_EAT = 9 * ET_COLS, I_ECI = I_EAT + ET_COLS * NO, I_EK = I_ECI + ET_COLS * NO,
+                  I_NH = I_EK + ET_COLS * NO, I_NS = I_NH + 1, I_CNT = I_NS + 1, I_SRK = I_CNT + ET_HT,
+                  I_SCI = I_SRK + HCH, I_WC = I_SCI + HCH * NO, I_END = I_WC + 8;
+    EtTab T;
+    T.sp = ti;
+    T.ss = ti + I_SS;
+    T.sx = ti + I_SX;
+    T.hidx = ti + I_HIDX;
+    T.hr = ti + I_HR;
+    T.hl = ti + I_HL;
+    T.s2 = ti + I_S2;
+    T.sidx = ti + I_SIDX;
+    T.sl = ti + I_SL;
+    T.eat = ti + I_EAT;
+    T.eci = ti + I_ECI;
+    T.ek = ti + I_EK;
+    T.nh = ti + I_NH;
+    T.ns = ti + I_NS;
+    int *cntl = ti + I_CNT;  // ET_HT: the members' list lengths
+    int *srk = ti + I_SRK;   // HCH: the listed columns' ranks among atom k-1's users (-1: none)
+    int *sci = ti + I_SCI;   // HCH x NO: coefficient index of each atom k-2 entry (-1: none)
+    int *wcnt = ti + I_WC;   // 8: per-wavefront counts of the two lists
+    int *suo = ti + I_END;   // K + 1: uoff
+    int *smo = suo + K + 1;  // K + 1: moff
+
+    unsigned *ecert = flags + PTF_ECERT, *ccert = flags + PTF_CCERT;
+    const int me = team * ET_HT + h;
+    unsigned long long *Wt = Wx + (size_t)team * 2 * ET_HT * dpad;   // this team's two pair buffers
+    unsigned long long *Dp = Wx + (size_t)ET_NT * 2 * ET_HT * dpad;  // each team's d_k pairs
+    float *Esp = (float *)(Wx + et_pairs_n(dpad));
+    int *Lst = (int *)(Esp + (size_t)ET_NT * ET_HT * ET_LMAX * dpad);
+    unsigned seq = 0;
+    unsigned long long iters = 0;
+
+    for (int i = tid; i <= K; i += ET_T) {
+        suo[i] = a.uoff[i];
+        smo[i] = a.moff[i];
+    }
+    for (int i = tid; i < (ET_W + 6) * RP; i += ET_T) sm[i] = 0.f;
+    __syncthreads();
+
+    float E[NR][ET_CB];
+    float u[NR];
+    int k = team;
+    int tb = 0, nc = 0;
+    if (k < K) {
+        nc = cols_of(pa_m(suo, k, cap), ET_HT, h, &tb);
+        et_table<NO>(a, suo, smo, k, tb, nc, T);
+        if (tid < dim) d0[tid] = a.D[(int64_t)k * dim + tid];
+        __syncthreads();
+        et_lists(T, nc, wcnt);
+        et_build<NR, NO>(a, E, nc, T, q, rb);
+    }
+
+    // One exchange of the power iteration: zs = the team's sum of E_h (E_h^T
+    // v), in a fixed order (every member gets the same bits).  The stores
+    // issued before it are certified at its barrier: the hand-over columns
+    // (ecert) or the list and listed columns (scert).
+    auto exchange = [&](const float *v, bool certE, bool certS, int kk) -> bool {
+        const unsigned s = ++seq;
+        unsigned long long *Wb = Wt + (size_t)(s & 1) * ET_HT * dpad;
+        float t[ET_CB];
+#pragma unroll
+        for (int j = 0; j < ET_CB; j++) {
+            float p = 0.f;
+#pragma unroll
+            for (int i = 0; i < NR; i++) p = fmaf(E[i][j], v[i], p);
+            t[j] = row_sum16(p);
+        }
+        float w[NR];
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            float x = 0.f;
+#pragma unroll
+            for (int j = 0; j < ET_CB; j++) x = fmaf(E[i][j], t[j], x);
+            w[i] = add_xor32(add_xor16(x));
+        }
+        if ((lane >> 4) == 0) {
+#pragma unroll
+            for (int i = 0; i < NR; i++) part[wave * RP + rb + 16 * i] = w[i];
+        }
+        if (certE || certS) {
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the certified stores are out
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            if (certE) st_flag(ecert + me, (unsigned)(kk + 1));
+            if (certS) st_flag(flags + ETF_SCERT + me, (unsigned)(kk + 1));
+        }
+        if (tid < dim) {
+            float x = 0.f;
+#pragma unroll
+            for (int wv = 0; wv < ET_W; wv++) x += part[wv * RP + tid];
+            st64_c(Wb + (size_t)h * dpad + tid, ((unsigned long long)s << 32) | __float_as_uint(x));
+        }
+        // lanes 4r .. 4r+3 read member 0..3's pair of row r
+        for (int idx = tid; idx < ET_HT * dim; idx += ET_T) {
+            const int rr = idx >> 2, mm = idx & 3;
+            const unsigned long long *pp = Wb + (size_t)mm * dpad + rr;
+            unsigned long long pv = ld64_c(pp);
+            long spins = 0;
+            while ((unsigned)(pv >> 32) != s) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++spins & 31) == 0) {
+                    if (ld_flag(flags + PSF_ABORT)) {
+                        *bail = 1;
+                        break;
+                    }
+                    if (spins > spin_limit(flags)) {
+                        st_flag(flags + PSF_STICKY, 1u);
+                        st_flag(flags + PSF_ABORT, 1u);
+                        *bail = 1;
+                        break;
+                    }
+                }
+                pv = ld64_c(pp);
+            }
+            const float z = quad_sum(__uint_as_float((unsigned)pv));
+            if (mm == 0) zs[rr] = z;
+        }
+        __syncthreads();
+        return *bail == 0;
+    };
+    // u_i = zs / |zs| and the stop rule (every wavefront takes the same
+    // decision); true when the iteration stops (E = 0 keeps u)
+    int it = 0;
+    float d2prev = 0.f;
+    auto step = [&]() -> bool {
+        float z[NR], n2 = 0.f;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            z[i] = zs[rb + 16 * i];
+            n2 = fmaf(z[i], z[i], n2);
+        }
+        n2 = row_sum16(n2);
+        if (n2 == 0.f) return true;
+        const float inv = 1.f / sqrtf(n2);
+        float d2 = 0.f;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const float un = z[i] * inv;
+            const float du = un - u[i];
+            d2 = fmaf(du, du, d2);
+            u[i] = un;
+        }
+        d2 = row_sum16(d2);
+        it++;
+        if (power_stop(it, d2, d2prev, a.tol2, a.est2, a.maxit)) return true;
+        d2prev = d2;
+        return false;
+    };
+
+    for (; k < K; k += ET_NT) {
+        // the lane coordinates and sizes pass through an opaque copy each
+        // atom, so that the addresses derived from them are formed where they
+        // are used instead of being held (and spilled) across the loop
+        asm volatile("" : "+s"(dim), "+s"(dpad));
+        asm volatile("" : "+v"(rb), "+v"(q));
+        const int m = pa_m(suo, k, cap);
+        float *Ecur = (k & 1) ? a.E1 : a.E0;
+        const float *Eprev = (k & 1) ? a.E0 : a.E1;
+        const int cwp = k >= 1 ? (pa_m(suo, k - 1, cap) + ET_HT - 1) / ET_HT : 1;
+        const int tprev = (k + ET_NT - 1) % ET_NT;
+        auto owner_wait = [&](int rk, int &polled) -> bool {
+            const int owner = rk / cwp;
+            if (owner == polled) return true;
+            polled = owner;
+            return et_poll(flags, PTF_ECERT + tprev * ET_HT + owner, 1, (unsigned)k, bail);
+        };
+        auto window = [&](int j) -> bool {
+            if (!et_wait(flags, PTF_CCERT + (j % ET_NT) * ET_HT, ET_HT, (unsigned)(j + 1), bail)) return false;
+            l1_drop();
+            et_window<NR, NO>(a, E, nc, T, q, rb, j);
+            return true;
+        };
+        // ---- window: atoms k-7 .. k-3, each once certified -----------------
+        for (int j = max(0, k - ET_WIN); j <= k - 3; j++)
+            if (!window(j)) return;
+        const int nh = *T.nh, ns = *T.ns;
+        // ---- speculation ------------------------------------------------------
+        const bool spec = k >= 1 && m > 0;
+        bool sp = false;
+        int total = 0, goff = 0;
+        if (spec) {
+            // this member's list: the listed columns as they stand (without
+            // atoms k-2 and k-1), each column's rank among atom k-1's users
+            // and its atom k-2 entries' coefficient indices
+            int *lm = Lst + (size_t)me * ET_LM;
+            float *em = Esp + (size_t)me * ET_LMAX * dpad;
+#pragma unroll
+            for (int s = 0; s < ET_CB; s++) {
+                const int tl = ET_QL * s + q;
+                const int si = tl < nc ? T.sidx[tl] : -1;
+                if (si >= 0 && si < HCH) {
+#pragma unroll
+                    for (int i = 0; i < NR; i++)
+                        if (rb + 16 * i < dim) em[(size_t)si * dpad + rb + 16 * i] = E[i][s];
+                    if (rb == 0) {
+                        int *le = lm + 1 + si * ET_LE;
+                        le[0] = T.hr[tl];
+#pragma unroll
+                        for (int e = 0; e < NO; e++)
+                            le[1 + e] = T.ek[tl * NO + e] == 2 && T.eat[tl * NO + e] == k - 2 ? T.eci[tl * NO + e] : -1;
+                    }
+                }
+            }
+            if (tid == 0) lm[0] = ns;
+            // the speculative exchange from d_k; its barrier certifies the list
+#pragma unroll
+            for (int i = 0; i < NR; i++) u[i] = d0[rb + 16 * i];
+            if (!exchange(u, false, true, k)) return;
+            if (tid < dim) zsp[tid] = zs[tid];
+            // the team's list
+            if (tid < ET_HT) {
+                const bool ok = et_poll(flags, ETF_SCERT + team * ET_HT + tid, 1, (unsigned)(k + 1), bail);
+                cntl[tid] = ok ? ld_i(Lst + (size_t)(team * ET_HT + tid) * ET_LM) : 0;
+            }
+            __syncthreads();
+            if (*bail) return;
+            int off[ET_HT + 1];
+            off[0] = 0;
+#pragma unroll
+            for (int mm = 0; mm < ET_HT; mm++) off[mm + 1] = off[mm] + cntl[mm];
+            total = off[ET_HT];
+            goff = off[h];
+            sp = total <= HCH;
+            if (sp) {
+                auto member_of = [&](int g, int &i) {
+                    int mm = 0;
+#pragma unroll
+                    for (int x = 1; x < ET_HT; x++) mm += g >= off[x] ? 1 : 0;
+                    i = g - off[mm];
+                    return mm;
+                };
+                for (int idx = tid; idx < total * (1 + NO); idx += ET_T) {
+                    const int g = idx / (1 + NO), f = idx - g * (1 + NO);
+                    int i;
+                    const int mm = member_of(g, i);
+                    const int v = ld_i(Lst + (size_t)(team * ET_HT + mm) * ET_LM + 1 + i * ET_LE + f);
+                    if (f == 0) srk[g] = v;
+                    else sci[g * NO + f - 1] = v;
+                }
+                for (int qq = tid; qq < total * dim; qq += ET_T) {
+                    const int g = qq / dim, r = qq - g * dim;
+                    int i;
+                    const int mm = member_of(g, i);
+                    stageS[g * RP + r] = ld_c(Esp + ((size_t)(team * ET_HT + mm) * ET_LMAX + i) * dpad + r);
+                }
+                __syncthreads();
+                // atom k-2's vector and the listed entries' coefficients, once
+                // certified
+                if (k >= 2) {
+                    if (tid < ET_HT) et_poll(flags, PTF_CCERT + ((k - 2) % ET_NT) * ET_HT + tid, 1, (unsigned)(k - 1), bail);
+                    __syncthreads();
+                    if (*bail) return;
+                    if (tid < dim) dk2[tid] = ld_c(a.D + (int64_t)(k - 2) * dim + tid);
+                    // atom k-2's coefficient of each listed column: its entries'
+                    // coefficients summed in entry order
+                    if (tid < total) {
+                        float c[NO];
+#pragma unroll
+                        for (int e = 0; e < NO; e++) c[e] = ld_c(a.coef + max(sci[tid * NO + e], 0));
+                        float xs = 0.f;
+#pragma unroll
+                        for (int e = 0; e < NO; e++) xs += sci[tid * NO + e] >= 0 ? c[e] : 0.f;
+                        x2[tid] = xs;
+                    }
+                } else if (tid < total) {
+                    x2[tid] = 0.f;
+                }
+                // the hand-over columns of E_{k-1}, each once its owner
+                // certified its columns
+                int polled = -1;
+                for (int qq = tid; qq < total * dim; qq += ET_T) {
+                    const int g = qq / dim, r = qq - g * dim;
+                    const int rk = srk[g];
+                    if (rk < 0) continue;
+                    if (!owner_wait(rk, polled)) break;
+                    stageH[g * RP + r] = ld_c(Eprev + (size_t)rk * dpad + r);
+                }
+            }
+        }
+        if (!sp && k >= 2 && !window(k - 2)) return;
+        // ---- atom start: d_{k-1} (threads < dim); on the plain path, this
+        //      member's own hand-over columns (from thread 256 on) ----------------
+        auto stage_own = [&](int c0, int t0, int tn) {
+            const int n = min(nh - c0, HCH);
+            int polled = -1;
+            for (int qq = (int)tid - t0; qq >= 0 && qq < n * dim; qq += tn) {
+                const int i = qq / dim, r = qq - i * dim;
+                const int rk = T.hr[T.hl[c0 + i]];
+                if (!owner_wait(rk, polled)) break;
+                stageH[i * RP + r] = ld_c(Eprev + (size_t)rk * dpad + r);
+            }
+        };
+        if (k >= 1 && tid < dim) {
+            const unsigned long long *pp = Dp + (size_t)tprev * dpad + tid;
+            unsigned long long pv = ld64_c(pp);
+            long spins = 0;
+            while ((unsigned)(pv >> 32) != (unsigned)k) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((++spins & 15) == 0) {
+                    if (ld_flag(flags + PSF_ABORT)) {
+                        *bail = 1;
+                        break;
+                    }
+                    if (spins > spin_limit(flags)) {
+                        st_flag(flags + PSF_STICKY, 1u);
+                        st_flag(flags + PSF_ABORT, 1u);
+                        *bail = 1;
+                        break;
+                    }
+                }
+                pv = ld64_c(pp);
+            }
+            dprev[tid] = __uint_as_float((unsigned)pv);
+        } else if (k >= 1 && !sp && tid >= 256) {
+            stage_own(0, 256, ET_T - 256);
+        }
+        __syncthreads();
+        if (*bail) return;
+        // ---- atom k-1's share (and, speculating, atom k-2's): x = e . d_{k-1}
+        //      from the hand-over column (its rows past dim masked), then
+        //      e -= d_{k-1} x --------------------------------------------------------
+        if (sp) {
+            float dv[NR], d2v[NR];
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                dv[i] = dprev[rb + 16 * i];
+                d2v[i] = dk2[rb + 16 * i];
+            }
+#pragma unroll
+            for (int s = 0; s < ET_CB; s++) {
+                const int tl = ET_QL * s + q;
+                const int si = tl < nc ? T.sidx[tl] : -1;
+                const bool mine = si >= 0;
+                if (!__any(mine)) continue;
+                const int g = goff + max(si, 0);
+                const float x2g = mine ? x2[g] : 0.f;
+                const bool hand = mine && srk[g] >= 0;
+                float ev[NR];
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    E[i][s] = fmaf(-d2v[i], x2g, E[i][s]);
+                    ev[i] = hand && rb + 16 * i < dim ? stageH[g * RP + rb + 16 * i] : 0.f;
+                }
+                const float xd = et_dot<NR>(ev, dv);
+                const float x = hand ? xd : 0.f;
+#pragma unroll
+                for (int i = 0; i < NR; i++) E[i][s] = fmaf(-dv[i], x, E[i][s]);
+            }
+        } else if (k >= 1 && nh > 0) {
+            float dv[NR];
+#pragma unroll
+            for (int i = 0; i < NR; i++) dv[i] = dprev[rb + 16 * i];
+            for (int c0 = 0; c0 < nh; c0 += HCH) {
+                if (c0 > 0) {
+                    __syncthreads();  // the previous pass's reads of the stage are done
+                    stage_own(c0, 0, ET_T);
+                    __syncthreads();
+                    if (*bail) return;
+                }
+#pragma unroll
+                for (int s = 0; s < ET_CB; s++) {
+                    const int tl = ET_QL * s + q;
+                    const int hi = tl < nc ? T.hidx[tl] - c0 : -1;
+                    const bool mine = hi >= 0 && hi < HCH;
+                    if (!__any(mine)) continue;
+                    float ev[NR];
+#pragma unroll
+                    for (int i = 0; i < NR; i++)
+                        ev[i] = mine && rb + 16 * i < dim ? stageH[max(hi, 0) * RP + rb + 16 * i] : 0.f;
+                    const float xd = et_dot<NR>(ev, dv);
+                    const float x = mine ? xd : 0.f;
+#pragma unroll
+                    for (int i = 0; i < NR; i++) E[i][s] = fmaf(-dv[i], x, E[i][s]);
+                }
+            }
+        }
+        // ---- the columns atom k+1's hand-over reads --------------------------
+#pragma unroll
+        for (int s = 0; s < ET_CB; s++) {
+            const int tl = ET_QL * s + q;
+            if (tl < nc && T.sx[tl]) {
+                float *col = Ecur + (int64_t)(tb + tl) * dpad;
+#pragma unroll
+                for (int i = 0; i < NR; i++)
+                    if (rb + 16 * i < dim) col[rb + 16 * i] = E[i][s];
+            }
+        }
+        // ---- power iteration from d_k ------------------------------------------
+#pragma unroll
+        for (int i = 0; i < NR; i++) u[i] = d0[rb + 16 * i];
+        it = 0;
+        d2prev = 0.f;
+        bool stop = m == 0;
+        bool certified = false;
+        if (sp) {
+            // the speculative sum corrected for each listed column (every
+            // member the same bits), lane row q taking columns q, q + 32
+            for (int g0 = 0; g0 < total; g0 += ET_QL) {
+                if (!__any(g0 + q < total)) continue;
+                const int g = min(g0 + q, total - 1);
+                const bool hand = srk[g] >= 0;
+                const float x2g = x2[g];
+                float en[NR], ev[NR], dv[NR];
+                // atom k-1's coefficient of the column, then e = e' - shares
+                // (the owner's operations on the same values)
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    ev[i] = hand && rb + 16 * i < dim ? stageH[g * RP + rb + 16 * i] : 0.f;
+                    dv[i] = dprev[rb + 16 * i];
+                }
+                const float xd = et_dot<NR>(ev, dv);
+                const float xh = hand ? xd : 0.f;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const float es = rb + 16 * i < dim ? stageS[g * RP + rb + 16 * i] : 0.f;
+                    en[i] = fmaf(-dv[i], xh, fmaf(-dk2[rb + 16 * i], x2g, es));
+                }
+                const float an = et_dot<NR>(en, u);
+#pragma unroll
+                for (int i = 0; i < NR; i++) ev[i] = rb + 16 * i < dim ? stageS[g * RP + rb + 16 * i] : 0.f;
+                const float as = et_dot<NR>(ev, u);
+                if (g0 + q < total) {
+#pragma unroll
+                    for (int i = 0; i < NR; i++) cpart[g * RP + rb + 16 * i] = en[i] * an - ev[i] * as;
+                }
+            }
+            __syncthreads();
+            if (tid < dim) {
+                // four partial sums over the listed columns (g mod 4), then
+                // (d0 + d1) + (d2 + d3): a fixed order
+                float dz[4] = {0.f, 0.f, 0.f, 0.f};
+                for (int g0 = 0; g0 < total; g0 += 4) {
+                    float c[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) c[j] = g0 + j < total ? cpart[(g0 + j) * RP + tid] : 0.f;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) dz[j] += c[j];
+                }
+                zs[tid] = zsp[tid] + ((dz[0] + dz[1]) + (dz[2] + dz[3]));
+            }
+            __syncthreads();
+            stop = step();
+        }
+        while (!stop) {
+            if (!exchange(u, !certified, false, k)) return;
+            certified = true;
+            stop = step();
+        }
+        if (!certified) ps_publish(ecert + me, (unsigned)(k + 1));
+        iters += (unsigned long long)it;
+        // ---- d_k out ---------------------------------------------------------
+        if (h == 0 && tid < 16) {
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                const int rr = rb + 16 * i;
+                if (rr < dim) {
+                    st64_c(Dp + (size_t)team * dpad + rr, ((unsigned long long)(k + 1) << 32) | __float_as_uint(u[i]));
+                    a.D[(int64_t)k * dim + rr] = u[i];
+                }
+            }
+        }
+        // ---- P4: atom k's final coefficients ---------------------------------
+#pragma unroll
+        for (int s = 0; s < ET_CB; s++) {
+            const int tl = ET_QL * s + q;
+            if (!__any(tl < nc)) continue;
+            float ev[NR];
+#pragma unroll
+            for (int i = 0; i < NR; i++) ev[i] = E[i][s];
+            const float x = et_dot<NR>(ev, u);
+            if (tl < nc && rb == 0) a.coef[T.ss[tl]] = x;
+        }
+        ps_publish(ccert + me, (unsigned)(k + 1));
+        // ---- the team's next atom kn = k + 8: table, starting vector, and
+        //      once atoms k-7 .. k are certified, its columns ---------------------
+        const int kn = k + ET_NT;
+        if (kn >= K) break;
+        nc = cols_of(pa_m(suo, kn, cap), ET_HT, h, &tb);
+        __syncthreads();
+        et_table<NO>(a, suo, smo, kn, tb, nc, T);
+        if (tid < dim) d0[tid] = a.D[(int64_t)kn * dim + tid];
+        __syncthreads();
+        et_lists(T, nc, wcnt);
+        // thread t < 32 waits for member t % 4 of the team of atom k - 7 + t / 4
+        {
+            const int j = k - ET_WIN + tid / ET_HT;
+            const bool w = tid < ET_NT * ET_HT && j >= 0;
+            if (w && bail[1] == 0 && tid == 0) {
+                st_flag(flags + PSF_STICKY, 1u);
+                st_flag(flags + PSF_ABORT, 1u);
+                *bail = 1;
+            }
+            if (w && bail[1] != 0)
+                et_poll(flags, PTF_CCERT + (j % ET_NT) * ET_HT + (tid % ET_HT), 1, (unsigned)(j + 1), bail);
+        }
+        __syncthreads();
+        if (*bail) return;
+        l1_drop();
+        et_build<NR, NO>(a, E, nc, T, q, rb);
+    }
+    if (h == 0 && tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], iters);
+}
+
+template <int NR, int NO>
+__global__ __launch_bounds__(ET_T) void et_sweep_kernel(SweepArgs a, unsigned *flags, unsigned long long *Wx)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
+    if (threadIdx.x == 0) {
+        s_role = ps_register(flags, ET_NT * ET_HT);
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    et_member<NR, NO>(a, flags, role % ET_NT, role / ET_NT, Wx, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+}
+
+// Several pictures' eight-team sweeps in one launch, picture i on XCD i (as
+// pt_multi_kernel).
+template <int NR, int NO>
+__global__ __launch_bounds__(ET_T) void et_multi_kernel(const PtPicture *__restrict__ pics, int n)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail[2];
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+    if (xcc >= (unsigned)n) return;
+    unsigned *flags = pics[xcc].flags;
+    if (threadIdx.x == 0) {
+        (void)atomicCAS(&flags[PSF_XSEL], 0u, xcc + 1u);
+        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+        s_role = slot < (unsigned)(ET_NT * ET_HT) ? (int)slot : -1;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    const SweepArgs a = pics[xcc].a;
+    et_member<NR, NO>(a, flags, role % ET_NT, role / ET_NT, pics[xcc].wx, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+}
+
 // Clears the sweep's flags (all but the sticky abort word and the role
 // counts) and the nw exchange pairs before a launch, and adds the launch's
 // nroles roles to the started count.
@@ -2282,6 +3156,58 @@ static size_t pt_lds_bytes(const ksvd_ctx *ctx)
            sizeof(int) * (2 * (size_t)PT_TABW + 2 * ((size_t)ctx->K + 1));
 }
 
+// Rows per thread of the eight-team sweep (16 NR >= padded dim): 4 or 9.
+static int et_nr(int dim) { return dim <= 64 ? 4 : 9; }
+
+// Bytes of the eight-team sweep's global scratch (et_pairs_n and the
+// members' lists and listed columns).
+static size_t et_scratch_bytes(int dim)
+{
+    const int dpad = (dim + 15) / 16 * 16;
+    return sizeof(unsigned long long) * et_pairs_n(dpad) + sizeof(float) * (size_t)ET_NT * ET_HT * ET_LMAX * dpad +
+           sizeof(int) * (size_t)ET_NT * ET_HT * ET_LM;
+}
+
+// LDS bytes of the eight-team sweep for this context's sizes
+static size_t et_lds_bytes(const ksvd_ctx *ctx)
+{
+    const int nr = et_nr(ctx->dim);
+    const size_t rp = 16 * (size_t)nr, hch = (size_t)et_hch(nr);
+    const size_t no = ctx->kmax > 5 ? 7 : 4;
+    return sizeof(float) * ((ET_W + 6 + 3 * hch) * rp + hch * no) +
+           sizeof(int) * (2 * ((size_t)ctx->K + 1) + (9 + 3 * no) * (size_t)ET_COLS + 2 + ET_HT + hch * (1 + no) + 8);
+}
+
+// Whether the eight-team sweep applies: KSVD_SWEEP_TEAMS=2 selects the
+// two-team sweep instead; cap up to 1024 (256 columns per member), OMP limit
+// up to 8, dim up to 144 (at 16 x 12 patches its register-held columns spill
+// and the two-team sweep is faster), its LDS within 150 KB.
+static bool et_applies(const ksvd_ctx *ctx, bool two_kernel)
+{
+    const char *tv = getenv("KSVD_SWEEP_TEAMS");
+    if (tv && tv[0] && atoi(tv) == 2) return false;
+    const int cap = ctx->prm.user_cap;
+    return cap > 0 && cap <= ET_HT * ET_COLS && ctx->kmax <= PS_NOMAX + 1 && ctx->dim <= 144 && !two_kernel &&
+           et_lds_bytes(ctx) <= 150 * 1024 && et_scratch_bytes(ctx->dim) <= ctx->gpart.bytes;
+}
+
+typedef void (*EtFn)(SweepArgs, unsigned *, unsigned long long *);
+typedef void (*EmFn)(const PtPicture *, int);
+
+static EtFn et_kernel(const ksvd_ctx *ctx)
+{
+    const bool wide = ctx->kmax > 5;
+    if (et_nr(ctx->dim) == 4) return wide ? et_sweep_kernel<4, 7> : et_sweep_kernel<4, 4>;
+    return wide ? et_sweep_kernel<9, 7> : et_sweep_kernel<9, 4>;
+}
+
+static EmFn et_multi(const ksvd_ctx *ctx)
+{
+    const bool wide = ctx->kmax > 5;
+    if (et_nr(ctx->dim) == 4) return wide ? et_multi_kernel<4, 7> : et_multi_kernel<4, 4>;
+    return wide ? et_multi_kernel<9, 7> : et_multi_kernel<9, 4>;
+}
+
 // Whether the two-team sweep applies to this context (cap up to 1024, OMP
 // limit up to 8, padded dim up to 192, its LDS within 160 KB).
 static bool pt_applies(const ksvd_ctx *ctx, bool two_kernel)
@@ -2387,6 +3313,21 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     if (large) return launch_large();
     // ---- persistent one-XCD sweep -------------------------------------------
     const int cap = a.cap;
+    if (et_applies(ctx, two_kernel)) {
+        const EtFn et = et_kernel(ctx);
+        KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)et));
+        unsigned *flags = (unsigned *)ctx->ticket.ptr;
+        unsigned long long *wx = (unsigned long long *)a.Gpart;
+        const int nw = (int)et_pairs_n(dpad);
+        hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
+                           ctx->stream, flags, wx, nw, (unsigned)(ET_NT * ET_HT), spin);
+        // the XCDs take workgroups round robin: 32 of the n_xcd x 32 land on each
+        hipLaunchKernelGGL(et, dim3(ctx->n_xcd * ET_NT * ET_HT), dim3(ET_T), et_lds_bytes(ctx), ctx->stream, a, flags,
+                           wx);
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        ctx->sweep_defers_zero = true;
+        return KSVD_OK;
+    }
     if (pt_applies(ctx, two_kernel)) {
         const size_t lds = pt_lds_bytes(ctx);
         {
@@ -2481,7 +3422,8 @@ int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
         if (c->device != c0->device || c->dim != c0->dim || c->K != c0->K || c->kmax != c0->kmax ||
             c->prm.user_cap != c0->prm.user_cap || c->force_two_kernel)
             return 0;
-        if (large ? !kl_layout(c, KG_SPLIT, &L) : !pt_applies(c, two_kernel)) return 0;
+        if (large ? !kl_layout(c, KG_SPLIT, &L) : !(et_applies(c, two_kernel) || pt_applies(c, two_kernel))) return 0;
+        if (!large && et_applies(c, two_kernel) != et_applies(c0, two_kernel)) return 0;
     }
     const size_t bytes = std::max(sizeof(PtPicture), sizeof(KlPicture)) * PT_MULTI_MAX;
     int rc = ksvd_reserve(c0, c0->multi, bytes);
@@ -2507,7 +3449,7 @@ int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
         }
         KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(KlPicture) * n, hipMemcpyHostToDevice));
     }
-    c0->multi_mode = large ? 2 : 1;
+    c0->multi_mode = large ? 2 : et_applies(c0, two_kernel) ? 3 : 1;
     return 1;
 }
 
@@ -2524,6 +3466,10 @@ int sweep_multi_reset(ksvd_ctx *const *ctxs, int n)
         unsigned long long *wx = (unsigned long long *)c->gpart.ptr;
         int nw = 2 * 2 * PT_HT * dpad + 2 * dpad;
         unsigned nroles = 2 * PT_HT;
+        if (ctxs[0]->multi_mode == 3) {
+            nw = (int)et_pairs_n(dpad);
+            nroles = ET_NT * ET_HT;
+        }
         if (large) {
             KlLayout L;
             kl_layout(c, KG_SPLIT, &L);
@@ -2567,6 +3513,15 @@ int launch_sweep_multi(ksvd_ctx *const *ctxs, int n)
         KSVD_HIP_CHECK(c0, hipGetLastError());
         return KSVD_OK;
     }
+    if (c0->multi_mode == 3) {
+        const EmFn em = et_multi(c0);
+        KSVD_HIP_CHECK(c0, allow_max_lds((const void *)em));
+        hipLaunchKernelGGL(em, dim3(c0->n_xcd * ET_NT * ET_HT), dim3(ET_T), et_lds_bytes(c0), c0->stream,
+                           (const PtPicture *)c0->multi.ptr, n);
+        KSVD_HIP_CHECK(c0, hipGetLastError());
+        for (int i = 0; i < n; i++) ctxs[i]->sweep_defers_zero = true;
+        return KSVD_OK;
+    }
     const bool wide = c0->kmax > 5;
     typedef void (*PmFn)(const PtPicture *, int);
     PmFn pm = wide ? pt_multi_kernel<7, 2> : pt_multi_kernel<4, 5>;
@@ -2607,6 +3562,8 @@ int check_sweep(ksvd_ctx *ctx)
 size_t sweep_gpart_bytes(int dim)
 {
     const int nbt = (dim + 15) / 16;
-    // B1 (or PS_HMAX) partial Grams, then the reduced one
-    return sizeof(float) * (size_t)(std::max(B1, PS_HMAX) + 1) * (nbt * (nbt + 1) / 2) * 256;
+    // B1 (or PS_HMAX) partial Grams, then the reduced one; the eight-team
+    // sweep's pairs and speculation inputs
+    return std::max(sizeof(float) * (size_t)(std::max(B1, PS_HMAX) + 1) * (nbt * (nbt + 1) / 2) * 256,
+                    et_scratch_bytes(std::min(dim, 192)));
 }
