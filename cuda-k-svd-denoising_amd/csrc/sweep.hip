@@ -64,6 +64,7 @@ struct SweepArgs {
     int maxit;
     float tol2;  // stop when |u_i - u_{i-1}|^2 falls below it
     float est2;  // or when the estimated |u_i - limit|^2 does (power_stop)
+    int spec;    // eight-team sweep: speculate on the first exchange (0: the plain path for every atom)
 };
 
 // Stop rule of every sweep's power iteration, after iterate i with d2 =
@@ -2263,7 +2264,7 @@ __device__ __forceinline__ void et_member(const SweepArgs &a, unsigned *flags, i
             if (!window(j)) return;
         const int nh = *T.nh, ns = *T.ns;
         // ---- speculation ------------------------------------------------------
-        const bool spec = k >= 1 && m > 0;
+        const bool spec = a.spec && k >= 1 && m > 0;
         bool sp = false;
         int total = 0, goff = 0;
         if (spec) {
@@ -3096,6 +3097,12 @@ static bool env_flag(const char *name)
     return v && v[0] && v[0] != '0';
 }
 
+static bool env_flag_default(const char *name, bool dflt)
+{
+    const char *v = getenv(name);
+    return v && v[0] ? v[0] != '0' : dflt;
+}
+
 int flush_cap_zero(ksvd_ctx *ctx)
 {
     if (!ctx->cap_zero_pending) return KSVD_OK;
@@ -3145,6 +3152,8 @@ static SweepArgs sweep_args(ksvd_ctx *ctx, float tol)
     a.maxit = 300;
     a.tol2 = tol * tol;
     a.est2 = 1e-12f;  // estimated error of the kept atom below 1e-6
+    // KSVD_SWEEP_SPEC=0 (tests): the eight-team sweep's plain path for every atom
+    a.spec = env_flag_default("KSVD_SWEEP_SPEC", true) ? 1 : 0;
     return a;
 }
 

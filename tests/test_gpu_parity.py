@@ -199,13 +199,22 @@ def signfix(D):
     (dict(CFG1, k=1), 1024, "persistent"), (dict(CFG1, k=8), 1024, "persistent"),
     (dict(CFG1, k=8), 1024, "two_kernel"), (dict(CFG1, k=1), 1024, "two_kernel"),
     (CFG2, 1536, "persistent"), (dict(CFG2, k=8), 1536, "persistent"), (CFG2, 64, "persistent"),
+    (CFG1, 1024, "two_team"), (CFG2, 1024, "two_team"), (dict(CFG1, k=8), 1024, "two_team"),
+    (CFG1, 1024, "plain"), (CFG2, 1024, "plain"), (dict(CFG1, k=8), 1024, "plain"),
 ], ids=["128sq_cap", "512sq_cap", "128sq_nocap", "128sq_cap_2k", "512sq_cap_2k", "128sq_k1", "128sq_k8",
-        "128sq_k8_2k", "128sq_k1_2k", "512sq_cap1536_master", "512sq_k8_cap1536_master", "512sq_cap64"])
+        "128sq_k8_2k", "128sq_k1_2k", "512sq_cap1536_master", "512sq_k8_cap1536_master", "512sq_cap64",
+        "128sq_cap_2team", "512sq_cap_2team", "128sq_k8_2team", "128sq_cap_plain", "512sq_cap_plain",
+        "128sq_k8_plain"])
 def test_update_parity(kl, orc, cfg, cap, sweep, monkeypatch):
-    # the persistent one-XCD sweeps run whenever the cap allows (the two-team
-    # sweep for caps up to 1024, the master/helper sweep above); the
-    # two-kernel sweep is the one for no cap, or when forced
+    # the persistent one-XCD sweeps run whenever the cap allows (the
+    # eight-team sweep for caps up to 1024 and patches up to 144, speculating
+    # on each atom's first exchange, or its plain path with
+    # KSVD_SWEEP_SPEC=0; the two-team sweep with KSVD_SWEEP_TEAMS=2; the
+    # master/helper sweep above 1024); the two-kernel sweep is the one for no
+    # cap, or when forced
     monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1" if sweep == "two_kernel" else "0")
+    monkeypatch.setenv("KSVD_SWEEP_TEAMS", "2" if sweep == "two_team" else "8")
+    monkeypatch.setenv("KSVD_SWEEP_SPEC", "0" if sweep == "plain" else "1")
     clean, noisy, ks = make(kl, cfg, user_cap=cap)
     ks.extract()
     ks.init_dictionary()
