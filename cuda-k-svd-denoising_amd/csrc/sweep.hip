@@ -1954,25 +1954,29 @@ __device__ __forceinline__ void et_build(const SweepArgs &a, float (&E)[NR][ET_C
     }
 #pragma unroll 1
     for (int e = 0; e < NO; e++) {
-        float xv[ET_CB], dv[ET_CB][NR];
+        // two column slots at a time (their loads in flight together)
 #pragma unroll
-        for (int j = 0; j < ET_CB; j++) {
-            const int tl = ET_QL * j + q;
-            const int tc = min(tl, max(nc - 1, 0));
-            const bool k1 = tl < nc && T.ek[tc * NO + e] == 1;
-            const float x = a.coef[k1 ? T.eci[tc * NO + e] : 0];
-            xv[j] = k1 ? x : 0.f;
-            const float *dr = a.D + (int64_t)(k1 ? T.eat[tc * NO + e] : 0) * dim;
+        for (int j0 = 0; j0 < ET_CB; j0 += 2) {
+            float xv[2], dv[2][NR];
 #pragma unroll
-            for (int i = 0; i < NR; i++) {
-                const float d = dr[min(rb + 16 * i, dim - 1)];
-                dv[j][i] = rb + 16 * i < dim ? d : 0.f;
+            for (int jj = 0; jj < 2; jj++) {
+                const int tl = ET_QL * (j0 + jj) + q;
+                const int tc = min(tl, max(nc - 1, 0));
+                const bool k1 = tl < nc && T.ek[tc * NO + e] == 1;
+                const float x = a.coef[k1 ? T.eci[tc * NO + e] : 0];
+                xv[jj] = k1 ? x : 0.f;
+                const float *dr = a.D + (int64_t)(k1 ? T.eat[tc * NO + e] : 0) * dim;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const float d = dr[min(rb + 16 * i, dim - 1)];
+                    dv[jj][i] = rb + 16 * i < dim ? d : 0.f;
+                }
             }
+#pragma unroll
+            for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+                for (int i = 0; i < NR; i++) E[i][j0 + jj] = fmaf(-dv[jj][i], xv[jj], E[i][j0 + jj]);
         }
-#pragma unroll
-        for (int j = 0; j < ET_CB; j++)
-#pragma unroll
-            for (int i = 0; i < NR; i++) E[i][j] = fmaf(-dv[j][i], xv[j], E[i][j]);
     }
 }
 
@@ -2060,7 +2064,7 @@ __device__ __forceinline__ void et_member(const SweepArgs &a, unsigned *flags, i
 {
     constexpr int RP = 16 * NR;      // row pitch of the LDS vectors
     constexpr int HCH = et_hch(NR);  // listed columns staged (the plain path's hand-over chunk too)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int rb = lane & 15, q = wave * 4 + (lane >> 4);
     const int K = a.K, cap = a.cap;
     int dim = a.dim, dpad = a.dpad;
@@ -2241,7 +2245,8 @@ __device__ __forceinline__ void et_member(const SweepArgs &a, unsigned *flags, i
         // atom, so that the addresses derived from them are formed where they
         // are used instead of being held (and spilled) across the loop
         asm volatile("" : "+s"(dim), "+s"(dpad));
-        asm volatile("" : "+v"(rb), "+v"(q));
+        asm volatile("" : "+v"(rb), "+v"(q), "+v"(tid), "+v"(lane));
+        asm volatile("" : "+s"(wave));
         const int m = pa_m(suo, k, cap);
         float *Ecur = (k & 1) ? a.E1 : a.E0;
         const float *Eprev = (k & 1) ? a.E0 : a.E1;
