@@ -24,7 +24,7 @@ tests/test_gpu_bf16.py and tests/test_gpu_e2e.py).  `headline_form` and
 sample of patches.
 
 `roofline` is for the kernel that dominates the step: the persistent
-dictionary sweep (pt_sweep_kernel, the two-team sweep, one launch per
+dictionary sweep (et_sweep_kernel, the eight-team sweep, one launch per
 iteration).  Its
 algorithmic bytes are those of the reference's update for the capped users
 of every atom (re-read each user's patch and code, write its new
@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (91771.2 * 2 + 11629.7) * 1024, "omp": (131707.4 * 2 + 19818.2) * 1024},
+    "2048": {"sweep": (112745.7 * 2 + 42044.3) * 1024, "omp": (131707.4 * 2 + 19818.2) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
@@ -242,8 +242,8 @@ def run_batch(K, specs, inputs, iters, group=8):
     rebuild; speckle's exp2 on the host).  Pictures of one patch size go in
     groups of up to `group` (one per XCD): each picture on its own context,
     their iterations together (ksvd_run_batch: each picture's pursuit on its
-    own stream, one dictionary sweep launch for the group where the two-team
-    sweep applies, one picture after another otherwise).  Contexts are reused
+    own stream, one dictionary sweep launch for the group where a persistent
+    one-XCD sweep applies, one picture after another otherwise).  Contexts are reused
     from group to group.  Returns (seconds, patch-iterations)."""
     pools = {}
     pi = 0
@@ -433,7 +433,7 @@ def main():
         "plain_fp32": {"value": round(plain["rate"], 1), "ms_per_step": round(plain["T"] * 1e3 / args.steps, 4),
                        "images_per_s": round(plain["ips"], 3), "psnr_raw": round(plain["psnr_raw"], 4),
                        "psnr_normalized": round(plain["psnr_norm"], 4)},
-        "roofline": {"bound": "hbm", "kernel": "dictionary update of the plain form: pt_sweep_kernel (two-team persistent "
+        "roofline": {"bound": "hbm", "kernel": "dictionary update of the plain form: et_sweep_kernel (eight-team persistent "
                                "sweep) + its users / column-table / cap-zeroing kernels",
                      "achieved": round(sweep_gbs, 3), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(sweep_gbs / PEAK_HBM_GBS, 6),

@@ -300,12 +300,16 @@ def test_one_sweep_2048(big, orc):
 
 
 @pytest.mark.parametrize("k,cap,sweep", [(5, 1024, "persistent"), (5, 0, "two_kernel"), (8, 1536, "persistent"),
-                                         (5, 64, "persistent")],
-                         ids=["k5_cap1024", "k5_nocap", "k8_cap1536_master", "k5_cap64"])
+                                         (5, 64, "persistent"), (5, 1024, "two_team"), (5, 1024, "plain")],
+                         ids=["k5_cap1024", "k5_nocap", "k8_cap1536_master", "k5_cap64", "k5_cap1024_2team",
+                              "k5_cap1024_plain"])
 def test_sweep_every_atom_512(kl, orc, k, cap, sweep, monkeypatch):
     """The gap-independent check at 512x512 on the sweeps that the cap and
-    OMP limit select (two-team, master/helper, two-kernel)."""
+    OMP limit select (eight-team, master/helper, two-kernel), the eight-team
+    sweep's plain path (no speculation) and the two-team sweep."""
     monkeypatch.setenv("KSVD_SWEEP_TWO_KERNEL", "1" if sweep == "two_kernel" else "0")
+    monkeypatch.setenv("KSVD_SWEEP_TEAMS", "2" if sweep == "two_team" else "8")
+    monkeypatch.setenv("KSVD_SWEEP_SPEC", "0" if sweep == "plain" else "1")
     cfg = dict(CFG2, k=k)
     clean, noisy = kl.synth_image(cfg["seed"], cfg["W"], cfg["H"], cfg["sigma"])
     ks = kl.KSvd(patch=cfg["patch"], slide=cfg["slide"], atoms=cfg["atoms"], omp_iter=k, user_cap=cap)

@@ -251,7 +251,7 @@ def test_update_parity(kl, orc, cfg, cap, sweep, monkeypatch):
 
 
 def test_cap_clearing_before_reuse(kl, orc):
-    """The two-team sweep clears the coefficients of the users beyond the cap
+    """The persistent sweeps clear the coefficients of the users beyond the cap
     in a pass that runs only before the codes are read again.  Two updates in
     a row (the second reads the first's codes through its users lists) must
     give the bits of an update, a read-back of the codes, and an update from
@@ -574,7 +574,7 @@ def test_geometry_modes(kl, orc, case):
 def speckle_frame(kl, W=96, H=80):
     """Log-domain luma of a colour picture with multiplicative speckle: a
     dictionary sweep whose results once depended on where a context's buffers
-    sat (stale L1 lines in the two-team sweep)."""
+    sat (stale L1 lines in the persistent sweep)."""
     rng = np.random.default_rng(5)
     clean = np.stack([kl.synth_image(s, W, H, 0.0)[0] for s in (11, 12, 13)], axis=-1) + 10.0
     noisy = (clean * rng.gamma(4.0, 0.25, size=clean.shape)).astype(np.float32)
