@@ -1,16 +1,18 @@
 // Batched Orthogonal Matching Pursuit on gfx950.
 //
-// omp_mfma_kernel: persistent, one workgroup per CU, four wavefronts.  The
-// whole dictionary (K x dim FP32, atom-major, row pitch LD = 4Q + 4 floats so
-// that the 16-byte LDS reads of one wavefront hit distinct banks) sits in LDS
-// for the kernel's lifetime.  Each wavefront owns a tile of 16 patches:
+// omp_mfma_kernel: persistent, one workgroup per CU, eight wavefronts (two
+// per SIMD; four with OMP limits above 5).  The whole dictionary (K x dim
+// FP32, atom-major, row pitch LD = 4Q + 4 floats so that the 16-byte LDS
+// reads of one wavefront hit distinct banks) sits in LDS for the kernel's
+// lifetime.  Each wavefront owns a tile of 16 patches at a time:
 //   * lane (col, g), col = lane % 16, g = lane / 16, holds quarter g of patch
 //     col: elements g*Q .. g*Q+Q-1 of y and of the residual r;
 //   * step 0's correlation C0 (K x 16) = D^T Y is v_mfma_f32_16x16x4_f32
 //     with the reduction index permuted so that MFMA number kk takes element
 //     g*Q + kk from lane group g (A from LDS, B straight from y's registers);
 //     later steps take D^T r = C0 - G[:, S] x from the dictionary Gram
-//     (Batch-OMP), the rows of the support atoms read from L2;
+//     (Batch-OMP), the rows of the support atoms read from L2 a few blocks
+//     of atoms at a time, each block folded into the argmax at once;
 //   * |C| argmax per patch: in-lane scan in ascending atom order, then two
 //     cross-lane steps with the lowest index winning ties (cublasIsamax);
 //   * the support Gram grows by one Cholesky row per step (entries from the
@@ -31,7 +33,11 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int OMP_WAVES = 4;
+// Wavefronts per workgroup (one workgroup per CU, the dictionary in its LDS):
+// two per SIMD where the support's registers leave room (OMP limit <= 5), so
+// one wavefront's MFMA correlation and L2 waits overlap the other's pursuit
+// steps; one per SIMD with the 8 x 8 Cholesky factor.
+constexpr int omp_waves(int km) { return km <= 5 ? 8 : 4; }
 
 // v + (v at lane ^ 16) and v + (v at lane ^ 32) on the VALU: the permlane
 // swaps of a value with itself return, in their two results, every lane's
@@ -56,8 +62,19 @@ __device__ __forceinline__ float patch_sum(float v)
     return add_xor32(add_xor16(v));
 }
 
-template <int Q, int KT, int KM>
-__global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
+// Folds four correlations of atoms a0 .. a0+3 into a running first-index
+// argmax of |c| (ascending atom order within the lane).
+__device__ __forceinline__ void amax_fold(const f4 &c, int a0, float &best, int &bi)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float v = fabsf(c[i]);
+        if (v > best) { best = v; bi = a0 + i; }
+    }
+}
+
+template <int Q, int KT, int KM, int W>
+__global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
     const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
     const float *__restrict__ G, int kmax, int ntiles, const int *__restrict__ plist,
     const unsigned *__restrict__ plist_n, int *__restrict__ idx_out, float *__restrict__ coef_out,
@@ -65,6 +82,8 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
 {
     constexpr int LD = 4 * Q + 4;
     constexpr int KP = 16 * KT;
+    constexpr int AH = W > 4 ? 4 : KT;  // blocks of A read from LDS at a time
+    static_assert(W % 4 == 0, "the tile order takes whole sets of four wavefronts");
     extern __shared__ __attribute__((aligned(16))) float Ds[];  // KP x LD, zero beyond (K, dim)
     for (int q = threadIdx.x; q < KP * LD; q += blockDim.x) {
         const int a = q / LD, e = q - a * LD;
@@ -72,15 +91,26 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
     }
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, g = lane >> 4;
-    const int e0 = g * Q;
     unsigned long long ndup = 0, ndeg = 0;
 
     // list mode: the tiles cover plist[0 .. *plist_n)
     const int64_t nl = plist ? (int64_t)*plist_n : 0;
     const int ntl = plist ? (int)((nl + 15) / 16) : ntiles;
-    for (int tile = blockIdx.x * OMP_WAVES + wave; tile < ntl; tile += gridDim.x * OMP_WAVES) {
+    // Tiles go round robin over the launch's wavefronts in an order that
+    // takes one wavefront of every SIMD (wavefront w of a workgroup runs on
+    // SIMD w % 4) before the second wavefront of any: the last, partial
+    // round then adds at most one tile to any SIMD.
+    const int wave = threadIdx.x >> 6;
+    const int slot0 = (wave / 4) * (gridDim.x * 4) + blockIdx.x * 4 + (wave & 3);
+    for (int tile = slot0; tile < ntl; tile += gridDim.x * W) {
+        // The lane's coordinates pass through an opaque copy per tile, so
+        // the LDS addresses and bounds masks derived from them are formed
+        // where they are used instead of being hoisted out of the tile loop
+        // and held (or spilled) across it.
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));
+        const int col = lane & 15, g = lane >> 4;
+        const int e0 = g * Q;
         const int64_t slot = (int64_t)tile * 16 + col;
         const int64_t p = plist ? (slot < nl ? (int64_t)plist[slot] : P) : slot;
         const bool valid = p < P;
@@ -116,70 +146,95 @@ __global__ __launch_bounds__(64 * OMP_WAVES, 1) void omp_mfma_kernel(
         f4 c0[KT];  // D^T y (step 0's correlation)
         for (int t = 0; t < kmax; t++) {
             if (__ballot(active) == 0) break;
-            f4 acc[KT];
+            // again per step: nothing derived from the coordinates stays
+            // live across the steps
+            int lane = threadIdx.x & 63;
+            asm volatile("" : "+v"(lane));
+            const int col = lane & 15, g = lane >> 4;
+            const int e0 = g * Q;
+            float best = -1.f;
+            int bi = 0x7fffffff;
             if (t == 0) {
                 // ---- correlation C = D^T Y on MFMA ----------------------------
 #pragma unroll
-                for (int tt = 0; tt < KT; tt++) acc[tt] = f4{0.f, 0.f, 0.f, 0.f};
+                for (int tt = 0; tt < KT; tt++) c0[tt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kb = 0; kb < Q; kb += 4) {
-                    f4 av[KT];
+                for (int kb = 0; kb < Q; kb += 4)
 #pragma unroll
-                    for (int tt = 0; tt < KT; tt++)
-                        av[tt] = *(const f4 *)(Ds + (tt * 16 + col) * LD + e0 + kb);
+                    for (int th = 0; th < KT; th += AH) {
+                        f4 av[AH];
 #pragma unroll
-                    for (int sub = 0; sub < 4; sub++)
+                        for (int tt = 0; tt < AH; tt++)
+                            av[tt] = *(const f4 *)(Ds + ((th + tt) * 16 + col) * LD + e0 + kb);
 #pragma unroll
-                        for (int tt = 0; tt < KT; tt++)
-                            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], y[kb + sub], acc[tt], 0, 0, 0);
-                }
+                        for (int sub = 0; sub < 4; sub++)
 #pragma unroll
-                for (int tt = 0; tt < KT; tt++) c0[tt] = acc[tt];
+                            for (int tt = 0; tt < AH; tt++)
+                                c0[th + tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[tt][sub], y[kb + sub], c0[th + tt], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++) amax_fold(c0[tt], tt * 16 + 4 * g, best, bi);
             } else {
                 // ---- correlation from the Gram: D^T r = D^T y - G[:, S] x ------
                 // (Batch-OMP; the lane's atoms tt*16 + 4g .. +3, rows S_q of
-                // the symmetric G, support entries in order)
+                // the symmetric G, support entries in order).  CH blocks of
+                // atoms at a time: every support row's entries of the blocks
+                // are loaded together, corrected and folded into the argmax,
+                // so no full copy of the correlations is held beside C0.
+                // A step t < kmax <= KM corrects by at most KM - 1 rows.
+                constexpr int NQ = KM - 1;
+                int smax = 0;  // the wavefront's largest support
 #pragma unroll
-                for (int tt = 0; tt < KT; tt++) acc[tt] = c0[tt];
+                for (int q = 0; q < NQ; q++) smax += __ballot(q < s) != 0ull;
+                float xq[NQ];
+                const float *gr[NQ];
 #pragma unroll
-                for (int q = 0; q < KM; q++) {
-                    if (__ballot(q < s) == 0ull) break;
-                    const float xq = q < s ? x[q] : 0.f;
-                    const float *gr = G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g;
+                for (int q = 0; q < NQ; q++) {
+                    xq[q] = q < s ? x[q] : 0.f;
+                    gr[q] = G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g;
+                }
+                constexpr int CH = 2;
+#pragma unroll
+                for (int tc = 0; tc < KT; tc += CH) {
+                    f4 a[CH];
+#pragma unroll
+                    for (int c = 0; c < CH; c++) a[c] = c0[tc + c];
                     // row entries of atoms >= K (the zero rows of the LDS
                     // image) are 0, so their correlations stay 0
-                    f4 gv[KT];
-                    if (K == KP) {  // whole rows: no per-lane bounds
+                    f4 gv[NQ][CH];
 #pragma unroll
-                        for (int tt = 0; tt < KT; tt++) gv[tt] = *(const f4 *)(gr + 16 * tt);
-                    } else {
+                    for (int q = 0; q < NQ; q++)
+                        if (q < smax) {
+                            if (K == KP) {  // whole rows: no per-lane bounds
 #pragma unroll
-                        for (int tt = 0; tt < KT; tt++) {
-                            const int a0 = tt * 16 + 4 * g;
-                            if ((K & 3) == 0 && a0 + 3 < K) {
-                                gv[tt] = *(const f4 *)(gr + 16 * tt);
+                                for (int c = 0; c < CH; c++) gv[q][c] = *(const f4 *)(gr[q] + 16 * (tc + c));
                             } else {
 #pragma unroll
-                                for (int i = 0; i < 4; i++) gv[tt][i] = a0 + i < K ? gr[16 * tt + i] : 0.f;
+                                for (int c = 0; c < CH; c++) {
+                                    const int a0 = (tc + c) * 16 + 4 * g;
+                                    if ((K & 3) == 0 && a0 + 3 < K) {
+                                        gv[q][c] = *(const f4 *)(gr[q] + 16 * (tc + c));
+                                    } else {
+#pragma unroll
+                                        for (int i = 0; i < 4; i++)
+                                            gv[q][c][i] = a0 + i < K ? gr[q][16 * (tc + c) + i] : 0.f;
+                                    }
+                                }
                             }
                         }
-                    }
 #pragma unroll
-                    for (int tt = 0; tt < KT; tt++)
+                    for (int q = 0; q < NQ; q++)
+                        if (q < smax) {
 #pragma unroll
-                        for (int i = 0; i < 4; i++) acc[tt][i] = fmaf(-gv[tt][i], xq, acc[tt][i]);
+                            for (int c = 0; c < CH; c++)
+#pragma unroll
+                                for (int i = 0; i < 4; i++) a[c][i] = fmaf(-gv[q][c][i], xq[q], a[c][i]);
+                        }
+#pragma unroll
+                    for (int c = 0; c < CH; c++) amax_fold(a[c], (tc + c) * 16 + 4 * g, best, bi);
                 }
             }
-            // ---- first index of max |c| -------------------------------------
-            float best = -1.f;
-            int bi = 0x7fffffff;
-#pragma unroll
-            for (int tt = 0; tt < KT; tt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const float v = fabsf(acc[tt][i]);
-                    if (v > best) { best = v; bi = tt * 16 + 4 * g + i; }
-                }
+            // ---- first index of max |c| across the patch's four lanes --------
 #pragma unroll
             for (int off = 16; off <= 32; off <<= 1) {
                 const float ob = __shfl_xor(best, off);
@@ -580,12 +635,13 @@ int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float 
     constexpr int KT = 16;
     constexpr int LD = 4 * Q + 4;
     const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
-    auto kern = omp_mfma_kernel<Q, KT, KM>;
+    constexpr int W = omp_waves(KM);
+    auto kern = omp_mfma_kernel<Q, KT, KM, W>;
     KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
     const int ntiles = (int)((ctx->P + 15) / 16);
-    int grid = (ntiles + OMP_WAVES - 1) / OMP_WAVES;
+    int grid = (ntiles + W - 1) / W;
     if (grid > ctx->num_cus) grid = ctx->num_cus;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * OMP_WAVES), lds, ctx->stream, (const float *)ctx->Y.ptr,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, ctx->stream, (const float *)ctx->Y.ptr,
                        ctx->dim, ctx->P, (const float *)ctx->D.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
                        ntiles, plist, plist_n, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
                        rnorm, (unsigned long long *)ctx->counters.ptr);
@@ -730,9 +786,10 @@ hipError_t noalloc_launch(const float *Y, int dim, int64_t P, const float *D, in
     constexpr int LD = 4 * Q + 4;
     const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
     const int ntiles = (int)((P + 15) / 16);
-    int grid = (ntiles + OMP_WAVES - 1) / OMP_WAVES;
+    constexpr int W = omp_waves(KM);
+    int grid = (ntiles + W - 1) / W;
     if (grid > ncu) grid = ncu;
-    hipLaunchKernelGGL((omp_mfma_kernel<Q, KT, KM>), dim3(grid), dim3(64 * OMP_WAVES), lds, st, Y, dim, P, D, K, G,
+    hipLaunchKernelGGL((omp_mfma_kernel<Q, KT, KM, W>), dim3(grid), dim3(64 * W), lds, st, Y, dim, P, D, K, G,
                        kmax, ntiles, (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt,
                        (float *)nullptr, counters);
     return hipGetLastError();
@@ -750,8 +807,10 @@ int noalloc_setup(int dev, unsigned char **ptr, int *ncu)
         int cus = 0;
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipDeviceGetAttribute", e);
-        const void *kerns[] = {(const void *)omp_mfma_kernel<16, 16, 5>, (const void *)omp_mfma_kernel<36, 16, 5>,
-                               (const void *)omp_mfma_kernel<16, 16, 8>, (const void *)omp_mfma_kernel<36, 16, 8>};
+        const void *kerns[] = {(const void *)omp_mfma_kernel<16, 16, 5, omp_waves(5)>,
+                               (const void *)omp_mfma_kernel<36, 16, 5, omp_waves(5)>,
+                               (const void *)omp_mfma_kernel<16, 16, 8, omp_waves(8)>,
+                               (const void *)omp_mfma_kernel<36, 16, 8, omp_waves(8)>};
         for (const void *k : kerns)
             if ((e = allow_max_lds(k)) != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipFuncSetAttribute", e);
         noalloc_cus[dev] = cus > 0 ? cus : 256;
