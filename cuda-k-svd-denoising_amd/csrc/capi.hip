@@ -244,7 +244,9 @@ int ksvd_create(const ksvd_params *p, ksvd_ctx **out)
     KSVD_HIP_CHECK(ctx, hipMemsetAsync(ctx->counters.ptr, 0, ctx->counters.bytes, ctx->stream));
     rc = ksvd_reserve(ctx, ctx->D, sizeof(float) * (size_t)ctx->dim * ctx->K);
     if (rc) return rc;
-    rc = ksvd_reserve(ctx, ctx->G, sizeof(float) * (size_t)ctx->K * ctx->K);
+    // the dictionary Gram, with 256 floats of slack after it: the pursuit
+    // reads whole 16-byte groups of a row and zeroes the entries past K
+    rc = ksvd_reserve(ctx, ctx->G, sizeof(float) * ((size_t)ctx->K * ctx->K + 256));
     if (rc) return rc;
     rc = ksvd_reserve(ctx, ctx->wbuf, sizeof(float) * (size_t)ctx->dim * ctx->dim);
     if (rc) return rc;

@@ -62,7 +62,7 @@ __device__ __forceinline__ float patch_sum(float v)
     return add_xor32(add_xor16(v));
 }
 
-// Folds four correlations of atoms a0 .. a0+3 into a running first-index
+// Folds four correlations c of atoms a0 .. a0+3 into a running first-index
 // argmax of |c| (ascending atom order within the lane).
 __device__ __forceinline__ void amax_fold(const f4 &c, int a0, float &best, int &bi)
 {
@@ -73,7 +73,62 @@ __device__ __forceinline__ void amax_fold(const f4 &c, int a0, float &best, int 
     }
 }
 
-template <int Q, int KT, int KM, int W>
+// Step t > 0 of the pursuit, for supports of at most NR atoms: D^T r = C0 -
+// G[:, S] x (Batch-OMP; the lane's atoms tt*16 + 4g .. +3, rows S_q of the
+// symmetric G, support entries in order), two blocks of atoms at a time:
+// every row's entries of the blocks are loaded together, corrected and
+// folded into the argmax, so no full copy of the correlations is held beside
+// C0.  All NR rows are read without branches; rows past a lane's support
+// repeat its first row with weight 0, which leaves the sums as they are.
+// FULL: K is the whole 16 KT.  Otherwise the reads run past row ends (into
+// the next row, or the 256 floats of slack after G) and the entries of atoms
+// >= K are replaced by 0, so their correlations stay 0 (zero rows of the LDS
+// image in step 0).
+template <int KT, int NR, bool FULL, int KMX>
+__device__ __forceinline__ void gram_argmax(const f4 (&c0)[KT], const float *__restrict__ G, int K, const int (&S)[KMX],
+                                            const float (&x)[KMX], int s, int g, float &best, int &bi)
+{
+    constexpr int CH = 2;
+    float xq[NR];
+    const float *gr[NR];
+    const int s0 = s > 0 ? S[0] : 0;
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        xq[q] = q < s ? x[q] : 0.f;
+        gr[q] = G + (int64_t)(q < s ? S[q] : s0) * K + 4 * g;
+    }
+#pragma unroll
+    for (int tc = 0; tc < KT; tc += CH) {
+        f4 a[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) a[c] = c0[tc + c];
+        f4 gv[NR][CH];
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                gv[q][c] = *(const f4 *)(gr[q] + 16 * (tc + c));
+                if (!FULL) {
+                    const int a0 = (tc + c) * 16 + 4 * g;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) gv[q][c][i] = a0 + i < K ? gv[q][c][i] : 0.f;
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < NR; q++)
+#pragma unroll
+            for (int c = 0; c < CH; c++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) a[c][i] = fmaf(-gv[q][c][i], xq[q], a[c][i]);
+#pragma unroll
+        for (int c = 0; c < CH; c++) amax_fold(a[c], (tc + c) * 16 + 4 * g, best, bi);
+    }
+}
+
+// Start of row i of a packed lower triangle.
+__host__ __device__ constexpr int tri(int i) { return i * (i + 1) / 2; }
+
+template <int Q, int KT, int KM, int W, bool FULL>
 __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
     const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
     const float *__restrict__ G, int kmax, int ntiles, const int *__restrict__ plist,
@@ -133,12 +188,14 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
         for (int kk = 0; kk < Q; kk++) yy = fmaf(y[kk], y[kk], yy);
         yy = patch_sum(yy);
         const float band = 1e-6f + yy * 0x1p-10f + sqrtf(yy) * 0x1p-8f;
+        // the Cholesky factor's lower triangle, row by row: entry (i, q),
+        // q <= i, at tri(i) + q
         int S[KM];
-        float x[KM], bv[KM], L[KM * KM];
+        float x[KM], bv[KM], L[KM * (KM + 1) / 2];
 #pragma unroll
         for (int i = 0; i < KM; i++) { S[i] = -1; x[i] = 0.f; bv[i] = 0.f; }
 #pragma unroll
-        for (int i = 0; i < KM * KM; i++) L[i] = 0.f;
+        for (int i = 0; i < KM * (KM + 1) / 2; i++) L[i] = 0.f;
         int s = 0;
         bool active = valid;
         float rlast = yy;  // squared residual norm of the current support
@@ -176,63 +233,27 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                 for (int tt = 0; tt < KT; tt++) amax_fold(c0[tt], tt * 16 + 4 * g, best, bi);
             } else {
                 // ---- correlation from the Gram: D^T r = D^T y - G[:, S] x ------
-                // (Batch-OMP; the lane's atoms tt*16 + 4g .. +3, rows S_q of
-                // the symmetric G, support entries in order).  CH blocks of
-                // atoms at a time: every support row's entries of the blocks
-                // are loaded together, corrected and folded into the argmax,
-                // so no full copy of the correlations is held beside C0.
-                // A step t < kmax <= KM corrects by at most KM - 1 rows.
-                constexpr int NQ = KM - 1;
-                int smax = 0;  // the wavefront's largest support
+                // A step t < kmax <= KM corrects by at most KM - 1 rows: two
+                // code paths, one for the first two steps (at most two rows)
+                // and one for the rest.
+                if (t <= 2) gram_argmax<KT, 2, FULL>(c0, G, K, S, x, s, g, best, bi);
+                else gram_argmax<KT, KM - 1, FULL>(c0, G, K, S, x, s, g, best, bi);
+            }
+            // y again from memory for b_j below (it is not kept in registers
+            // past step 0's correlation), its loads in flight during the
+            // cross-lane argmax
+            float yq[Q];
 #pragma unroll
-                for (int q = 0; q < NQ; q++) smax += __ballot(q < s) != 0ull;
-                float xq[NQ];
-                const float *gr[NQ];
+            for (int kb = 0; kb < Q; kb += 4) {
+                f4 yv;
+                if (valid && e0 + kb + 3 < dim) {
+                    yv = *(const f4 *)(Y + p * dim + e0 + kb);
+                } else {
 #pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    xq[q] = q < s ? x[q] : 0.f;
-                    gr[q] = G + (int64_t)(q < s ? S[q] : 0) * K + 4 * g;
+                    for (int c = 0; c < 4; c++) yv[c] = (valid && e0 + kb + c < dim) ? Y[p * dim + e0 + kb + c] : 0.f;
                 }
-                constexpr int CH = 2;
 #pragma unroll
-                for (int tc = 0; tc < KT; tc += CH) {
-                    f4 a[CH];
-#pragma unroll
-                    for (int c = 0; c < CH; c++) a[c] = c0[tc + c];
-                    // row entries of atoms >= K (the zero rows of the LDS
-                    // image) are 0, so their correlations stay 0
-                    f4 gv[NQ][CH];
-#pragma unroll
-                    for (int q = 0; q < NQ; q++)
-                        if (q < smax) {
-                            if (K == KP) {  // whole rows: no per-lane bounds
-#pragma unroll
-                                for (int c = 0; c < CH; c++) gv[q][c] = *(const f4 *)(gr[q] + 16 * (tc + c));
-                            } else {
-#pragma unroll
-                                for (int c = 0; c < CH; c++) {
-                                    const int a0 = (tc + c) * 16 + 4 * g;
-                                    if ((K & 3) == 0 && a0 + 3 < K) {
-                                        gv[q][c] = *(const f4 *)(gr[q] + 16 * (tc + c));
-                                    } else {
-#pragma unroll
-                                        for (int i = 0; i < 4; i++)
-                                            gv[q][c][i] = a0 + i < K ? gr[q][16 * (tc + c) + i] : 0.f;
-                                    }
-                                }
-                            }
-                        }
-#pragma unroll
-                    for (int q = 0; q < NQ; q++)
-                        if (q < smax) {
-#pragma unroll
-                            for (int c = 0; c < CH; c++)
-#pragma unroll
-                                for (int i = 0; i < 4; i++) a[c][i] = fmaf(-gv[q][c][i], xq[q], a[c][i]);
-                        }
-#pragma unroll
-                    for (int c = 0; c < CH; c++) amax_fold(a[c], (tc + c) * 16 + 4 * g, best, bi);
-                }
+                for (int c = 0; c < 4; c++) yq[kb + c] = yv[c];
             }
             // ---- first index of max |c| across the patch's four lanes --------
 #pragma unroll
@@ -249,10 +270,10 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 #pragma unroll
                 for (int kb = 0; kb < Q; kb += 4) {
                     const f4 dv = *(const f4 *)(dj + kb);
-                    bp = fmaf(dv[0], y[kb + 0], bp);
-                    bp = fmaf(dv[1], y[kb + 1], bp);
-                    bp = fmaf(dv[2], y[kb + 2], bp);
-                    bp = fmaf(dv[3], y[kb + 3], bp);
+                    bp = fmaf(dv[0], yq[kb + 0], bp);
+                    bp = fmaf(dv[1], yq[kb + 1], bp);
+                    bp = fmaf(dv[2], yq[kb + 2], bp);
+                    bp = fmaf(dv[3], yq[kb + 3], bp);
                 }
             }
             bp = patch_sum(bp);
@@ -283,8 +304,8 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                     if (i < s) {
                         float gi = gS[i];
 #pragma unroll
-                        for (int q = 0; q < i; q++) gi = fmaf(-L[i * KM + q], w[q], gi);
-                        w[i] = gi / L[i * KM + i];
+                        for (int q = 0; q < i; q++) gi = fmaf(-L[tri(i) + q], w[q], gi);
+                        w[i] = gi / L[tri(i) + i];
                         piv = fmaf(-w[i], w[i], piv);
                     }
                 }
@@ -297,8 +318,8 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                     for (int i = 0; i < KM; i++)
                         if (i == s) {
 #pragma unroll
-                            for (int q = 0; q < i; q++) L[i * KM + q] = w[q];
-                            L[i * KM + i] = lss;
+                            for (int q = 0; q < i; q++) L[tri(i) + q] = w[q];
+                            L[tri(i) + i] = lss;
                             S[i] = j;
                             bv[i] = bp;
                         }
@@ -310,8 +331,8 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                         if (i < s) {
                             float v = bv[i];
 #pragma unroll
-                            for (int q = 0; q < i; q++) v = fmaf(-L[i * KM + q], z[q], v);
-                            z[i] = v / L[i * KM + i];
+                            for (int q = 0; q < i; q++) v = fmaf(-L[tri(i) + q], z[q], v);
+                            z[i] = v / L[tri(i) + i];
                         }
                     }
 #pragma unroll
@@ -320,8 +341,8 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                             float v = z[i];
 #pragma unroll
                             for (int q = i + 1; q < KM; q++)
-                                if (q < s) v = fmaf(-L[q * KM + i], x[q], v);
-                            x[i] = v / L[i * KM + i];
+                                if (q < s) v = fmaf(-L[tri(q) + i], x[q], v);
+                            x[i] = v / L[tri(i) + i];
                         }
                     }
                 }
@@ -341,8 +362,17 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
             if (__ballot(near) != 0ull) {
                 float ne = 0.f;
                 if (near) {
+                    // y again from memory: it is not kept in registers
+                    // past step 0's correlation
 #pragma unroll
                     for (int kb = 0; kb < Q; kb += 4) {
+                        f4 yv;
+                        if (e0 + kb + 3 < dim) {
+                            yv = *(const f4 *)(Y + p * dim + e0 + kb);
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < 4; c++) yv[c] = e0 + kb + c < dim ? Y[p * dim + e0 + kb + c] : 0.f;
+                        }
                         f4 a4 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                         for (int i = 0; i < KM; i++)
@@ -355,7 +385,7 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                             }
 #pragma unroll
                         for (int c4 = 0; c4 < 4; c4++) {
-                            const float rv = y[kb + c4] - a4[c4];
+                            const float rv = yv[c4] - a4[c4];
                             ne = fmaf(rv, rv, ne);
                         }
                     }
@@ -636,7 +666,8 @@ int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float 
     constexpr int LD = 4 * Q + 4;
     const size_t lds = sizeof(float) * (size_t)(16 * KT) * LD;
     constexpr int W = omp_waves(KM);
-    auto kern = omp_mfma_kernel<Q, KT, KM, W>;
+    // whole Gram rows (no per-lane bounds) when K fills the 256-atom tile
+    auto kern = ctx->K == 16 * KT ? omp_mfma_kernel<Q, KT, KM, W, true> : omp_mfma_kernel<Q, KT, KM, W, false>;
     KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
     const int ntiles = (int)((ctx->P + 15) / 16);
     int grid = (ntiles + W - 1) / W;
@@ -789,9 +820,9 @@ hipError_t noalloc_launch(const float *Y, int dim, int64_t P, const float *D, in
     constexpr int W = omp_waves(KM);
     int grid = (ntiles + W - 1) / W;
     if (grid > ncu) grid = ncu;
-    hipLaunchKernelGGL((omp_mfma_kernel<Q, KT, KM, W>), dim3(grid), dim3(64 * W), lds, st, Y, dim, P, D, K, G,
-                       kmax, ntiles, (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt,
-                       (float *)nullptr, counters);
+    auto kern = K == 16 * KT ? omp_mfma_kernel<Q, KT, KM, W, true> : omp_mfma_kernel<Q, KT, KM, W, false>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, st, Y, dim, P, D, K, G, kmax, ntiles,
+                       (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt, (float *)nullptr, counters);
     return hipGetLastError();
 }
 
@@ -807,10 +838,14 @@ int noalloc_setup(int dev, unsigned char **ptr, int *ncu)
         int cus = 0;
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipDeviceGetAttribute", e);
-        const void *kerns[] = {(const void *)omp_mfma_kernel<16, 16, 5, omp_waves(5)>,
-                               (const void *)omp_mfma_kernel<36, 16, 5, omp_waves(5)>,
-                               (const void *)omp_mfma_kernel<16, 16, 8, omp_waves(8)>,
-                               (const void *)omp_mfma_kernel<36, 16, 8, omp_waves(8)>};
+        const void *kerns[] = {(const void *)omp_mfma_kernel<16, 16, 5, omp_waves(5), true>,
+                               (const void *)omp_mfma_kernel<36, 16, 5, omp_waves(5), true>,
+                               (const void *)omp_mfma_kernel<16, 16, 8, omp_waves(8), true>,
+                               (const void *)omp_mfma_kernel<36, 16, 8, omp_waves(8), true>,
+                               (const void *)omp_mfma_kernel<16, 16, 5, omp_waves(5), false>,
+                               (const void *)omp_mfma_kernel<36, 16, 5, omp_waves(5), false>,
+                               (const void *)omp_mfma_kernel<16, 16, 8, omp_waves(8), false>,
+                               (const void *)omp_mfma_kernel<36, 16, 8, omp_waves(8), false>};
         for (const void *k : kerns)
             if ((e = allow_max_lds(k)) != hipSuccess) return noalloc_fail(KSVD_ERR_HIP, "hipFuncSetAttribute", e);
         noalloc_cus[dev] = cus > 0 ? cus : 256;
