@@ -239,22 +239,6 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                 if (t <= 2) gram_argmax<KT, 2, FULL>(c0, G, K, S, x, s, g, best, bi);
                 else gram_argmax<KT, KM - 1, FULL>(c0, G, K, S, x, s, g, best, bi);
             }
-            // y again from memory for b_j below (it is not kept in registers
-            // past step 0's correlation), its loads in flight during the
-            // cross-lane argmax
-            float yq[Q];
-#pragma unroll
-            for (int kb = 0; kb < Q; kb += 4) {
-                f4 yv;
-                if (valid && e0 + kb + 3 < dim) {
-                    yv = *(const f4 *)(Y + p * dim + e0 + kb);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 4; c++) yv[c] = (valid && e0 + kb + c < dim) ? Y[p * dim + e0 + kb + c] : 0.f;
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++) yq[kb + c] = yv[c];
-            }
             // ---- first index of max |c| across the patch's four lanes --------
 #pragma unroll
             for (int off = 16; off <= 32; off <<= 1) {
@@ -270,10 +254,10 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 #pragma unroll
                 for (int kb = 0; kb < Q; kb += 4) {
                     const f4 dv = *(const f4 *)(dj + kb);
-                    bp = fmaf(dv[0], yq[kb + 0], bp);
-                    bp = fmaf(dv[1], yq[kb + 1], bp);
-                    bp = fmaf(dv[2], yq[kb + 2], bp);
-                    bp = fmaf(dv[3], yq[kb + 3], bp);
+                    bp = fmaf(dv[0], y[kb + 0], bp);
+                    bp = fmaf(dv[1], y[kb + 1], bp);
+                    bp = fmaf(dv[2], y[kb + 2], bp);
+                    bp = fmaf(dv[3], y[kb + 3], bp);
                 }
             }
             bp = patch_sum(bp);
@@ -362,17 +346,8 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
             if (__ballot(near) != 0ull) {
                 float ne = 0.f;
                 if (near) {
-                    // y again from memory: it is not kept in registers
-                    // past step 0's correlation
 #pragma unroll
                     for (int kb = 0; kb < Q; kb += 4) {
-                        f4 yv;
-                        if (e0 + kb + 3 < dim) {
-                            yv = *(const f4 *)(Y + p * dim + e0 + kb);
-                        } else {
-#pragma unroll
-                            for (int c = 0; c < 4; c++) yv[c] = e0 + kb + c < dim ? Y[p * dim + e0 + kb + c] : 0.f;
-                        }
                         f4 a4 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                         for (int i = 0; i < KM; i++)
@@ -385,7 +360,7 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                             }
 #pragma unroll
                         for (int c4 = 0; c4 < 4; c4++) {
-                            const float rv = yv[c4] - a4[c4];
+                            const float rv = y[kb + c4] - a4[c4];
                             ne = fmaf(rv, rv, ne);
                         }
                     }
