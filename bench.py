@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 # a device copy of known size (FETCH x 2, WRITE x 1):
 # profiles/hbm_traffic_ksvd_2048sq.md.
 MEASURED_TRAFFIC = {
-    "2048": {"sweep": (112745.7 * 2 + 42044.3) * 1024, "omp": (131707.4 * 2 + 19818.2) * 1024},
+    "2048": {"sweep": (112745.7 * 2 + 42044.3) * 1024, "omp": (131621.6 * 2 + 20223.8) * 1024},
 }
 
 def omp_flops_per_patch(n, K, k):
