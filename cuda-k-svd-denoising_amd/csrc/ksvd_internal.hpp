@@ -78,7 +78,7 @@ struct ksvd_ctx {
     DevBuf mlp_h1c, mlp_h2c, mlp_zc, mlp_fb, mlp_cnt;        // proposal chunk, fallback list and its length
     DevBuf mlp_rej, mlp_bcnt;                                // per-patch fallback flags, per-block counts
     DevBuf multi;                                            // a batch's pictures for the multi-picture sweep
-    int multi_mode = 0;                                      // of that batch: 1 two-team sweeps, 2 large patches, 3 eight-team sweeps
+    int multi_mode = 0;                                      // of that batch: 1 two-team sweeps, 2 large patches, 3 eight-team sweeps, 4 one-launch large patches
     bool mlp_ready = false, mlp_x_valid = false, mlp_trained = false;
     bool mlp_gated = false;  // the proposer stood down for this dictionary (acceptance below break-even)
     // The two-team sweep leaves the coefficients of the users beyond the cap

@@ -2651,7 +2651,8 @@ __global__ void ps_reset_kernel(unsigned *flags, unsigned long long *wx, int nw,
 // ===========================================================================
 // Large patches (padded dim above 192, up to 1024): three launches per atom,
 // E_k and its Gram in global memory, for the patch sizes the LDS-resident
-// sweeps cannot hold (22 x 22 patches: dim 484).
+// sweeps cannot hold where the one-launch large-patch sweep below does not
+// apply (dim above 512, caps above 1024 or none, KSVD_SWEEP_KL=1).
 //   kl_build_kernel(k), one pass of many workgroups: E_k's capped columns,
 //     e_t = y_p - sum_{j != k} d_j x_{j,p} in the column table's entry order,
 //     every coefficient read from memory (atom k-1's were written, final, by
@@ -3075,6 +3076,286 @@ __global__ __launch_bounds__(KQ_T) void kq_power_multi(const KlPicture *__restri
     kq_power_body(p->a, p->G, gp, nsplit, p->E, k, p->flags, p->wx, s_role, s_bail, gs, u, red);
 }
 
+// ===========================================================================
+// Large patches in one launch (padded dim above 192, up to 512; caps up to
+// 1024): the KX_R workgroups of one XCD form one team that takes the atoms in
+// order.  Role r holds its share of atom k's capped columns (at most KX_CW) in
+// LDS, built from the column table as kl_build_kernel builds them (same
+// arithmetic, same bits), and the power iteration runs on E_k itself, with no
+// Gram: each iteration every role forms its partial z_r = E_r (E_r^T u) (the
+// column dots lane-strided with a fixed butterfly, then each row summed over
+// the role's columns in order) and stores it as tagged pairs, tag = (k << 12)
+// | iteration, into one of two buffers by the iteration's parity; every role
+// reads all KX_R partials of each row and adds them in role order, so every
+// role holds the same z, u and stop decision bit for bit.  A role reads
+// iteration i's pairs before it stores those of i + 1, and the roles meet
+// between atoms, so no pair is overwritten while it may still be read.  When
+// the iteration stops, role 0 writes d_k = u, every role writes its columns'
+// coefficients x_t = e_t . d_k (kl_dot's arithmetic) and zeroes its share of
+// the users beyond the cap, then publishes its done flag; atom k+1's columns
+// are built once every role is done with atom k (they read atom k's vector
+// and coefficients).  Waits give up as the other persistent sweeps' do.
+constexpr int KX_R = 32;     // roles: one XCD's CUs
+constexpr int KX_T = 1024;   // threads per role
+constexpr int KX_CW = 32;    // columns per role at most (caps up to KX_R * KX_CW)
+constexpr int KX_NO = 8;     // other support entries per column staged in LDS (ompIter <= 9)
+constexpr int KX_B = 8;      // columns whose loads are in flight together in the build
+
+// LDS of one role: its columns (pitch ep = dim rounded up to 4), then the
+// column table entries and coefficients of the build.
+__host__ __device__ constexpr int kx_ep(int dim) { return (dim + 3) / 4 * 4; }
+__host__ __device__ constexpr size_t kx_lds_bytes(int dim)
+{
+    return sizeof(float) * (size_t)KX_CW * kx_ep(dim) + sizeof(int) * KX_CW * (2 + KX_NO) + sizeof(float) * KX_CW * KX_NO;
+}
+
+__device__ void kx_team(const SweepArgs &a, unsigned *flags, unsigned long long *wx, int role, float *sm, int *bail)
+{
+    int tid = threadIdx.x, lane = tid & 63;
+    int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int K = a.K, cap = a.cap, kmax = a.kmax;
+    int dim = a.dim, dpad = a.dpad;
+    const int no = kmax > 1 ? kmax - 1 : 1;
+    int ep = kx_ep(dim);
+    float *Es = sm;                                // KX_CW x ep: this role's columns
+    int *sp = (int *)(Es + (size_t)KX_CW * ep);    // KX_CW: patch of each column
+    int *sself = sp + KX_CW;                       // KX_CW: the column's own coefficient slot
+    int *sat = sself + KX_CW;                      // KX_CW x KX_NO: other atoms of the column (-1: none)
+    float *sx = (float *)(sat + KX_CW * KX_NO);    // KX_CW x KX_NO: their coefficients
+    __shared__ float u[KSVD_MAX_DIM];
+    __shared__ float vc[KX_CW];
+    __shared__ float red[KX_T / 64];
+    // the build: column group g of cg takes rows r of columns g + cg j
+    int cg = KX_T / dim;
+    int g = tid / dim, r = tid - g * dim;
+    unsigned long long iters = 0;
+    for (int k = 0; k < K; k++) {
+        // the thread coordinates and sizes pass through an opaque copy each
+        // atom, so that the addresses derived from them are formed where they
+        // are used instead of being held (and spilled) across the loop
+        asm volatile("" : "+s"(dim), "+s"(dpad), "+s"(ep), "+s"(cg), "+s"(wave));
+        asm volatile("" : "+v"(tid), "+v"(lane), "+v"(g), "+v"(r));
+        int c;
+        const int m = atom_m(a.uoff, k, cap, &c);
+        const int cw = (m + KX_R - 1) / KX_R;
+        const int tb = min(m, role * cw);
+        const int nc = min(m, tb + cw) - tb;
+        const int64_t cb = a.moff[k] + tb;
+        // ---- this role's columns of E_k: e = y - sum_j d_j x_j in the column
+        //      table's entry order (kl_build_kernel's sums) -------------------------
+        if (tid < nc) {
+            sp[tid] = a.tab_p[cb + tid];
+            sself[tid] = a.tab_self[cb + tid];
+        }
+        for (int q = tid; q < nc * KX_NO; q += KX_T) {
+            const int t = q / KX_NO, i = q - t * KX_NO;
+            int at = -1;
+            float xv = 0.f;
+            if (i < no) {
+                const int4 o = a.tab_oth[(cb + t) * no + i];
+                if (o.x >= 0) {
+                    at = o.x;
+                    xv = a.coef[o.y];
+                }
+            }
+            sat[q] = at;
+            sx[q] = xv;
+        }
+        if (tid < dim) u[tid] = a.D[(int64_t)k * dim + tid];
+        __syncthreads();
+        if (g < cg) {
+            for (int t0 = g; t0 < nc; t0 += cg * KX_B) {
+                float v[KX_B];
+#pragma unroll
+                for (int j = 0; j < KX_B; j++) v[j] = a.Y[(int64_t)sp[min(t0 + j * cg, nc - 1)] * dim + r];
+                // every entry's atom row loaded without a branch (row 0 for
+                // empty entries, whose term is then not taken), so that all
+                // the loads are in flight together
+#pragma unroll
+                for (int i = 0; i < KX_NO; i++) {
+                    if (i >= no) break;
+                    float dv[KX_B];
+#pragma unroll
+                    for (int j = 0; j < KX_B; j++) {
+                        const int t = min(t0 + j * cg, nc - 1);  // a column of this atom's table
+                        const int at = t0 + j * cg < nc ? sat[t * KX_NO + i] : -1;
+                        dv[j] = a.D[(int64_t)max(at, 0) * dim + r];
+                    }
+#pragma unroll
+                    for (int j = 0; j < KX_B; j++) {
+                        const int t = min(t0 + j * cg, nc - 1);
+                        const int at = t0 + j * cg < nc ? sat[t * KX_NO + i] : -1;
+                        const float nv = fmaf(-dv[j], sx[t * KX_NO + i], v[j]);
+                        v[j] = at >= 0 ? nv : v[j];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < KX_B; j++) {
+                    const int t = t0 + j * cg;
+                    if (t < nc) Es[t * ep + r] = v[j];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- power iteration from d_k -------------------------------------------
+        int it = 0;
+        float d2p = 0.f;
+        if (m > 0) {
+            for (;;) {
+                // the column dots e_t . u, a wavefront per column
+                for (int t = wave; t < nc; t += KX_T / 64) {
+                    float s = 0.f;
+                    for (int f = lane; f < dim; f += 64) s = fmaf(Es[t * ep + f], u[f], s);
+                    s = wave_sum_dpp(s);
+                    if (lane == 0) vc[t] = s;
+                }
+                __syncthreads();
+                const unsigned tag = ((unsigned)k << 12) | (unsigned)(it + 1);
+                unsigned long long *W = wx + (size_t)(it & 1) * KX_R * dpad;      // stage 1: partials
+                unsigned long long *Z = wx + (size_t)2 * KX_R * dpad + (size_t)(it & 1) * dpad;  // stage 2: rows
+                float zv = 0.f;
+                if (tid < dim) {
+                    float z = 0.f;
+                    for (int t = 0; t < nc; t++) z = fmaf(Es[t * ep + tid], vc[t], z);
+                    st64_c(W + (size_t)role * dpad + tid, ((unsigned long long)tag << 32) | __float_as_uint(z));
+                }
+                if (bail[1] == 0 && tid == 0) {
+                    st_flag(flags + PSF_STICKY, 1u);
+                    st_flag(flags + PSF_ABORT, 1u);
+                    bail[0] = 1;
+                }
+                // stage 1: this role's rows [role rpr, role rpr + rpr), lane
+                // group (row, source role) of 32 lanes each; the 32 partials
+                // added by a fixed butterfly, the sum stored as a tagged pair
+                const int rpr = (dim + KX_R - 1) / KX_R;
+                const int row = role * rpr + (tid >> 5), src = tid & 31;
+                if (tid < KX_R * rpr && bail[1] != 0) {
+                    float pz = 0.f;
+                    if (row < dim) {
+                        const unsigned long long *pp = W + (size_t)src * dpad + row;
+                        unsigned long long pv = ld64_c(pp);
+                        long spins = 0;
+                        while ((unsigned)(pv >> 32) != tag) {
+                            __builtin_amdgcn_s_sleep(1);
+                            if ((++spins & 31) == 0) {
+                                if (ld_flag(flags + PSF_ABORT)) {
+                                    bail[0] = 1;
+                                    break;
+                                }
+                                if (spins > spin_limit(flags)) {
+                                    st_flag(flags + PSF_STICKY, 1u);
+                                    st_flag(flags + PSF_ABORT, 1u);
+                                    bail[0] = 1;
+                                    break;
+                                }
+                            }
+                            pv = ld64_c(pp);
+                        }
+                        pz = __uint_as_float((unsigned)pv);
+                    }
+#pragma unroll
+                    for (int off = 1; off < 32; off <<= 1) pz += __shfl_xor(pz, off);
+                    if (src == 0 && row < dim) st64_c(Z + row, ((unsigned long long)tag << 32) | __float_as_uint(pz));
+                }
+                // stage 2: every row's sum
+                if (tid < dim && bail[1] != 0) {
+                    const unsigned long long *pp = Z + tid;
+                    unsigned long long pv = ld64_c(pp);
+                    long spins = 0;
+                    while ((unsigned)(pv >> 32) != tag) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if ((++spins & 31) == 0) {
+                            if (ld_flag(flags + PSF_ABORT)) {
+                                bail[0] = 1;
+                                break;
+                            }
+                            if (spins > spin_limit(flags)) {
+                                st_flag(flags + PSF_STICKY, 1u);
+                                st_flag(flags + PSF_ABORT, 1u);
+                                bail[0] = 1;
+                                break;
+                            }
+                        }
+                        pv = ld64_c(pp);
+                    }
+                    zv = __uint_as_float((unsigned)pv);
+                }
+                const float n2 = kq_block_sum(zv * zv, red);
+                if (bail[0]) return;
+                if (n2 == 0.f) break;  // E = 0: d_k stays (its coefficients become 0)
+                const float un = zv / sqrtf(n2);
+                const float du = tid < dim ? un - u[tid] : 0.f;
+                const float d2 = kq_block_sum(du * du, red);
+                it++;
+                if (tid < dim) u[tid] = un;
+                __syncthreads();
+                if (power_stop(it, d2, d2p, a.tol2, a.est2, a.maxit)) break;
+                d2p = d2;
+            }
+            if (role == 0 && tid < dim) a.D[(int64_t)k * dim + tid] = u[tid];
+            // the new coefficients of this role's columns
+            for (int t = wave; t < nc; t += KX_T / 64) {
+                float s = 0.f;
+                for (int f = lane; f < dim; f += 64) s = fmaf(Es[t * ep + f], u[f], s);
+                s = wave_sum_dpp(s);
+                if (lane == 0) a.coef[sself[t]] = s;
+            }
+        }
+        iters += (unsigned long long)it;
+        // users beyond the cap: coefficient 0
+        const int base = a.uoff[k];
+        for (int t = m + role * KX_T + tid; t < c; t += KX_R * KX_T)
+            a.coef[(int64_t)a.users[base + t] * kmax + a.uslot[base + t]] = 0.f;
+        // ---- every role done with atom k before atom k+1's columns ------------
+        ps_publish(flags + PSF_DONE + role, (unsigned)(k + 1));
+        if (k + 1 < K) {
+            if (!ps_wait(flags, PSF_DONE, KX_R, (unsigned)(k + 1), bail)) return;
+            l1_drop();
+        }
+    }
+    if (role == 0 && tid == 0) atomicAdd(&a.counters[CNT_EIG_IT], iters);
+}
+
+__global__ __launch_bounds__(KX_T) void kx_sweep_kernel(SweepArgs a, unsigned *flags, unsigned long long *wx)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
+    if (threadIdx.x == 0) {
+        s_role = ps_register(flags, KX_R);
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    kx_team(a, flags, wx, role, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+}
+
+// Several pictures' large-patch sweeps in one launch, picture i on XCD i (its
+// own flags and pairs), each the sweep above bit for bit.
+__global__ __launch_bounds__(KX_T) void kx_multi_kernel(const PtPicture *__restrict__ pics, int n)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    __shared__ int s_role, s_bail[2];  // bail word, spin limit
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;  // XCC_ID[3:0]
+    if (xcc >= (unsigned)n) return;
+    unsigned *flags = pics[xcc].flags;
+    if (threadIdx.x == 0) {
+        (void)atomicCAS(&flags[PSF_XSEL], 0u, xcc + 1u);
+        const unsigned slot = atomicAdd(&flags[PSF_REG + xcc], 1u);
+        s_role = slot < (unsigned)KX_R ? (int)slot : -1;
+        s_bail[0] = 0;
+        s_bail[1] = (int)ld_flag(flags + PSF_SPIN);
+    }
+    __syncthreads();
+    const int role = s_role;
+    if (role < 0) return;
+    const SweepArgs a = pics[xcc].a;
+    kx_team(a, flags, pics[xcc].wx, role, sm, s_bail);
+    if (threadIdx.x == 0 && s_bail[0] == 0) atomicAdd(&flags[PSF_FINISHED], 1u);
+}
+
 __global__ __launch_bounds__(256) void kl_coef_kernel(SweepArgs a, const float *__restrict__ E, int k)
 {
     __shared__ float u[KSVD_MAX_DIM];
@@ -3258,6 +3539,18 @@ static bool kl_layout(const ksvd_ctx *ctx, int nsplit, KlLayout *L)
            L->lds_q <= 150 * 1024;
 }
 
+// Whether the one-launch large-patch sweep applies (padded dim above 192 up
+// to 512, caps up to 1024, its support entries staged, its pairs within the
+// partial-Gram buffer); KSVD_SWEEP_KL=1 selects the three launches per atom.
+static bool kx_applies(const ksvd_ctx *ctx, bool two_kernel)
+{
+    const int dpad = (ctx->dim + 15) / 16 * 16;
+    return !two_kernel && !env_flag("KSVD_SWEEP_KL") && ctx->prm.user_cap > 0 &&
+           ctx->prm.user_cap <= KX_R * KX_CW && ctx->dim <= KX_T / 2 && ctx->kmax - 1 <= KX_NO &&
+           sizeof(unsigned long long) * 2 * (KX_R + 1) * (size_t)dpad <= ctx->gpart.bytes &&
+           kx_lds_bytes(ctx->dim) <= 120 * 1024;
+}
+
 int launch_sweep(ksvd_ctx *ctx, float tol)
 {
     ctx->sweep_defers_zero = false;
@@ -3278,6 +3571,19 @@ int launch_sweep(ksvd_ctx *ctx, float tol)
     // ---- large patches: E in global memory, four kernels per atom ------------
     auto launch_large = [&]() -> int {
         if (dim > KSVD_MAX_DIM) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "patch dimension above 1024");
+        if (kx_applies(ctx, two_kernel)) {
+            // one launch: the columns in the LDS of one XCD's workgroups
+            unsigned *flags = (unsigned *)ctx->ticket.ptr;
+            unsigned long long *wx = (unsigned long long *)a.Gpart;
+            const int nw = 2 * (KX_R + 1) * dpad;
+            KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kx_sweep_kernel));
+            hipLaunchKernelGGL(ps_reset_kernel, dim3((std::max(nw, (int)PSF_WORDS) + 255) / 256), dim3(256), 0,
+                               ctx->stream, flags, wx, nw, (unsigned)KX_R, spin);
+            hipLaunchKernelGGL(kx_sweep_kernel, dim3(ctx->n_xcd * KX_R), dim3(KX_T), kx_lds_bytes(dim), ctx->stream, a,
+                               flags, wx);
+            KSVD_HIP_CHECK(ctx, hipGetLastError());
+            return KSVD_OK;
+        }
         float *E = (float *)ctx->E.ptr;  // m x dim, m <= min(users, cap): the E buffer's columns
         const int nb = (dim + KG_TB - 1) / KG_TB;
         const int ntg = nb * (nb + 1) / 2;
@@ -3438,11 +3744,13 @@ int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
             return 0;
         if (large ? !kl_layout(c, KG_SPLIT, &L) : !(et_applies(c, two_kernel) || pt_applies(c, two_kernel))) return 0;
         if (!large && et_applies(c, two_kernel) != et_applies(c0, two_kernel)) return 0;
+        if (large && kx_applies(c, two_kernel) != kx_applies(c0, two_kernel)) return 0;
     }
+    const bool kx = large && kx_applies(c0, two_kernel);
     const size_t bytes = std::max(sizeof(PtPicture), sizeof(KlPicture)) * PT_MULTI_MAX;
     int rc = ksvd_reserve(c0, c0->multi, bytes);
     if (rc) return rc;
-    if (!large) {
+    if (!large || kx) {
         PtPicture h[PT_MULTI_MAX];
         for (int i = 0; i < n; i++) {
             h[i].a = sweep_args(ctxs[i], tol);
@@ -3463,7 +3771,7 @@ int sweep_multi_setup(ksvd_ctx *const *ctxs, int n, float tol)
         }
         KSVD_HIP_CHECK(c0, ksvd_copy(c0, c0->multi.ptr, h, sizeof(KlPicture) * n, hipMemcpyHostToDevice));
     }
-    c0->multi_mode = large ? 2 : et_applies(c0, two_kernel) ? 3 : 1;
+    c0->multi_mode = kx ? 4 : large ? 2 : et_applies(c0, two_kernel) ? 3 : 1;
     return 1;
 }
 
@@ -3483,6 +3791,10 @@ int sweep_multi_reset(ksvd_ctx *const *ctxs, int n)
         if (ctxs[0]->multi_mode == 3) {
             nw = (int)et_pairs_n(dpad);
             nroles = ET_NT * ET_HT;
+        }
+        if (ctxs[0]->multi_mode == 4) {
+            nw = 2 * (KX_R + 1) * dpad;
+            nroles = KX_R;
         }
         if (large) {
             KlLayout L;
@@ -3524,6 +3836,13 @@ int launch_sweep_multi(ksvd_ctx *const *ctxs, int n)
             hipLaunchKernelGGL(kq_power_multi, dim3(c0->n_xcd * KQ_R), dim3(KQ_T), L.lds_q, c0->stream, pics, n, L.gp,
                                L.nsplit, k);
         }
+        KSVD_HIP_CHECK(c0, hipGetLastError());
+        return KSVD_OK;
+    }
+    if (c0->multi_mode == 4) {
+        KSVD_HIP_CHECK(c0, allow_max_lds((const void *)kx_multi_kernel));
+        hipLaunchKernelGGL(kx_multi_kernel, dim3(c0->n_xcd * KX_R), dim3(KX_T), kx_lds_bytes(c0->dim), c0->stream,
+                           (const PtPicture *)c0->multi.ptr, n);
         KSVD_HIP_CHECK(c0, hipGetLastError());
         return KSVD_OK;
     }

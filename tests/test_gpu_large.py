@@ -1,7 +1,9 @@
 """Large and rectangular patches: 16 x 12 (dim 192, the LDS-resident
 two-team sweep at its widest) and 22 x 22 (dim 484: the generic pursuit and
-the Gram-free sweep with E in global memory), the patch sizes of the batched
-workload's files and per-file overrides.
+the one-launch large-patch sweep with each workgroup's columns in LDS, or,
+with KSVD_SWEEP_KL=1, three launches per atom with E and its Gram in global
+memory), the patch sizes of the batched workload's files and per-file
+overrides.
 
 Bars, against the oracle on the same seeded inputs:
   * OMP: identical supports on >= 99.9 % of patches, coefficients within
@@ -51,8 +53,14 @@ def test_large_patch_omp(kl, orc, cfg):
     assert err64 <= 1e-4
 
 
+@pytest.mark.parametrize("path", ["default", "kl"])
 @pytest.mark.parametrize("cfg", CASES, ids=IDS)
-def test_large_patch_sweep(kl, orc, cfg):
+def test_large_patch_sweep(kl, orc, cfg, path, monkeypatch):
+    """path "kl": KSVD_SWEEP_KL=1 keeps 22 x 22 patches on the three launches
+    per atom (build, Gram, persistent power iteration) instead of the
+    one-launch sweep with the columns in LDS."""
+    if path == "kl":
+        monkeypatch.setenv("KSVD_SWEEP_KL", "1")
     clean, noisy, ks = context(kl, cfg)
     Y, D0 = ks.patches(), ks.dictionary()
     io, co, no, _ = orc.omp(Y, D0, cfg["k"], real="f32")
