@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
 #include "ksvd_internal.hpp"
@@ -128,23 +129,42 @@ __device__ __forceinline__ void gram_argmax(const f4 (&c0)[KT], const float *__r
 // Start of row i of a packed lower triangle.
 __host__ __device__ constexpr int tri(int i) { return i * (i + 1) / 2; }
 
-template <int Q, int KT, int KM, int W, bool FULL>
+// Quarter-row of atom j (elements e0 .. e0 + 4Q - 1 of its row, 0 past dim)
+// from global memory, 16 bytes at a time where they lie inside the row.
+template <int Q>
+__device__ __forceinline__ f4 drow4(const float *__restrict__ D, int dim, int j, int e)
+{
+    if (e + 3 < dim) return *(const f4 *)(D + (int64_t)j * dim + e);
+    f4 v;
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = e + c < dim ? D[(int64_t)j * dim + e + c] : 0.f;
+    return v;
+}
+
+// GC (global correlation): step 0's correlations come from C0 (row p of the
+// launch's patches at C0 + p K, the GEMM of corr_gemm_kernel) instead of the
+// MFMA, and the atoms from global memory instead of LDS, for patch
+// dimensions whose dictionary does not fit the LDS (up to 4Q).
+template <int Q, int KT, int KM, int W, bool FULL, bool GC = false>
 __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
     const float *__restrict__ Y, int dim, int64_t P, const float *__restrict__ D, int K,
     const float *__restrict__ G, int kmax, int ntiles, const int *__restrict__ plist,
     const unsigned *__restrict__ plist_n, int *__restrict__ idx_out, float *__restrict__ coef_out,
-    int *__restrict__ cnt_out, float *__restrict__ rnorm_out, unsigned long long *__restrict__ counters)
+    int *__restrict__ cnt_out, float *__restrict__ rnorm_out, unsigned long long *__restrict__ counters,
+    const float *__restrict__ C0 = nullptr)
 {
     constexpr int LD = 4 * Q + 4;
     constexpr int KP = 16 * KT;
     constexpr int AH = W > 4 ? 4 : KT;  // blocks of A read from LDS at a time
     static_assert(W % 4 == 0, "the tile order takes whole sets of four wavefronts");
     extern __shared__ __attribute__((aligned(16))) float Ds[];  // KP x LD, zero beyond (K, dim)
-    for (int q = threadIdx.x; q < KP * LD; q += blockDim.x) {
-        const int a = q / LD, e = q - a * LD;
-        Ds[q] = (a < K && e < dim) ? D[(int64_t)a * dim + e] : 0.f;
+    if (!GC) {
+        for (int q = threadIdx.x; q < KP * LD; q += blockDim.x) {
+            const int a = q / LD, e = q - a * LD;
+            Ds[q] = (a < K && e < dim) ? D[(int64_t)a * dim + e] : 0.f;
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     unsigned long long ndup = 0, ndeg = 0;
 
@@ -211,7 +231,23 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
             const int e0 = g * Q;
             float best = -1.f;
             int bi = 0x7fffffff;
-            if (t == 0) {
+            if (t == 0 && GC) {
+                // ---- correlation C = D^T Y from the GEMM ----------------------
+                const int64_t pc = valid ? p : 0;
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++) {
+                    const int a0 = tt * 16 + 4 * g;
+                    if (FULL) {
+                        c0[tt] = *(const f4 *)(C0 + pc * K + a0);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) c0[tt][i] = a0 + i < K ? C0[pc * K + a0 + i] : 0.f;
+                    }
+                    if (!valid) c0[tt] = f4{0.f, 0.f, 0.f, 0.f};
+                }
+#pragma unroll
+                for (int tt = 0; tt < KT; tt++) amax_fold(c0[tt], tt * 16 + 4 * g, best, bi);
+            } else if (t == 0) {
                 // ---- correlation C = D^T Y on MFMA ----------------------------
 #pragma unroll
                 for (int tt = 0; tt < KT; tt++) c0[tt] = f4{0.f, 0.f, 0.f, 0.f};
@@ -253,7 +289,7 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
                 const float *dj = Ds + (j < KP ? j : 0) * LD + e0;
 #pragma unroll
                 for (int kb = 0; kb < Q; kb += 4) {
-                    const f4 dv = *(const f4 *)(dj + kb);
+                    const f4 dv = GC ? drow4<Q>(D, dim, j < K ? j : 0, e0 + kb) : *(const f4 *)(dj + kb);
                     bp = fmaf(dv[0], y[kb + 0], bp);
                     bp = fmaf(dv[1], y[kb + 1], bp);
                     bp = fmaf(dv[2], y[kb + 2], bp);
@@ -352,7 +388,7 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 #pragma unroll
                         for (int i = 0; i < KM; i++)
                             if (i < s) {
-                                const f4 dv = *(const f4 *)(Ds + S[i] * LD + e0 + kb);
+                                const f4 dv = GC ? drow4<Q>(D, dim, S[i], e0 + kb) : *(const f4 *)(Ds + S[i] * LD + e0 + kb);
                                 a4[0] = fmaf(dv[0], x[i], a4[0]);
                                 a4[1] = fmaf(dv[1], x[i], a4[1]);
                                 a4[2] = fmaf(dv[2], x[i], a4[2]);
@@ -650,7 +686,7 @@ int launch_mfma(ksvd_ctx *ctx, const int *plist, const unsigned *plist_n, float 
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, ctx->stream, (const float *)ctx->Y.ptr,
                        ctx->dim, ctx->P, (const float *)ctx->D.ptr, ctx->K, (const float *)ctx->G.ptr, ctx->kmax,
                        ntiles, plist, plist_n, (int *)ctx->idx.ptr, (float *)ctx->coef.ptr, (int *)ctx->cnt.ptr,
-                       rnorm, (unsigned long long *)ctx->counters.ptr);
+                       rnorm, (unsigned long long *)ctx->counters.ptr, (const float *)nullptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
     return KSVD_OK;
 }
@@ -687,6 +723,44 @@ int launch_batch_c(ksvd_ctx *ctx, int64_t p0, int64_t np, float *C0)
     return launch_batch_k<CPL, 16>(ctx, p0, np, C0);
 }
 
+// The tile pursuit (16 patches a wavefront) on a chunk of patches whose
+// step-0 correlations are in C0: omp_mfma_kernel's steps, the atoms from
+// global memory.
+template <int Q, int KM>
+int launch_tiles_gc(ksvd_ctx *ctx, int64_t p0, int64_t np, const float *C0)
+{
+    constexpr int KT = 16;
+    constexpr int W = Q > 48 ? 4 : omp_waves(KM);  // a 64-element quarter needs a SIMD's registers
+    const int kmax = ctx->kmax, dim = ctx->dim;
+    auto kern = ctx->K == 16 * KT ? omp_mfma_kernel<Q, KT, KM, W, true, true>
+                                  : omp_mfma_kernel<Q, KT, KM, W, false, true>;
+    const int ntiles = (int)((np + 15) / 16);
+    int grid = (ntiles + W - 1) / W;
+    if (grid > ctx->num_cus) grid = ctx->num_cus;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), 0, ctx->stream, (const float *)ctx->Y.ptr + p0 * dim, dim, np,
+                       (const float *)ctx->D.ptr, ctx->K, (const float *)ctx->G.ptr, kmax, ntiles, (const int *)nullptr,
+                       (const unsigned *)nullptr, (int *)ctx->idx.ptr + p0 * kmax, (float *)ctx->coef.ptr + p0 * kmax,
+                       (int *)ctx->cnt.ptr + p0, (float *)nullptr, (unsigned long long *)ctx->counters.ptr, C0);
+    KSVD_HIP_CHECK(ctx, hipGetLastError());
+    return KSVD_OK;
+}
+
+// Where the tile pursuit takes the GEMM's correlations: patch dims 145-256
+// (a quarter of up to 64 elements a lane), atoms up to 256, OMP limit up to 8;
+// KSVD_OMP_WAVE_PER_PATCH=1 keeps the one-wavefront-per-patch pursuit.
+static bool tiles_gc_ok(const ksvd_ctx *ctx)
+{
+    const char *v = getenv("KSVD_OMP_WAVE_PER_PATCH");
+    return ctx->dim <= 256 && ctx->K <= 256 && ctx->kmax <= 8 && !(v && v[0] == '1');
+}
+
+static int launch_tiles_gc_any(ksvd_ctx *ctx, int64_t p0, int64_t np, const float *C0)
+{
+    if (ctx->kmax <= 5)
+        return ctx->dim <= 192 ? launch_tiles_gc<48, 5>(ctx, p0, np, C0) : launch_tiles_gc<64, 5>(ctx, p0, np, C0);
+    return ctx->dim <= 192 ? launch_tiles_gc<48, 8>(ctx, p0, np, C0) : launch_tiles_gc<64, 8>(ctx, p0, np, C0);
+}
+
 // The GEMM-correlation pursuit over every patch, in chunks whose C0 stays
 // under 128 MB.
 int launch_batch(ksvd_ctx *ctx)
@@ -705,7 +779,10 @@ int launch_batch(ksvd_ctx *ctx)
                            dim3(256), 0, ctx->stream, (const float *)ctx->Y.ptr + p0 * ctx->dim, ctx->dim, np,
                            (const float *)ctx->D.ptr, K, C0);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
-        rc = K <= 256 ? launch_batch_c<4>(ctx, p0, np, C0) : launch_batch_c<16>(ctx, p0, np, C0);
+        if (tiles_gc_ok(ctx))
+            rc = launch_tiles_gc_any(ctx, p0, np, C0);
+        else
+            rc = K <= 256 ? launch_batch_c<4>(ctx, p0, np, C0) : launch_batch_c<16>(ctx, p0, np, C0);
         if (rc) return rc;
     }
     return KSVD_OK;
@@ -797,7 +874,8 @@ hipError_t noalloc_launch(const float *Y, int dim, int64_t P, const float *D, in
     if (grid > ncu) grid = ncu;
     auto kern = K == 16 * KT ? omp_mfma_kernel<Q, KT, KM, W, true> : omp_mfma_kernel<Q, KT, KM, W, false>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), lds, st, Y, dim, P, D, K, G, kmax, ntiles,
-                       (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt, (float *)nullptr, counters);
+                       (const int *)nullptr, (const unsigned *)nullptr, idx, coef, cnt, (float *)nullptr, counters,
+                       (const float *)nullptr);
     return hipGetLastError();
 }
 
