@@ -36,8 +36,17 @@ def context(kl, cfg, **kw):
     return clean, noisy, ks
 
 
-@pytest.mark.parametrize("cfg", CASES, ids=IDS)
-def test_large_patch_omp(kl, orc, cfg):
+OMP_CASES = CASES + [dict(CASES[0], atoms=200, seed=43), dict(CASES[0], seed=44, wave_per_patch=True)]
+OMP_IDS = IDS + ["16x12_200atoms", "16x12_wave_per_patch"]
+
+
+@pytest.mark.parametrize("cfg", OMP_CASES, ids=OMP_IDS)
+def test_large_patch_omp(kl, orc, cfg, monkeypatch):
+    """16 x 12 runs the 16-patch tile pursuit with the GEMM's correlations
+    (whole Gram rows at 256 atoms, bounded reads at 200);
+    KSVD_OMP_WAVE_PER_PATCH=1 and 22 x 22 run one wavefront per patch."""
+    if cfg.get("wave_per_patch"):
+        monkeypatch.setenv("KSVD_OMP_WAVE_PER_PATCH", "1")
     clean, noisy, ks = context(kl, cfg)
     ks.omp()
     g = ks.codes()
