@@ -204,8 +204,17 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
         // ||y||^2, and the band below which the retirement test needs the
         // explicit residual (see the residual norm below)
         float yy = 0.f;
+        if (Q > 64) {
+            f4 y4 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kk = 0; kk < Q; kk++) yy = fmaf(y[kk], y[kk], yy);
+            for (int kk = 0; kk < Q; kk += 4)
+#pragma unroll
+                for (int c = 0; c < 4; c++) y4[c] = fmaf(y[kk + c], y[kk + c], y4[c]);
+            yy = (y4[0] + y4[1]) + (y4[2] + y4[3]);
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < Q; kk++) yy = fmaf(y[kk], y[kk], yy);
+        }
         yy = patch_sum(yy);
         const float band = 1e-6f + yy * 0x1p-10f + sqrtf(yy) * 0x1p-8f;
         // the Cholesky factor's lower triangle, row by row: entry (i, q),
@@ -285,7 +294,18 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
             const int j = bi;
             // ---- b_j = d_j . y ------------------------------------------------
             float bp = 0.f;
-            {
+            if (Q > 64) {
+                // long quarters: four partial sums (elements 4i + c), added
+                // in a fixed order, keep the rounding of a 4Q-term dot low
+                f4 b4 = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < Q; kb += 4) {
+                    const f4 dv = drow4<Q>(D, dim, j < K ? j : 0, e0 + kb);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) b4[c] = fmaf(dv[c], y[kb + c], b4[c]);
+                }
+                bp = (b4[0] + b4[1]) + (b4[2] + b4[3]);
+            } else {
                 const float *dj = Ds + (j < KP ? j : 0) * LD + e0;
 #pragma unroll
                 for (int kb = 0; kb < Q; kb += 4) {
@@ -730,7 +750,7 @@ template <int Q, int KM>
 int launch_tiles_gc(ksvd_ctx *ctx, int64_t p0, int64_t np, const float *C0)
 {
     constexpr int KT = 16;
-    constexpr int W = Q > 48 ? 4 : omp_waves(KM);  // a 64-element quarter needs a SIMD's registers
+    constexpr int W = Q > 48 ? 4 : omp_waves(KM);  // a quarter above 48 elements needs a SIMD's registers
     const int kmax = ctx->kmax, dim = ctx->dim;
     auto kern = ctx->K == 16 * KT ? omp_mfma_kernel<Q, KT, KM, W, true, true>
                                   : omp_mfma_kernel<Q, KT, KM, W, false, true>;
@@ -745,20 +765,25 @@ int launch_tiles_gc(ksvd_ctx *ctx, int64_t p0, int64_t np, const float *C0)
     return KSVD_OK;
 }
 
-// Where the tile pursuit takes the GEMM's correlations: patch dims 145-256
-// (a quarter of up to 64 elements a lane), atoms up to 256, OMP limit up to 8;
-// KSVD_OMP_WAVE_PER_PATCH=1 keeps the one-wavefront-per-patch pursuit.
+// Where the tile pursuit takes the GEMM's correlations: patch dims 145-496
+// (a quarter of up to 124 elements a lane), atoms up to 256, OMP limit up to
+// 8; KSVD_OMP_WAVE_PER_PATCH=1 keeps the one-wavefront-per-patch pursuit.
 static bool tiles_gc_ok(const ksvd_ctx *ctx)
 {
     const char *v = getenv("KSVD_OMP_WAVE_PER_PATCH");
-    return ctx->dim <= 256 && ctx->K <= 256 && ctx->kmax <= 8 && !(v && v[0] == '1');
+    return ctx->dim <= 496 && ctx->K <= 256 && ctx->kmax <= 8 && !(v && v[0] == '1');
 }
 
 static int launch_tiles_gc_any(ksvd_ctx *ctx, int64_t p0, int64_t np, const float *C0)
 {
+    const int dim = ctx->dim;
     if (ctx->kmax <= 5)
-        return ctx->dim <= 192 ? launch_tiles_gc<48, 5>(ctx, p0, np, C0) : launch_tiles_gc<64, 5>(ctx, p0, np, C0);
-    return ctx->dim <= 192 ? launch_tiles_gc<48, 8>(ctx, p0, np, C0) : launch_tiles_gc<64, 8>(ctx, p0, np, C0);
+        return dim <= 192   ? launch_tiles_gc<48, 5>(ctx, p0, np, C0)
+               : dim <= 256 ? launch_tiles_gc<64, 5>(ctx, p0, np, C0)
+                            : launch_tiles_gc<124, 5>(ctx, p0, np, C0);
+    return dim <= 192   ? launch_tiles_gc<48, 8>(ctx, p0, np, C0)
+           : dim <= 256 ? launch_tiles_gc<64, 8>(ctx, p0, np, C0)
+                        : launch_tiles_gc<124, 8>(ctx, p0, np, C0);
 }
 
 // The GEMM-correlation pursuit over every patch, in chunks whose C0 stays

@@ -42,9 +42,10 @@ OMP_IDS = IDS + ["16x12_200atoms", "16x12_wave_per_patch"]
 
 @pytest.mark.parametrize("cfg", OMP_CASES, ids=OMP_IDS)
 def test_large_patch_omp(kl, orc, cfg, monkeypatch):
-    """16 x 12 runs the 16-patch tile pursuit with the GEMM's correlations
-    (whole Gram rows at 256 atoms, bounded reads at 200);
-    KSVD_OMP_WAVE_PER_PATCH=1 and 22 x 22 run one wavefront per patch."""
+    """Both sizes run the 16-patch tile pursuit with the GEMM's correlations
+    (whole Gram rows at 256 atoms, bounded reads at 200; 22 x 22 quarters of
+    121 elements at one wavefront per SIMD); KSVD_OMP_WAVE_PER_PATCH=1 runs one
+    wavefront per patch."""
     if cfg.get("wave_per_patch"):
         monkeypatch.setenv("KSVD_OMP_WAVE_PER_PATCH", "1")
     clean, noisy, ks = context(kl, cfg)
