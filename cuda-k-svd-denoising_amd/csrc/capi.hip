@@ -783,7 +783,7 @@ int ksvd_run(ksvd_ctx *ctx)
     return ksvd_run_final_code(ctx);
 }
 
-int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
+static int rebuild_common(ksvd_ctx *ctx, float sigma2, float *out, bool speckle)
 {
     if (!ctx) return KSVD_ERR_ARG;
     if (!ctx->has_codes || !ctx->has_dict || !ctx->has_image)
@@ -795,11 +795,21 @@ int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
     if ((rc = flush_cap_zero(ctx))) return rc;
     Span s;
     if ((rc = span_begin(ctx, &s, SPAN_REBUILD))) return rc;
-    if ((rc = launch_rebuild(ctx, sigma2))) return rc;
+    if ((rc = launch_rebuild(ctx, sigma2, speckle))) return rc;
     if ((rc = span_end(ctx, &s))) return rc;
     ctx->has_rebuild = true;
     if (out) return sync_copy_d2h(ctx, out, ctx->rout.ptr, sizeof(float) * (size_t)ctx->m * ctx->n);
     return KSVD_OK;
+}
+
+int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out)
+{
+    return rebuild_common(ctx, sigma2, out, false);
+}
+
+int ksvd_rebuild_speckle(ksvd_ctx *ctx, float sigma2, float *out)
+{
+    return rebuild_common(ctx, sigma2, out, true);
 }
 
 int ksvd_get_rebuild_parts(ksvd_ctx *ctx, float *sum, float *weight, float *lambda, double *dsum)

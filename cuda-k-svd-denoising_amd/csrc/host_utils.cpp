@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -182,18 +183,47 @@ extern "C" void ksvd_rgb_to_y(const float *r, const float *g, const float *b, in
     ksvd_rgb_to_ycbcr(r, g, b, count, y, nullptr, nullptr);
 }
 
+namespace {
+
 // The speckle transforms of the reference's host path (myLog / myExp,
-// denoisingLib.h:124-142): log2 with 0 kept at 0, and 2^v.
+// denoisingLib.h:124-142): log2 with 0 kept at 0, and 2^v.  Each element is
+// computed alone by the same libm call, so splitting a large buffer over
+// host threads (up to 16, a million elements or more each) gives the same
+// bits as one thread.
+template <class F>
+void each_element(float *buf, int64_t count, F f)
+{
+    const int64_t chunk = 1 << 20;
+    int64_t nt = count / chunk;
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min<int64_t>(std::min<int64_t>(nt, hw), 16);
+    if (nt <= 1) {
+        for (int64_t q = 0; q < count; q++) buf[q] = f(buf[q]);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t per = (count + nt - 1) / nt;
+    for (int64_t t = 0; t < nt; t++) {
+        const int64_t a = t * per, b = std::min(count, a + per);
+        th.emplace_back([=] {
+            for (int64_t q = a; q < b; q++) buf[q] = f(buf[q]);
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
 extern "C" void ksvd_speckle_log(float *buf, int64_t count)
 {
-    if (!buf) return;
-    for (int64_t q = 0; q < count; q++) buf[q] = buf[q] == 0.f ? 0.f : log2f(buf[q]);
+    if (!buf || count <= 0) return;
+    each_element(buf, count, [](float v) { return v == 0.f ? 0.f : log2f(v); });
 }
 
 extern "C" void ksvd_speckle_exp(float *buf, int64_t count)
 {
-    if (!buf) return;
-    for (int64_t q = 0; q < count; q++) buf[q] = powf(2.f, buf[q]);
+    if (!buf || count <= 0) return;
+    each_element(buf, count, [](float v) { return powf(2.f, v); });
 }
 
 extern "C" void ksvd_image_to_frame(const float *img, int W, int H, float *buf)

@@ -194,6 +194,6 @@ void drop_update_graph(ksvd_ctx *ctx);
 size_t sweep_gpart_bytes(int dim);
 // Synchronises the stream; fails if a persistent sweep gave up waiting.
 int check_sweep(ksvd_ctx *ctx);
-int launch_rebuild(ksvd_ctx *ctx, float sigma2);
+int launch_rebuild(ksvd_ctx *ctx, float sigma2, bool speckle);
 // The calling thread's last ksvd_omp_noalloc error, or nullptr (omp.hip).
 const char *noalloc_last_error();

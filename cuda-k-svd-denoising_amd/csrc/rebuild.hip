@@ -253,6 +253,10 @@ __global__ __launch_bounds__(256) void rebuild_lambda_kernel(const double *__res
     }
 }
 
+// SPECKLE: the output leaves the log domain, v -> 2^v (createImage's
+// speckle branch, cuda_svd_denoiser.cu:483-484, which the reference runs on
+// the host with powf(2, v)); exp2f is within 2 ulp of it.
+template <bool SPECKLE>
 __global__ __launch_bounds__(256) void rebuild_blend_kernel(const float *__restrict__ img,
                                                             const float *__restrict__ sum,
                                                             const float *__restrict__ wgt,
@@ -263,18 +267,20 @@ __global__ __launch_bounds__(256) void rebuild_blend_kernel(const float *__restr
     if (q >= N) return;
     const float lam = lam_p[0];
     const float w = wgt[q];
+    float v;
     if (w == 0.f) {
-        out[q] = img[q];
-        return;
+        v = img[q];
+    } else {
+        float num = img[q] * lam;
+        num = num + sum[q];
+        v = num / (lam + w);
     }
-    float num = img[q] * lam;
-    num = num + sum[q];
-    out[q] = num / (lam + w);
+    out[q] = SPECKLE ? exp2f(v) : v;
 }
 
 }  // namespace
 
-int launch_rebuild(ksvd_ctx *ctx, float sigma2)
+int launch_rebuild(ksvd_ctx *ctx, float sigma2, bool speckle)
 {
     const int pw = ctx->prm.patch_w, ph = ctx->prm.patch_h;
     const int sw = ctx->prm.slide_w, sh = ctx->prm.slide_h;
@@ -326,7 +332,8 @@ int launch_rebuild(ksvd_ctx *ctx, float sigma2)
     }
     hipLaunchKernelGGL(rebuild_lambda_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->rpart.ptr,
                        nparts, sigma2, N, (float *)ctx->rlam.ptr, (double *)((float *)ctx->rlam.ptr + 2));
-    hipLaunchKernelGGL(rebuild_blend_kernel, dim3(nblk), dim3(256), 0, ctx->stream, (const float *)ctx->img.ptr,
+    auto blend = speckle ? rebuild_blend_kernel<true> : rebuild_blend_kernel<false>;
+    hipLaunchKernelGGL(blend, dim3(nblk), dim3(256), 0, ctx->stream, (const float *)ctx->img.ptr,
                        (const float *)ctx->rsum.ptr, (const float *)ctx->rweight.ptr, (const float *)ctx->rlam.ptr, N,
                        (float *)ctx->rout.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());

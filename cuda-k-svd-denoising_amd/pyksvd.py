@@ -91,6 +91,7 @@ SIGNATURES = {
     "ksvd_run_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
     "ksvd_run_final_code": (ctypes.c_int, [_P]),
     "ksvd_rebuild": (ctypes.c_int, [_P, ctypes.c_float, _FP]),
+    "ksvd_rebuild_speckle": (ctypes.c_int, [_P, ctypes.c_float, _FP]),
     "ksvd_get_rebuild_parts": (ctypes.c_int, [_P, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_double)]),
     "ksvd_rebuild_device_ptr": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "ksvd_get_dims": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64),
@@ -250,9 +251,12 @@ class KSvd:
     def run(self):
         self._check(lib().ksvd_run(self._h))
 
-    def rebuild(self, sigma2, copy=True):
+    def rebuild(self, sigma2, copy=True, speckle=False):
+        """speckle: the frame is log-domain and the output returns by 2^v on
+        the device (ksvd_rebuild_speckle)."""
         out = np.empty(self.m * self.n, dtype=np.float32) if copy else None
-        self._check(lib().ksvd_rebuild(self._h, ctypes.c_float(sigma2), _fp(out) if copy else None))
+        fn = lib().ksvd_rebuild_speckle if speckle else lib().ksvd_rebuild
+        self._check(fn(self._h, ctypes.c_float(sigma2), _fp(out) if copy else None))
         return out.reshape(self.n, self.m) if copy else None
 
     def rebuild_picture(self, sigma2):

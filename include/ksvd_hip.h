@@ -202,6 +202,13 @@ int ksvd_run_final_code(ksvd_ctx *ctx);
  * its final transpose.  Pixels no patch covers keep their input value. */
 int ksvd_rebuild(ksvd_ctx *ctx, float sigma2, float *out);
 
+/* ksvd_rebuild of a log-domain (speckle) frame: the blend's output leaves
+ * the log domain on the device, v -> 2^v, as createImage does on the host
+ * when speckle is set (cuda_svd_denoiser.cu:483-484, myExp
+ * denoisingLib.h:135-142); within 2 ulp of ksvd_speckle_exp on the output of
+ * ksvd_rebuild.  ksvd_get_rebuild_parts still returns the log-domain sums. */
+int ksvd_rebuild_speckle(ksvd_ctx *ctx, float sigma2, float *out);
+
 /* After ksvd_rebuild: the overlap-add sums and weights (m*n each, host) and
  * lambda; any pointer may be NULL. */
 int ksvd_get_rebuild_parts(ksvd_ctx *ctx, float *sum, float *weight, float *lambda, double *dsum);

@@ -350,6 +350,39 @@ def test_rebuild_paths_bitexact(kl, orc, case):
     ks.close()
 
 
+@pytest.mark.parametrize("path", ["tiles", "pixels"])
+def test_rebuild_speckle_on_device(kl, orc, path):
+    """ksvd_rebuild_speckle: the log-domain output leaves the log domain on
+    the device.  Tolerance: 2 ulp against the host's powf(2, v)
+    (ksvd_speckle_exp, the reference's myExp) on ksvd_rebuild's bits, which
+    equal the oracle's; the sums and weights are the plain rebuild's."""
+    W, H = 96, 80
+    frame = speckle_frame(kl, W, H)
+    slide = (2, 2) if path == "tiles" else (1, 1)  # slide 1: more covering patches a tile than threads
+    ks = kl.KSvd(patch=(6, 6), slide=slide, atoms=64, omp_iter=5, ksvd_iter=2)
+    ks.set_frame(frame, W, H)
+    ks.extract()
+    ks.init_dictionary()
+    ks.run()
+    s2 = 0.05
+    plain = ks.rebuild(s2)
+    sg, wg, lamg, _ = ks.rebuild_parts()
+    idx, coef, cnt = ks.codes()
+    oo = orc.rebuild(idx, coef, cnt, ks.dictionary(), frame, W, H, 6, 6, slide[0], slide[1], s2)[0]
+    np.testing.assert_array_equal(plain.ravel(), oo.ravel())
+    dev = ks.rebuild(s2, speckle=True)
+    host = kl.speckle_exp(plain)
+    assert (host > 0).all() and (dev > 0).all()
+    ulp = np.abs(dev.ravel().view(np.int32).astype(np.int64) - host.ravel().view(np.int32).astype(np.int64))
+    print(f"\nspeckle rebuild on the device: max {ulp.max()} ulp from powf(2, v), {np.count_nonzero(ulp)} of {ulp.size} differ")
+    assert ulp.max() <= 2
+    sg2, wg2, lamg2, _ = ks.rebuild_parts()
+    np.testing.assert_array_equal(sg2, sg)
+    np.testing.assert_array_equal(wg2, wg)
+    assert lamg2 == lamg
+    ks.close()
+
+
 def test_two_contexts_interleaved(kl):
     """Two contexts of different geometry on one device, their pursuits,
     sweeps and rebuilds interleaved, give what each gives alone (kernels whose

@@ -134,6 +134,24 @@ def test_speckle_transforms(kl):
     assert kl.speckle_exp(np.zeros(1, np.float32))[0] == 1.0  # a zero pixel comes back as 1
 
 
+def test_speckle_transforms_threaded_same_bits(kl):
+    """Buffers of a million elements or more are split over host threads:
+    the same bits as the single-thread loop over pieces below that size."""
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.0, 300.0, 5_000_003).astype(np.float32)
+    x[::1000] = 0.0
+    lg = kl.speckle_log(x)
+    ex = kl.speckle_exp(lg)
+    piece = 700_001
+    lg1 = np.concatenate([kl.speckle_log(x[i:i + piece]) for i in range(0, x.size, piece)])
+    ex1 = np.concatenate([kl.speckle_exp(lg[i:i + piece]) for i in range(0, x.size, piece)])
+    np.testing.assert_array_equal(lg, lg1)
+    np.testing.assert_array_equal(ex, ex1)
+    assert (lg[::1000] == 0).all()
+    nz = x != 0
+    assert np.max(np.abs(ex[nz] - x[nz]) / x[nz]) < 1e-6
+
+
 def test_normalize_matches_minmax(kl):
     x = np.array([[-3.0, 1.0], [5.0, 2.0]], np.float32)
     y = kl.normalize(x)
