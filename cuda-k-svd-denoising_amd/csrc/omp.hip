@@ -451,52 +451,94 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 // triangular solves, ||r||^2 = ||y||^2 - b.x with the explicit residual only
 // near zero, duplicate and degenerate stops.
 
-// C0[p][a] = d_a . y_p on v_mfma_f32_16x16x4_f32.  Workgroup tile: 128 atoms x
-// 64 patches, four wavefronts of 64 atoms x 32 patches (eight 16 x 16 blocks);
-// the reduction runs over the patch dimension in chunks of 16 staged in LDS
-// (zero past dim, K and P).
-constexpr int CG_TA = 128, CG_TP = 64, CG_KC = 16;
+// C0[p][a] = d_a . y_p on v_mfma_f32_16x16x4_f32.  Workgroup tile: 256
+// atoms x 64 patches, so every patch is read from HBM once (D, at most 512 KB,
+// stays in L2); wavefront w takes atoms [64w, 64w + 64) of the tile, sixteen
+// 16 x 16 blocks.  The reduction runs over the patch dimension in chunks of
+// 16 staged in LDS (zero past dim, K and P), double-buffered: the next
+// chunk's loads are in flight while the MFMAs consume this one, and one
+// barrier per chunk separates them.  Lane (col, g) holds elements
+// [4g, 4g + 4) of a chunk row, read with one 16-byte LDS load; MFMA step s
+// takes element 4g + s of A and B alike, so the four steps sum the chunk.
+// VEC: dim and K are multiples of 4 (16-byte global loads and stores).
+constexpr int CG_TA = 256, CG_TP = 64, CG_KC = 16, CG_LD = CG_KC + 4;
 
+template <bool VEC>
+__device__ __forceinline__ f4 cg_load4(const float *__restrict__ row, int e, int dim, bool ok)
+{
+    f4 v = f4{0.f, 0.f, 0.f, 0.f};
+    if (VEC) {
+        if (ok && e < dim) v = *(const f4 *)(row + e);
+    } else if (ok) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (e + i < dim) v[i] = row[e + i];
+    }
+    return v;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict__ Y, int dim, int64_t P,
                                                         const float *__restrict__ D, int K, float *__restrict__ C0)
 {
-    __shared__ float Ds[CG_TA][CG_KC + 1];
-    __shared__ float Ys[CG_TP][CG_KC + 1];
+    __shared__ __attribute__((aligned(16))) float As[2][CG_TA][CG_LD];
+    __shared__ __attribute__((aligned(16))) float Bs[2][CG_TP][CG_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wa = wave & 1, wp = wave >> 1;
     const int col = lane & 15, g = lane >> 4;
     const int a0 = blockIdx.y * CG_TA;
     const int64_t p0 = (int64_t)blockIdx.x * CG_TP;
-    f4 acc[4][2];
+    // this thread's loads: rows tid / 4 + 64 j of the atom tile, row tid / 4
+    // of the patch tile, elements [4 (tid & 3), +4) of each chunk
+    const int lr = tid >> 2, lc = 4 * (tid & 3);
+    const float *arow[4];
+    bool aok[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int at = a0 + lr + 64 * j;
+        aok[j] = at < K;
+        arow[j] = D + (int64_t)(aok[j] ? at : 0) * dim;
+    }
+    const bool bok = p0 + lr < P;
+    const float *brow = Y + (bok ? p0 + lr : 0) * dim;
+    f4 acc[4][4];
 #pragma unroll
     for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-        for (int pb = 0; pb < 2; pb++) acc[ab][pb] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int e0 = 0; e0 < dim; e0 += CG_KC) {
-        for (int q = tid; q < CG_TA * CG_KC; q += 256) {
-            const int r = q / CG_KC, c = q - r * CG_KC;
-            const int at = a0 + r, e = e0 + c;
-            Ds[r][c] = (at < K && e < dim) ? D[(int64_t)at * dim + e] : 0.f;
+        for (int pb = 0; pb < 4; pb++) acc[ab][pb] = f4{0.f, 0.f, 0.f, 0.f};
+    f4 pa[4], pbv;
+#pragma unroll
+    for (int j = 0; j < 4; j++) pa[j] = cg_load4<VEC>(arow[j], lc, dim, aok[j]);
+    pbv = cg_load4<VEC>(brow, lc, dim, bok);
+#pragma unroll
+    for (int j = 0; j < 4; j++) *(f4 *)&As[0][lr + 64 * j][lc] = pa[j];
+    *(f4 *)&Bs[0][lr][lc] = pbv;
+    __syncthreads();
+    const int nch = (dim + CG_KC - 1) / CG_KC;
+    for (int c = 0; c < nch; c++) {
+        const int cur = c & 1;
+        const bool more = c + 1 < nch;
+        if (more) {
+            const int e = (c + 1) * CG_KC + lc;
+#pragma unroll
+            for (int j = 0; j < 4; j++) pa[j] = cg_load4<VEC>(arow[j], e, dim, aok[j]);
+            pbv = cg_load4<VEC>(brow, e, dim, bok);
         }
-        for (int q = tid; q < CG_TP * CG_KC; q += 256) {
-            const int r = q / CG_KC, c = q - r * CG_KC;
-            const int64_t pp = p0 + r;
-            const int e = e0 + c;
-            Ys[r][c] = (pp < P && e < dim) ? Y[pp * dim + e] : 0.f;
-        }
-        __syncthreads();
+        f4 av[4], bv[4];
 #pragma unroll
-        for (int kk = 0; kk < CG_KC; kk += 4) {
-            float av[4], bv[2];
+        for (int ab = 0; ab < 4; ab++) av[ab] = *(const f4 *)&As[cur][wave * 64 + ab * 16 + col][4 * g];
 #pragma unroll
-            for (int ab = 0; ab < 4; ab++) av[ab] = Ds[wa * 64 + ab * 16 + col][kk + g];
+        for (int pb = 0; pb < 4; pb++) bv[pb] = *(const f4 *)&Bs[cur][pb * 16 + col][4 * g];
 #pragma unroll
-            for (int pb = 0; pb < 2; pb++) bv[pb] = Ys[wp * 32 + pb * 16 + col][kk + g];
+        for (int st = 0; st < 4; st++)
 #pragma unroll
             for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-                for (int pb = 0; pb < 2; pb++)
-                    acc[ab][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ab], bv[pb], acc[ab][pb], 0, 0, 0);
+                for (int pb = 0; pb < 4; pb++)
+                    acc[ab][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ab][st], bv[pb][st], acc[ab][pb], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) *(f4 *)&As[cur ^ 1][lr + 64 * j][lc] = pa[j];
+            *(f4 *)&Bs[cur ^ 1][lr][lc] = pbv;
         }
         __syncthreads();
     }
@@ -504,13 +546,17 @@ __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict_
 #pragma unroll
     for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-        for (int pb = 0; pb < 2; pb++) {
-            const int at = a0 + wa * 64 + ab * 16 + 4 * g;
-            const int64_t pp = p0 + wp * 32 + pb * 16 + col;
-            if (pp < P)
+        for (int pb = 0; pb < 4; pb++) {
+            const int at = a0 + wave * 64 + ab * 16 + 4 * g;
+            const int64_t pp = p0 + pb * 16 + col;
+            if (pp >= P) continue;
+            if (VEC && at + 3 < K) {
+                *(f4 *)(C0 + pp * K + at) = acc[ab][pb];
+            } else {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     if (at + i < K) C0[pp * K + at + i] = acc[ab][pb][i];
+            }
         }
 }
 
@@ -800,7 +846,10 @@ int launch_batch(ksvd_ctx *ctx)
     float *C0 = (float *)ctx->c0.ptr;
     for (int64_t p0 = 0; p0 < P; p0 += chunk) {
         const int64_t np = std::min(chunk, P - p0);
-        hipLaunchKernelGGL(corr_gemm_kernel, dim3((unsigned)((np + CG_TP - 1) / CG_TP), (unsigned)((K + CG_TA - 1) / CG_TA)),
+        // 16-byte loads and stores when rows and C0's rows stay 16-byte aligned
+        const bool vec = ctx->dim % 4 == 0 && K % 4 == 0;
+        auto gemm = vec ? corr_gemm_kernel<true> : corr_gemm_kernel<false>;
+        hipLaunchKernelGGL(gemm, dim3((unsigned)((np + CG_TP - 1) / CG_TP), (unsigned)((K + CG_TA - 1) / CG_TA)),
                            dim3(256), 0, ctx->stream, (const float *)ctx->Y.ptr + p0 * ctx->dim, ctx->dim, np,
                            (const float *)ctx->D.ptr, K, C0);
         KSVD_HIP_CHECK(ctx, hipGetLastError());
