@@ -110,7 +110,10 @@ __device__ inline RbTile rb_tile(int t, int ntb, int m, int n, int Pr, int Pc, i
 // KM: bound on the support size (kmax <= KM), and the stride of a slot's
 // support in LDS.  nslot: bound on a tile's
 // covering patches, at most RB_TA * RB_TB (one per thread, checked at launch).
-template <int KM>
+// DL: the dictionary in LDS; without it (dictionaries above the LDS, 16 x 12
+// and 22 x 22 patches) the dictionary is read from global memory, where it
+// stays in L2, and only the tile's supports are in LDS.
+template <int KM, bool DL>
 __global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
     const int *__restrict__ idx, const float *__restrict__ coef, const int *__restrict__ cnt,
     const float *__restrict__ D, int dim, int K, int kmax, int m, int n, int Pr, int Pc, int rext, int cext,
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
     extern __shared__ __attribute__((aligned(16))) float Ds[];  // K x dim, then the tile's supports
     __shared__ double red[RB_TA * RB_TB / 64];
     const int tid = threadIdx.x;
-    const int kd = K * dim;
+    const int kd = DL ? K * dim : 0;
     float2 *sup = (float2 *)(Ds + ((kd + 3) & ~3));  // [slot][KM]: (atom * dim as bits, coefficient)
     int *sn = (int *)(sup + (size_t)nslot * KM);  // [slot]: support size
     // this thread's covering patch of the next tile: its codes, loaded a tile ahead
@@ -142,7 +145,8 @@ __global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
         }
     };
     fetch(blockIdx.x);
-    if ((kd & 3) == 0) {
+    if (!DL) {
+    } else if ((kd & 3) == 0) {
         for (int i = tid; i < kd / 4; i += RB_TA * RB_TB) ((float4 *)Ds)[i] = ((const float4 *)D)[i];
     } else {
         for (int i = tid; i < kd; i += RB_TA * RB_TB) Ds[i] = D[i];
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
 #pragma unroll
                     for (int i = 0; i < KM; i++) u[i] = sj[i];
 #pragma unroll
-                    for (int i = 0; i < KM; i++) dv[i] = Ds[__float_as_int(u[i].x) + e];
+                    for (int i = 0; i < KM; i++) dv[i] = DL ? Ds[__float_as_int(u[i].x) + e] : D[__float_as_int(u[i].x) + e];
                     float v = 0.f;
 #pragma unroll
                     for (int i = 0; i < KM; i++)
@@ -224,12 +228,13 @@ __global__ __launch_bounds__(RB_TA * RB_TB) void rebuild_tiles_kernel(
     }
 }
 
-// LDS bytes of rebuild_tiles_kernel: the dictionary and nslot supports of
-// stride km.
-size_t rebuild_tiles_lds(int K, int dim, int km, int64_t nslot)
+// LDS bytes of rebuild_tiles_kernel: the dictionary (when it is held there)
+// and nslot supports of stride km.
+size_t rebuild_tiles_lds(int K, int dim, int km, int64_t nslot, bool dl)
 {
     const size_t slots = (size_t)nslot;
-    return sizeof(float) * (((size_t)K * dim + 3) & ~(size_t)3) + slots * km * sizeof(float2) + slots * sizeof(int);
+    const size_t kd = dl ? (size_t)K * dim : 0;
+    return sizeof(float) * ((kd + 3) & ~(size_t)3) + slots * km * sizeof(float2) + slots * sizeof(int);
 }
 
 __global__ __launch_bounds__(256) void rebuild_lambda_kernel(const double *__restrict__ part, int nparts,
@@ -299,24 +304,37 @@ int launch_rebuild(ksvd_ctx *ctx, float sigma2, bool speckle)
     const int nblk = (int)((N + 255) / 256);  // blend grid, and the capacity of the partials
     const int64_t slots = (int64_t)rb_span(RB_TA, rext, rstep) * rb_span(RB_TB, cext, cstep);
     const int km = ctx->kmax <= 4 ? 4 : ctx->kmax <= 5 ? 5 : ctx->kmax <= 8 ? 8 : 16;
-    const size_t lds = rebuild_tiles_lds(ctx->K, ctx->dim, km, slots);
+    const size_t lds_cap = 160 * 1024 - sizeof(double) * (RB_TA * RB_TB / 64);
+    const bool dl = rebuild_tiles_lds(ctx->K, ctx->dim, km, slots, true) <= lds_cap;
+    const size_t lds = rebuild_tiles_lds(ctx->K, ctx->dim, km, slots, dl);
     int nparts = nblk;
-    if (ctx->kmax <= 16 && slots <= RB_TA * RB_TB && lds <= 160 * 1024 - sizeof(double) * (RB_TA * RB_TB / 64)) {
+    if (ctx->kmax <= 16 && slots <= RB_TA * RB_TB && lds <= lds_cap) {
         const int ntb = (ctx->m + RB_TB - 1) / RB_TB;
         const int64_t nt64 = (int64_t)((ctx->n + RB_TA - 1) / RB_TA) * ntb;
         if (nt64 > INT32_MAX) return ksvd_fail(ctx, KSVD_ERR_UNSUPPORTED, "rebuild: too many tiles");
         const int ntiles = (int)nt64;
-        int grid = ctx->num_cus;
+        // with the dictionary in L2, two workgroups a CU hide its loads
+        int grid = dl ? ctx->num_cus : 2 * ctx->num_cus;
         if (grid > ntiles) grid = ntiles;
         if (grid > nblk) grid = nblk;
         nparts = grid;
-        auto kern = km == 4 ? rebuild_tiles_kernel<4>
-                  : km == 5 ? rebuild_tiles_kernel<5>
-                  : km == 8 ? rebuild_tiles_kernel<8>
-                            : rebuild_tiles_kernel<16>;
-        if (ctx->rebuild_lds_set != (size_t)km) {  // one host call per context, not per rebuild
+        typedef void (*RbFn)(const int *, const float *, const int *, const float *, int, int, int, int, int, int,
+                             int, int, int, int, int, int, int, int, const float *, float *, float *, double *);
+        RbFn kern;
+        if (dl)
+            kern = km == 4 ? rebuild_tiles_kernel<4, true>
+                 : km == 5 ? rebuild_tiles_kernel<5, true>
+                 : km == 8 ? rebuild_tiles_kernel<8, true>
+                           : rebuild_tiles_kernel<16, true>;
+        else
+            kern = km == 4 ? rebuild_tiles_kernel<4, false>
+                 : km == 5 ? rebuild_tiles_kernel<5, false>
+                 : km == 8 ? rebuild_tiles_kernel<8, false>
+                           : rebuild_tiles_kernel<16, false>;
+        const size_t key = (size_t)km * 2 + (dl ? 1 : 0);
+        if (ctx->rebuild_lds_set != key) {  // one host call per context and variant, not per rebuild
             KSVD_HIP_CHECK(ctx, allow_max_lds((const void *)kern));
-            ctx->rebuild_lds_set = (size_t)km;
+            ctx->rebuild_lds_set = key;
         }
         hipLaunchKernelGGL(kern, dim3(grid), dim3(RB_TA * RB_TB), lds, ctx->stream,
                            (const int *)ctx->idx.ptr, (const float *)ctx->coef.ptr, (const int *)ctx->cnt.ptr,

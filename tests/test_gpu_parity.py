@@ -308,7 +308,8 @@ def test_rebuild_bitexact_sums(kl, orc):
 
 # (m, n, patch, slide, atoms, k): the tiled rebuild (dictionary and the
 # sorted supports of a 16 x 64-pixel tile's covering patches in LDS) runs when
-# they fit; otherwise the per-pixel rebuild does.
+# they fit, with the dictionary read from global memory when only the
+# supports fit; otherwise the per-pixel rebuild does.
 REBUILD_CASES = [
     (150, 97, (7, 5), (2, 3), 64, 3),     # tiled, supports of stride 4, ragged tiles
     (200, 131, (6, 6), (2, 2), 128, 8),   # tiled, stride 8, 385 covering patches a tile
@@ -316,6 +317,9 @@ REBUILD_CASES = [
     (120, 90, (8, 8), (2, 2), 64, 12),    # tiled, stride 16
     (96, 80, (8, 8), (1, 1), 64, 5),      # per-pixel: 1,728 covering patches a tile > 1,024 threads
     (130, 70, (12, 12), (3, 3), 256, 12), # per-pixel: supports of stride 16 overflow LDS beside D
+    (200, 150, (16, 12), (3, 3), 256, 5), # tiled, D (192 x 256) above the LDS: read from global memory
+    (170, 120, (16, 12), (3, 3), 256, 8), # the same, stride 8
+    (160, 130, (22, 22), (3, 3), 256, 5), # the same at 22 x 22, 377 covering patches a tile
 ]
 
 
