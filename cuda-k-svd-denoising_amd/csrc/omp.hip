@@ -461,6 +461,10 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 // [4g, 4g + 4) of a chunk row, read with one 16-byte LDS load; MFMA step s
 // takes element 4g + s of A and B alike, so the four steps sum the chunk.
 // VEC: dim and K are multiples of 4 (16-byte global loads and stores).
+// TP: patches per tile, 64 or 128 (twice the MFMA work per LDS chunk of D,
+// at two thirds of the occupancy); both give the same bits, since every
+// correlation is summed over the same chunks and steps in the same order,
+// and the first call for a shape times both and keeps the faster.
 constexpr int CG_TA = 256, CG_TP = 64, CG_KC = 16, CG_LD = CG_KC + 4;
 
 template <bool VEC>
@@ -477,18 +481,20 @@ __device__ __forceinline__ f4 cg_load4(const float *__restrict__ row, int e, int
     return v;
 }
 
-template <bool VEC>
+template <bool VEC, int TP>
 __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict__ Y, int dim, int64_t P,
                                                         const float *__restrict__ D, int K, float *__restrict__ C0)
 {
+    constexpr int NB = TP / 64;  // patch rows this thread loads per chunk
+    constexpr int PB = TP / 16;  // 16-patch blocks of each wavefront
     __shared__ __attribute__((aligned(16))) float As[2][CG_TA][CG_LD];
-    __shared__ __attribute__((aligned(16))) float Bs[2][CG_TP][CG_LD];
+    __shared__ __attribute__((aligned(16))) float Bs[2][TP][CG_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, g = lane >> 4;
     const int a0 = blockIdx.y * CG_TA;
-    const int64_t p0 = (int64_t)blockIdx.x * CG_TP;
-    // this thread's loads: rows tid / 4 + 64 j of the atom tile, row tid / 4
-    // of the patch tile, elements [4 (tid & 3), +4) of each chunk
+    const int64_t p0 = (int64_t)blockIdx.x * TP;
+    // this thread's loads: rows tid / 4 + 64 j of the atom tile and of the
+    // patch tile, elements [4 (tid & 3), +4) of each chunk
     const int lr = tid >> 2, lc = 4 * (tid & 3);
     const float *arow[4];
     bool aok[4];
@@ -498,20 +504,27 @@ __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict_
         aok[j] = at < K;
         arow[j] = D + (int64_t)(aok[j] ? at : 0) * dim;
     }
-    const bool bok = p0 + lr < P;
-    const float *brow = Y + (bok ? p0 + lr : 0) * dim;
-    f4 acc[4][4];
+    const float *brow[NB];
+    bool bok[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        bok[j] = p0 + lr + 64 * j < P;
+        brow[j] = Y + (bok[j] ? p0 + lr + 64 * j : 0) * dim;
+    }
+    f4 acc[4][PB];
 #pragma unroll
     for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-        for (int pb = 0; pb < 4; pb++) acc[ab][pb] = f4{0.f, 0.f, 0.f, 0.f};
-    f4 pa[4], pbv;
+        for (int pb = 0; pb < PB; pb++) acc[ab][pb] = f4{0.f, 0.f, 0.f, 0.f};
+    f4 pa[4], pbv[NB];
 #pragma unroll
     for (int j = 0; j < 4; j++) pa[j] = cg_load4<VEC>(arow[j], lc, dim, aok[j]);
-    pbv = cg_load4<VEC>(brow, lc, dim, bok);
+#pragma unroll
+    for (int j = 0; j < NB; j++) pbv[j] = cg_load4<VEC>(brow[j], lc, dim, bok[j]);
 #pragma unroll
     for (int j = 0; j < 4; j++) *(f4 *)&As[0][lr + 64 * j][lc] = pa[j];
-    *(f4 *)&Bs[0][lr][lc] = pbv;
+#pragma unroll
+    for (int j = 0; j < NB; j++) *(f4 *)&Bs[0][lr + 64 * j][lc] = pbv[j];
     __syncthreads();
     const int nch = (dim + CG_KC - 1) / CG_KC;
     for (int c = 0; c < nch; c++) {
@@ -521,24 +534,26 @@ __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict_
             const int e = (c + 1) * CG_KC + lc;
 #pragma unroll
             for (int j = 0; j < 4; j++) pa[j] = cg_load4<VEC>(arow[j], e, dim, aok[j]);
-            pbv = cg_load4<VEC>(brow, e, dim, bok);
+#pragma unroll
+            for (int j = 0; j < NB; j++) pbv[j] = cg_load4<VEC>(brow[j], e, dim, bok[j]);
         }
-        f4 av[4], bv[4];
+        f4 av[4], bv[PB];
 #pragma unroll
         for (int ab = 0; ab < 4; ab++) av[ab] = *(const f4 *)&As[cur][wave * 64 + ab * 16 + col][4 * g];
 #pragma unroll
-        for (int pb = 0; pb < 4; pb++) bv[pb] = *(const f4 *)&Bs[cur][pb * 16 + col][4 * g];
+        for (int pb = 0; pb < PB; pb++) bv[pb] = *(const f4 *)&Bs[cur][pb * 16 + col][4 * g];
 #pragma unroll
         for (int st = 0; st < 4; st++)
 #pragma unroll
             for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-                for (int pb = 0; pb < 4; pb++)
+                for (int pb = 0; pb < PB; pb++)
                     acc[ab][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ab][st], bv[pb][st], acc[ab][pb], 0, 0, 0);
         if (more) {
 #pragma unroll
             for (int j = 0; j < 4; j++) *(f4 *)&As[cur ^ 1][lr + 64 * j][lc] = pa[j];
-            *(f4 *)&Bs[cur ^ 1][lr][lc] = pbv;
+#pragma unroll
+            for (int j = 0; j < NB; j++) *(f4 *)&Bs[cur ^ 1][lr + 64 * j][lc] = pbv[j];
         }
         __syncthreads();
     }
@@ -546,7 +561,7 @@ __global__ __launch_bounds__(256) void corr_gemm_kernel(const float *__restrict_
 #pragma unroll
     for (int ab = 0; ab < 4; ab++)
 #pragma unroll
-        for (int pb = 0; pb < 4; pb++) {
+        for (int pb = 0; pb < PB; pb++) {
             const int at = a0 + wave * 64 + ab * 16 + 4 * g;
             const int64_t pp = p0 + pb * 16 + col;
             if (pp >= P) continue;
@@ -832,6 +847,61 @@ static int launch_tiles_gc_any(ksvd_ctx *ctx, int64_t p0, int64_t np, const floa
                         : launch_tiles_gc<124, 8>(ctx, p0, np, C0);
 }
 
+// The step-0 GEMM over np patches at Yc into C0.  16-byte loads and stores
+// when the rows of Y, D and C0 stay 16-byte aligned.  The patch tile (64 or
+// 128, the same bits) is chosen per (dim, K, vec) by timing both on the
+// first call for that shape in the process (KSVD_GEMM_TP=64 or 128 forces
+// one); a stream under capture takes 64 without timing.
+std::mutex cg_mu;
+int cg_pick[2][KSVD_MAX_DIM + 1][2];  // [vec][dim][K <= 256]: 0 unknown, else the tile
+
+int launch_corr_gemm(ksvd_ctx *ctx, const float *Yc, int64_t np, float *C0)
+{
+    const int K = ctx->K, dim = ctx->dim;
+    const bool vec = dim % 4 == 0 && K % 4 == 0;
+    auto grid = [&](int tp) { return dim3((unsigned)((np + tp - 1) / tp), (unsigned)((K + CG_TA - 1) / CG_TA)); };
+    auto run = [&](int tp) {
+        auto kern = tp == 128 ? (vec ? corr_gemm_kernel<true, 128> : corr_gemm_kernel<false, 128>)
+                              : (vec ? corr_gemm_kernel<true, 64> : corr_gemm_kernel<false, 64>);
+        hipLaunchKernelGGL(kern, grid(tp), dim3(256), 0, ctx->stream, Yc, dim, np, (const float *)ctx->D.ptr, K, C0);
+    };
+    int tp = 0;
+    const char *ev = getenv("KSVD_GEMM_TP");
+    if (ev && (atoi(ev) == 64 || atoi(ev) == 128)) tp = atoi(ev);
+    const bool known_shape = dim <= KSVD_MAX_DIM;
+    if (!tp && known_shape) {
+        std::lock_guard<std::mutex> lock(cg_mu);
+        tp = cg_pick[vec][dim][K > 256];
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KSVD_HIP_CHECK(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (!tp && (cs != hipStreamCaptureStatusNone || !known_shape || np < 16 * 1024)) tp = 64;
+    if (tp) {
+        run(tp);
+        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        return KSVD_OK;
+    }
+    // first call for this shape: both tiles, timed with events (each run
+    // writes the same C0), the faster kept
+    hipEvent_t ev3[3];
+    for (auto &e : ev3) KSVD_HIP_CHECK(ctx, hipEventCreate(&e));
+    float ms[2] = {0.f, 0.f};
+    hipError_t he = hipEventRecord(ev3[0], ctx->stream);
+    run(64);
+    if (he == hipSuccess) he = hipEventRecord(ev3[1], ctx->stream);
+    run(128);
+    if (he == hipSuccess) he = hipEventRecord(ev3[2], ctx->stream);
+    if (he == hipSuccess) he = hipGetLastError();
+    if (he == hipSuccess) he = hipEventSynchronize(ev3[2]);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms[0], ev3[0], ev3[1]);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms[1], ev3[1], ev3[2]);
+    for (auto &e : ev3) (void)hipEventDestroy(e);
+    KSVD_HIP_CHECK(ctx, he);
+    std::lock_guard<std::mutex> lock(cg_mu);
+    cg_pick[vec][dim][K > 256] = ms[1] < ms[0] ? 128 : 64;
+    return KSVD_OK;
+}
+
 // The GEMM-correlation pursuit over every patch, in chunks whose C0 stays
 // under 128 MB.
 int launch_batch(ksvd_ctx *ctx)
@@ -839,20 +909,14 @@ int launch_batch(ksvd_ctx *ctx)
     const int K = ctx->K;
     const int64_t P = ctx->P;
     int64_t chunk = ((int64_t)32 << 20) / K;
-    chunk = std::max<int64_t>(CG_TP, chunk / CG_TP * CG_TP);
+    chunk = std::max<int64_t>(128, chunk / 128 * 128);  // whole tiles of either shape
     if (chunk > P) chunk = P;
     int rc = ksvd_reserve(ctx, ctx->c0, sizeof(float) * (size_t)chunk * K);
     if (rc) return rc;
     float *C0 = (float *)ctx->c0.ptr;
     for (int64_t p0 = 0; p0 < P; p0 += chunk) {
         const int64_t np = std::min(chunk, P - p0);
-        // 16-byte loads and stores when rows and C0's rows stay 16-byte aligned
-        const bool vec = ctx->dim % 4 == 0 && K % 4 == 0;
-        auto gemm = vec ? corr_gemm_kernel<true> : corr_gemm_kernel<false>;
-        hipLaunchKernelGGL(gemm, dim3((unsigned)((np + CG_TP - 1) / CG_TP), (unsigned)((K + CG_TA - 1) / CG_TA)),
-                           dim3(256), 0, ctx->stream, (const float *)ctx->Y.ptr + p0 * ctx->dim, ctx->dim, np,
-                           (const float *)ctx->D.ptr, K, C0);
-        KSVD_HIP_CHECK(ctx, hipGetLastError());
+        if ((rc = launch_corr_gemm(ctx, (const float *)ctx->Y.ptr + p0 * ctx->dim, np, C0))) return rc;
         if (tiles_gc_ok(ctx))
             rc = launch_tiles_gc_any(ctx, p0, np, C0);
         else

@@ -63,6 +63,23 @@ def test_large_patch_omp(kl, orc, cfg, monkeypatch):
     assert err64 <= 1e-4
 
 
+@pytest.mark.parametrize("cfg", CASES + [dict(CASES[0], atoms=97, seed=45)], ids=IDS + ["16x12_97atoms"])
+def test_gemm_tiles_same_bits(kl, cfg, monkeypatch):
+    """The step-0 GEMM's two patch tiles (64 and 128 patches, forced with
+    KSVD_GEMM_TP; 97 atoms takes the element-wise loads) give the same codes
+    bit for bit, so the first-use choice between them never changes a
+    result."""
+    res = []
+    for tp in ("64", "128"):
+        monkeypatch.setenv("KSVD_GEMM_TP", tp)
+        clean, noisy, ks = context(kl, cfg)
+        ks.omp()
+        res.append(ks.codes())
+        ks.close()
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("path", ["default", "kl"])
 @pytest.mark.parametrize("cfg", CASES, ids=IDS)
 def test_large_patch_sweep(kl, orc, cfg, path, monkeypatch):
