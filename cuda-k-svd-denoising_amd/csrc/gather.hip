@@ -52,7 +52,11 @@ __global__ void init_dict_kernel(const float *__restrict__ Y, int dim, int64_t o
 }
 
 // G = D^T D (K x K).  One thread per Gram entry; element order, fused
-// multiply-adds.
+// multiply-adds.  The elements are loaded 16 at a time ahead of their
+// products (VEC: as 16-byte loads, for rows that stay 16-byte aligned), so
+// that a thread waits for one round of loads per 16 products instead of one
+// per product; the sum is the same chain, element by element.
+template <bool VEC>
 __global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, float *__restrict__ G)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -62,11 +66,31 @@ __global__ void dict_gram_kernel(const float *__restrict__ D, int dim, int K, fl
     // accumulated in FP64 and rounded once: the pursuit's support Grams have
     // condition numbers near 1e3 on a dictionary cut from overlapping
     // patches, where an FP32 running sum's error would reach the codes
+    constexpr int GB = 16;
     double s = 0.0;
-    for (int e = 0; e < dim; e++) s = fma((double)da[e], (double)db[e], s);
+    int e = 0;
+    for (; e + GB <= dim; e += GB) {
+        float xa[GB], xb[GB];
+        if (VEC) {
+#pragma unroll
+            for (int i = 0; i < GB; i += 4) {
+                const float4 va = *(const float4 *)(da + e + i), vb = *(const float4 *)(db + e + i);
+                xa[i] = va.x, xa[i + 1] = va.y, xa[i + 2] = va.z, xa[i + 3] = va.w;
+                xb[i] = vb.x, xb[i + 1] = vb.y, xb[i + 2] = vb.z, xb[i + 3] = vb.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                xa[i] = da[e + i];
+                xb[i] = db[e + i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < GB; i++) s = fma((double)xa[i], (double)xb[i], s);
+    }
+    for (; e < dim; e++) s = fma((double)da[e], (double)db[e], s);
     G[q] = (float)s;
 }
-
 
 }  // namespace
 
@@ -95,10 +119,14 @@ int launch_init_dict(ksvd_ctx *ctx)
     return KSVD_OK;
 }
 
+// 16-byte loads of the Gram's rows: dim a multiple of 4 on a 16-byte aligned D.
+static bool gram_vec(const float *D, int dim) { return dim % 4 == 0 && (uintptr_t)D % 16 == 0; }
+
 int launch_dict_gram(ksvd_ctx *ctx)
 {
     const int64_t kk = (int64_t)ctx->K * ctx->K;
-    hipLaunchKernelGGL(dict_gram_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, ctx->stream,
+    auto kern = gram_vec((const float *)ctx->D.ptr, ctx->dim) ? dict_gram_kernel<true> : dict_gram_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, ctx->stream,
                        (const float *)ctx->D.ptr, ctx->dim, ctx->K, (float *)ctx->G.ptr);
     KSVD_HIP_CHECK(ctx, hipGetLastError());
     ctx->gram_valid = true;
@@ -110,6 +138,7 @@ int launch_dict_gram(ksvd_ctx *ctx)
 hipError_t launch_dict_gram_raw(const float *D, int dim, int K, float *G, hipStream_t stream)
 {
     const int64_t kk = (int64_t)K * K;
-    hipLaunchKernelGGL(dict_gram_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, stream, D, dim, K, G);
+    auto kern = gram_vec(D, dim) ? dict_gram_kernel<true> : dict_gram_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, stream, D, dim, K, G);
     return hipGetLastError();
 }
