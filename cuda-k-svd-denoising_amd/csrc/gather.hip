@@ -32,23 +32,50 @@ __global__ __launch_bounds__(256) void gather_patches_kernel(const float *__rest
     }
 }
 
-// One thread per atom.  The squared norm is accumulated the way the
-// reference's in-kernel thrust::transform_reduce(square, 0, plus<float>)
-// does it (cuda_svd_denoiser.cu:59): the int init value makes the running
-// sum an int, so each float partial sum is truncated toward zero.
-__global__ void init_dict_kernel(const float *__restrict__ Y, int dim, int64_t off, int K,
-                                 float *__restrict__ D)
+// One thread per atom for its norm, then the block's atoms divided and
+// stored together.  The squared norm is accumulated the way the reference's
+// in-kernel thrust::transform_reduce(square, 0, plus<float>) does it
+// (cuda_svd_denoiser.cu:59): the int init value makes the running sum an int,
+// so each float partial sum is truncated toward zero.  The elements are
+// loaded 16 at a time ahead of that chain; the division pass walks the
+// block's rows (consecutive patches, so one contiguous run of Y and of D)
+// row by row, each row's elements across the threads.
+__global__ __launch_bounds__(256) void init_dict_kernel(const float *__restrict__ Y, int dim, int64_t off, int K,
+                                                        float *__restrict__ D)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    const float *y = Y + (off + k) * (int64_t)dim;
-    int acc = 0;
-    for (int e = 0; e < dim; e++) {
-        const float sq = y[e] * y[e];
-        acc = (int)((float)acc + sq);
+    __shared__ float snorm[256];
+    const int k0 = blockIdx.x * 256;
+    const int k = k0 + threadIdx.x;
+    if (k < K) {
+        const float *y = Y + (off + k) * (int64_t)dim;
+        constexpr int GB = 16;
+        int acc = 0;
+        int e = 0;
+        for (; e + GB <= dim; e += GB) {
+            float v[GB];
+#pragma unroll
+            for (int i = 0; i < GB; i++) v[i] = y[e + i];
+#pragma unroll
+            for (int i = 0; i < GB; i++) {
+                const float sq = v[i] * v[i];
+                acc = (int)((float)acc + sq);
+            }
+        }
+        for (; e < dim; e++) {
+            const float sq = y[e] * y[e];
+            acc = (int)((float)acc + sq);
+        }
+        snorm[threadIdx.x] = sqrtf((float)acc);
     }
-    const float norm = sqrtf((float)acc);
-    for (int e = 0; e < dim; e++) D[(int64_t)k * dim + e] = y[e] / norm;
+    __syncthreads();
+    const int na = min(256, K - k0);
+    const float *yb = Y + (off + k0) * (int64_t)dim;
+    float *db = D + (int64_t)k0 * dim;
+#pragma unroll 4
+    for (int a = 0; a < na; a++) {
+        const float nr = snorm[a];
+        for (int e = threadIdx.x; e < dim; e += 256) db[(int64_t)a * dim + e] = yb[(int64_t)a * dim + e] / nr;
+    }
 }
 
 // G = D^T D (K x K).  One thread per Gram entry; element order, fused
