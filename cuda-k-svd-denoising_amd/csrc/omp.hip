@@ -465,7 +465,7 @@ __global__ __launch_bounds__(64 * W, 1) void omp_mfma_kernel(
 // at two thirds of the occupancy); both give the same bits, since every
 // correlation is summed over the same chunks and steps in the same order,
 // and the first call for a shape times both and keeps the faster.
-constexpr int CG_TA = 256, CG_TP = 64, CG_KC = 16, CG_LD = CG_KC + 4;
+constexpr int CG_TA = 256, CG_KC = 16, CG_LD = CG_KC + 4;
 
 template <bool VEC>
 __device__ __forceinline__ f4 cg_load4(const float *__restrict__ row, int e, int dim, bool ok)
